@@ -1,0 +1,187 @@
+"""Host-side interface of the MI355X hot path, mirroring the reference's Go
+wrappers and its LLB-r module composition.
+
+- `Demag` wraps the demag plan (MaxwellPlan + FFTPlanX of
+  src/hotspin-core/modules/maxwell_convolution.go and gpu/fftplanX.go).
+- `Solver` wraps the resident LLB-r solver; its keyword arguments follow the
+  reference's quantities (msat0T0, lex, B_ext, J, n, Tc, T, alpha = the
+  llbar/damping/nonconservative/00/local lambda, gamma_LL, Therm_seed,
+  cutoff_dt) in internal (z, y, x) order.
+
+Everything runs through libhotspin_amd.so; host arrays are numpy float64.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import HotspinError, check, lib
+
+EULER, HEUN, RK4, RK12, RK23 = 0, 1, 2, 3, 4
+METHODS = {"euler": EULER, "heun": HEUN, "rk4": RK4, "rk12": RK12, "rk23": RK23}
+
+
+class LLBParams(C.Structure):
+    """hs_llb_params of include/hotspin_amd.h."""
+    _fields_ = [
+        ("N", C.c_int * 3), ("cell", C.c_double * 3), ("pbc", C.c_int * 3),
+        ("msat0T0Mul", C.c_double), ("msat0T0", C.c_void_p),
+        ("exchange", C.c_int), ("lexMul", C.c_double), ("lex", C.c_void_p),
+        ("demag", C.c_int), ("demag_accuracy", C.c_int),
+        ("zeeman", C.c_int), ("Bext", C.c_double * 3),
+        ("longfield", C.c_int), ("JMul", C.c_double), ("nMul", C.c_double),
+        ("TcMul", C.c_double), ("TMul", C.c_double),
+        ("J", C.c_void_p), ("n", C.c_void_p), ("Tc", C.c_void_p), ("T", C.c_void_p),
+        ("thermal", C.c_int), ("seed", C.c_uint64), ("stream", C.c_uint32), ("cutoff_dt", C.c_double),
+        ("gammaLL", C.c_double), ("torque", C.c_int), ("local00nc", C.c_int),
+        ("alphaMul", C.c_double * 3), ("alpha", C.c_void_p * 3),
+        ("normalize", C.c_int),
+    ]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("steps", C.c_long), ("rejected", C.c_long), ("rhs_evals", C.c_long),
+                ("t", C.c_double), ("last_err", C.c_double), ("dt", C.c_double)]
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def make_params(N, cell, pbc=(0, 0, 0), **kw):
+    """Build LLBParams from keyword arguments (same names as the fields;
+    `alpha_mask` = 3 optional masks).  Returns (params, keepalive)."""
+    p = LLBParams()
+    N = tuple(int(x) for x in N)
+    ncell = N[0] * N[1] * N[2]
+    p.N[:] = list(N)
+    p.cell[:] = [float(c) for c in cell]
+    p.pbc[:] = [int(x) for x in pbc]
+    keep = []
+    for k, val in kw.items():
+        if k in ("msat0T0", "lex", "J", "n", "Tc", "T"):
+            a = _f64(val)
+            if a is not None:
+                a = a.reshape(-1)
+                if a.size != ncell:
+                    raise ValueError("mask %s has %d cells, mesh has %d" % (k, a.size, ncell))
+                keep.append(a)
+                setattr(p, k, a.ctypes.data)
+        elif k == "alpha_mask":
+            for c in range(3):
+                if val is not None and val[c] is not None:
+                    a = _f64(val[c]).reshape(-1)
+                    keep.append(a)
+                    p.alpha[c] = a.ctypes.data
+        elif k in ("Bext", "alphaMul"):
+            getattr(p, k)[:] = [float(x) for x in val]
+        else:
+            setattr(p, k, val)
+    return p, keep
+
+
+class Solver:
+    """Resident LLB-r solve on one GPU (FP32 fields, FP64 state)."""
+
+    def __init__(self, N, cell, pbc=(0, 0, 0), stream=None, **kw):
+        self.N = tuple(int(x) for x in N)
+        self.ncell = self.N[0] * self.N[1] * self.N[2]
+        self.params, self._keep = make_params(N, cell, pbc, **kw)
+        L = lib()
+        self._h = L.hs_solver_create(C.byref(self.params), stream)
+        if not self._h:
+            check("hs_solver_create")
+            raise HotspinError("hs_solver_create failed")
+        self.step_count = 0
+
+    def set_m(self, m):
+        m = _f64(m).reshape(3, self.ncell)
+        rc = lib().hs_solver_set_m(self._h, m.ctypes.data)
+        check("hs_solver_set_m")
+        return rc
+
+    def get_m(self):
+        out = np.empty((3, self.ncell), np.float64)
+        lib().hs_solver_get_m(self._h, out.ctypes.data)
+        check("hs_solver_get_m")
+        return out
+
+    def rhs(self, step=0, attempt=0, dt=0.0):
+        k = np.empty((3, self.ncell), np.float64)
+        h = np.empty((3, self.ncell), np.float64)
+        lib().hs_solver_rhs(self._h, k.ctypes.data, h.ctypes.data, int(step), int(attempt), float(dt))
+        check("hs_solver_rhs")
+        return k, h
+
+    def run(self, method, nsteps, dt, maxErr=1e-5, mindt=1e-38, maxdt=1e38, step0=None):
+        meth = METHODS[method] if isinstance(method, str) else int(method)
+        d = C.c_double(dt)
+        st = RunStats()
+        s0 = self.step_count if step0 is None else int(step0)
+        rc = lib().hs_solver_run(self._h, meth, int(nsteps), s0, C.byref(d), float(maxErr), float(mindt),
+                                 float(maxdt), C.byref(st))
+        check("hs_solver_run")
+        if rc:
+            raise HotspinError("hs_solver_run returned %d" % rc)
+        self.step_count = s0 + st.steps
+        return d.value, st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hs_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Demag:
+    """Demag plan: Kernel_Arne on the device (FP64) + pruned FFT convolution."""
+
+    def __init__(self, N, cell, pbc=(0, 0, 0), accuracy=6, stream=None):
+        self.N = tuple(int(x) for x in N)
+        n = (C.c_int * 3)(*self.N)
+        c = (C.c_double * 3)(*[float(x) for x in cell])
+        p = (C.c_int * 3)(*[int(x) for x in pbc])
+        self._h = lib().hs_demag_create(n, c, p, int(accuracy), stream)
+        if not self._h:
+            check("hs_demag_create")
+            raise HotspinError("hs_demag_create failed")
+        P = (C.c_int * 3)()
+        K = (C.c_int * 3)()
+        lib().hs_demag_sizes(self._h, P, K)
+        self.P = tuple(P)
+        self.K = tuple(K)
+
+    def kernel(self):
+        out = np.empty((6,) + self.K, np.float64)
+        lib().hs_demag_kernel_f64(self._h, out.ctypes.data)
+        check("hs_demag_kernel_f64")
+        return out
+
+    def max_imag_ratio(self):
+        return lib().hs_demag_max_imag_ratio(self._h)
+
+    def field_f64(self, m_ptr, msk_ptr, msat0T0Mul, B_ptr, stream=None):
+        rc = lib().hs_demag_field_f64(self._h, m_ptr, msk_ptr, float(msat0T0Mul), B_ptr, stream)
+        check("hs_demag_field_f64")
+        return rc
+
+    def field_f32(self, m_ptr, msk_ptr, msat0T0Mul, B_ptr, stream=None):
+        rc = lib().hs_demag_field_f32(self._h, m_ptr, msk_ptr, float(msat0T0Mul), B_ptr, stream)
+        check("hs_demag_field_f32")
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hs_demag_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

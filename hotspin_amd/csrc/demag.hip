@@ -1,0 +1,754 @@
+// demag.hip -- zero-padded demag convolution on gfx950.
+//
+// B = mu0 Ms0T0Mul * IFFT( K^ . FFT(m .* msat0T0) ), the algorithm of
+// MaxwellPlan.UpdateB (src/hotspin-core/modules/maxwell_convolution.go:103-131)
+// with the reference's cuFFT 3-D D2Z/Z2D replaced by five pruned passes of
+// the library's own LDS-staged FFT (fft.hpp).  Mesh N=(N0,N1,N2) internal,
+// logic size P (x2 on open axes, zeropad.go:12-22), M = P2/2, H2 = M+1:
+//   F1  rows, R2C along axis 2 (M-point complex FFT + split), only the N0*N1
+//       non-zero rows are read: m (3N)         -> X1 (3, N0, N1, H2)
+//   F2  columns along axis 1, input rows j < N1:  X1 -> X2 (3, N0, P1, H2)
+//   F3K columns along axis 0 for all 3 components, the symmetric kernel
+//       product (tensSYMMVecMul, src/libhotspin/mul.cu:31-90) and the inverse
+//       along axis 0, keeping rows i < N0:          X2 -> X2 (in place)
+//   I2  inverse along axis 1, keeping j < N1:       X2 -> X1
+//   I1  C2R along axis 2, keeping k < N2:           X1 -> B (3N)
+// The kernel K^ is Kernel_Arne (kernel_magnetostatic.go:25-241) integrated
+// on the GPU in FP64 (one lane per destination cell and source direction,
+// same operation order as the Go code, no FMA contraction), transformed in
+// FP64 by the same passes, its real first octant kept and scaled by
+// 1/(P0 P1 P2) (maxwell_convolution.go:135-223,350-388).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <vector>
+
+#include "fft.hpp"
+#include "hs_common.hpp"
+
+#define HS_PI_D 3.141592653589793
+
+// ------------------------------------------------------------------ tiling
+template <typename T, int L>
+constexpr int col_w() {  // adjacent columns per block: <= 128 KB of LDS
+    int w = 16;
+    while (w > 1 && (long)L * w * (long)sizeof(cpx<T>) > 131072) w /= 2;
+    return w;
+}
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+template <typename T, int L>
+constexpr int col_nt() {
+    return clampi(L * col_w<T, L>() / 16, 64, 1024);
+}
+template <int M>
+constexpr int row_w() {
+    return clampi(4096 / (M > 0 ? M : 1), 1, 16);
+}
+template <int M>
+constexpr int row_nt() {
+    return clampi(M * row_w<M>() / 8, 64, 512);
+}
+constexpr int ZW = 16;  // columns per block of the axis-0 convolution pass
+template <int L>
+constexpr int z_nt() {
+    return clampi(L * 3 * ZW / 16, 64, 1024);
+}
+
+struct Geo {  // pencil a starts at (a / A2) * S1 + (a % A2) * S2; element t at + t * ts
+    long S1, S2, ts;
+    int A2;
+};
+__device__ __forceinline__ long geo_base(const Geo& g, int a) { return (long)(a / g.A2) * g.S1 + (long)(a % g.A2) * g.S2; }
+
+// ------------------------------------------------------------------ F1: R2C rows
+// x: rows of n2 reals (row r at x + r*n2), optional mask of the same layout
+// (applied per cell, rows taken modulo mrows), out: rows of H2 complex at
+// stride H2p.  z[n] = x[2n] + i x[2n+1]; X[k] = E + w^k O with
+// E = (Z[k] + conj Z[M-k])/2, O = -i (Z[k] - conj Z[M-k])/2, w = exp(-2 pi i/P2).
+template <typename T, typename TI, int M>
+__global__ void __launch_bounds__(row_nt<M>()) k_r2c_rows(const TI* __restrict__ x, const TI* __restrict__ msk,
+                                                         long mrows, cpx<T>* __restrict__ out, long nrows, int n2,
+                                                         int H2p, const cpx<T>* __restrict__ twM,
+                                                         const cpx<T>* __restrict__ twP) {
+    constexpr int W = row_w<M>();
+    constexpr int NT = row_nt<M>();
+    __shared__ cpx<T> lds[M * W];
+    const int tid = threadIdx.x;
+    long row0 = (long)blockIdx.x * W;
+    for (int e = tid; e < M * W; e += NT) {
+        int p = e / M, n = e % M;
+        long r = row0 + p;
+        cpx<T> v = {T(0), T(0)};
+        if (r < nrows) {
+            long base = r * n2;
+            long mb = (r % mrows) * n2;
+            int i0 = 2 * n, i1 = 2 * n + 1;
+            if (i0 < n2) v.x = T(x[base + i0]) * (msk ? T(msk[mb + i0]) : T(1));
+            if (i1 < n2) v.y = T(x[base + i1]) * (msk ? T(msk[mb + i1]) : T(1));
+        }
+        lds[n * W + p] = v;
+    }
+    __syncthreads();
+    fft_lds<T, M, W, NT, -1>(lds, twM, tid);
+    for (int e = tid; e < (M + 1) * W; e += NT) {
+        int p = e / (M + 1), k = e % (M + 1);
+        long r = row0 + p;
+        if (r >= nrows) continue;
+        cpx<T> zk = lds[(k % M) * W + p];
+        cpx<T> zm = lds[((M - k) % M) * W + p];
+        cpx<T> E = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+        cpx<T> O = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
+        cpx<T> w = twP[k];
+        out[r * H2p + k] = cadd(E, cmul(w, O));
+    }
+}
+
+// ------------------------------------------------------------------ I1: C2R rows
+// in: rows of H2 = M+1 complex (stride H2p), out: rows of n2 reals keeping
+// the first n2 of P2 = 2M.  Z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]),
+// z = IFFT_M(Z), x[2n] = Re z[n], x[2n+1] = Im z[n] (unnormalised C2R).
+template <typename T, typename TO, int M>
+__global__ void __launch_bounds__(row_nt<M>()) k_c2r_rows(const cpx<T>* __restrict__ in, TO* __restrict__ x,
+                                                         long nrows, int n2, int H2p, const cpx<T>* __restrict__ twM,
+                                                         const cpx<T>* __restrict__ twP) {
+    constexpr int W = row_w<M>();
+    constexpr int NT = row_nt<M>();
+    __shared__ cpx<T> lds[(M + 1) * W];
+    const int tid = threadIdx.x;
+    long row0 = (long)blockIdx.x * W;
+    for (int e = tid; e < (M + 1) * W; e += NT) {
+        int p = e / (M + 1), k = e % (M + 1);
+        long r = row0 + p;
+        cpx<T> v = {T(0), T(0)};
+        if (r < nrows) v = in[r * H2p + k];
+        lds[k * W + p] = v;
+    }
+    __syncthreads();
+    constexpr int PER = (M * W + NT - 1) / NT;
+    cpx<T> z[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int e = tid + q * NT;
+        if (e < M * W) {
+            int k = e / W, p = e % W;
+            cpx<T> a = lds[k * W + p];
+            cpx<T> b = lds[(M - k) * W + p];
+            cpx<T> s = {a.x + b.x, a.y - b.y};      // X[k] + conj X[M-k]
+            cpx<T> d = {a.x - b.x, a.y + b.y};      // X[k] - conj X[M-k]
+            cpx<T> w = twP[k];
+            w.y = -w.y;                            // w^-k
+            cpx<T> wd = cmul(w, d);
+            z[q] = {s.x - wd.y, s.y + wd.x};       // s + i w^-k d
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int e = tid + q * NT;
+        if (e < M * W) lds[e] = z[q];
+    }
+    __syncthreads();
+    fft_lds<T, M, W, NT, +1>(lds, twM, tid);
+    for (int e = tid; e < M * W; e += NT) {
+        int p = e / M, n = e % M;
+        long r = row0 + p;
+        if (r >= nrows) continue;
+        cpx<T> v = lds[n * W + p];
+        int i0 = 2 * n, i1 = 2 * n + 1;
+        if (i0 < n2) x[r * n2 + i0] = TO(v.x);
+        if (i1 < n2) x[r * n2 + i1] = TO(v.y);
+    }
+}
+
+// ------------------------------------------------------------------ F2 / I2: columns
+// Pencil a (blockIdx.y), columns col0..col0+W (blockIdx.x); input rows t < nin
+// (zero beyond), output rows t < nkeep.  in and out may be the same buffer.
+template <typename T, int L, int S>
+__global__ void __launch_bounds__((col_nt<T, L>())) k_cols(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin,
+                                                        int nkeep, int ncol, const cpx<T>* __restrict__ tw) {
+    constexpr int W = col_w<T, L>();
+    constexpr int NT = col_nt<T, L>();
+    __shared__ cpx<T> lds[L * W];
+    const int tid = threadIdx.x;
+    const int a = blockIdx.y;
+    const int col0 = blockIdx.x * W;
+    const long bi = geo_base(gi, a) + col0;
+    const long bo = geo_base(go, a) + col0;
+    for (int e = tid; e < L * W; e += NT) {
+        int t = e / W, p = e % W;
+        cpx<T> v = {T(0), T(0)};
+        if (t < nin && col0 + p < ncol) v = in[bi + t * gi.ts + p];
+        lds[e] = v;
+    }
+    __syncthreads();
+    fft_lds<T, L, W, NT, S>(lds, tw, tid);
+    for (int e = tid; e < nkeep * W; e += NT) {
+        int t = e / W, p = e % W;
+        if (col0 + p < ncol) out[bo + t * go.ts + p] = lds[e];
+    }
+}
+
+// ------------------------------------------------------------------ F3K: axis 0 + kernel product
+// X2 layout (3, N0, P1, H2p); block = (column tile, j'), all 3 components.
+// Kernel octants: 6 planes (XX,YY,ZZ,YZ,XZ,XY) of (K0, K1, H2).
+template <typename T, typename TK, int L>
+__global__ void __launch_bounds__(z_nt<L>()) k_zconv(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1,
+                                                     int H2, int H2p, int K1, T srcMul,
+                                                     const cpx<T>* __restrict__ tw) {
+    constexpr int W = ZW;
+    constexpr int WP = 3 * W;
+    constexpr int NT = z_nt<L>();
+    __shared__ cpx<T> lds[L * WP];
+    const int tid = threadIdx.x;
+    const int jp = blockIdx.y;
+    const int col0 = blockIdx.x * W;
+    const long cstride = (long)N0 * P1 * H2p;
+    const long tstride = (long)P1 * H2p;
+    const long base = (long)jp * H2p + col0;
+    for (int e = tid; e < L * WP; e += NT) {
+        int t = e / WP, q = e % WP;
+        int c = q / W, p = q % W;
+        cpx<T> v = {T(0), T(0)};
+        if (t < N0 && col0 + p < H2) v = X[c * cstride + base + t * tstride + p];
+        lds[e] = v;
+    }
+    __syncthreads();
+    fft_lds<T, L, WP, NT, -1>(lds, tw, tid);
+    const int jj = (jp > P1 / 2) ? P1 - jp : jp;
+    const T sj = (jp > P1 / 2) ? T(-1) : T(1);
+    const long kplane = (long)(L / 2 + 1) * K1 * H2;
+    for (int e = tid; e < L * W; e += NT) {
+        int t = e / W, p = e % W;
+        int col = col0 + p;
+        if (col >= H2) continue;
+        int ii = (t > L / 2) ? L - t : t;
+        T si = (t > L / 2) ? T(-1) : T(1);
+        long K = ((long)ii * K1 + jj) * H2 + col;
+        T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
+        T kyz = sj * T(kern[3 * kplane + K]);
+        T kxz = si * T(kern[4 * kplane + K]);
+        T kxy = si * sj * T(kern[5 * kplane + K]);
+        cpx<T> mx = cscale(lds[t * WP + p], srcMul);
+        cpx<T> my = cscale(lds[t * WP + W + p], srcMul);
+        cpx<T> mz = cscale(lds[t * WP + 2 * W + p], srcMul);
+        lds[t * WP + p] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
+        lds[t * WP + W + p] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
+        lds[t * WP + 2 * W + p] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
+    }
+    __syncthreads();
+    fft_lds<T, L, WP, NT, +1>(lds, tw, tid);
+    for (int e = tid; e < N0 * WP; e += NT) {
+        int t = e / WP, q = e % WP;
+        int c = q / W, p = q % W;
+        if (col0 + p < H2) X[c * cstride + base + t * tstride + p] = lds[e];
+    }
+}
+
+// ------------------------------------------------------------------ Kernel_Arne (FP64)
+// One lane per (source direction s, destination cell (xw,yw,zw) of the
+// computed octant); images of a periodic axis are visited in the reference's
+// loop order (z outer, then y, then x), kernel_magnetostatic.go:74-183.
+__device__ __forceinline__ double delta_d(int d) {
+    d = d < 0 ? -d : d;
+    if (d > 0) d -= 1;
+    return (double)d;
+}
+__device__ __forceinline__ int first_image(int w, int size, int r1) {
+    // smallest v >= r1 with v = w (mod size)
+    int q = (r1 - w);
+    int k = (q >= 0) ? (q + size - 1) / size : -((-q) / size);
+    return w + k * size;
+}
+__global__ void __launch_bounds__(64) k_kernel_arne(double* __restrict__ oct, int s0, int s1, int s2, double c0,
+                                                    double c1, double c2, int r10, int r11, int r12, int r20,
+                                                    int r21, int r22, double accuracy) {
+#pragma clang fp contract(off)
+    const int h0 = s0 / 2 + 1, h1 = s1 / 2 + 1, h2 = s2 / 2 + 1;
+    const long noct = (long)h0 * h1 * h2;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 3 * noct) return;
+    const int s = (int)(g / noct);
+    long I = g % noct;
+    const int zw = (int)(I % h2);
+    const int yw = (int)((I / h2) % h1);
+    const int xw = (int)(I / ((long)h1 * h2));
+    const int size[3] = {s0, s1, s2};
+    const double cs[3] = {c0, c1, c2};
+    const int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
+    double L = c0;
+    if (c1 < L) L = c1;
+    if (c2 < L) L = c2;
+    double acc[3] = {0, 0, 0};
+    bool any = false;
+    for (int z = first_image(zw, s2, r12); z <= r22; z += s2) {
+        double R2c = (double)z * c2;
+        for (int y = first_image(yw, s1, r11); y <= r21; y += s1) {
+            double R1c = (double)y * c1;
+            for (int x = first_image(xw, s0, r10); x <= r20; x += s0) {
+                any = true;
+                double R[3] = {(double)x * c0, R1c, R2c};
+                double dx = delta_d(x) * c0, dy = delta_d(y) * c1, dz = delta_d(z) * c2;
+                double d = sqrt(dx * dx + dy * dy + dz * dz);
+                if (d == 0) d = L;
+                double maxSize = d / accuracy;
+                int nv = (int)(fmax(cs[v] / maxSize, 1.0) + 0.5);
+                int nw = (int)(fmax(cs[w] / maxSize, 1.0) + 0.5);
+                int nx = (int)(fmax(c0 / maxSize, 1.0) + 0.5);
+                int ny = (int)(fmax(c1 / maxSize, 1.0) + 0.5);
+                int nz = (int)(fmax(c2 / maxSize, 1.0) + 0.5);
+                nv *= 2;
+                nw *= 2;
+                double scale = 1 / (double)((long)nv * nw * nx * ny * nz);
+                double surface = cs[v] * cs[w];
+                double charge = surface * scale;
+                double pu1 = cs[u] / 2.;
+                double pu2 = -pu1;
+                double B[3] = {0, 0, 0};
+                double pole[3];
+                for (int i = 0; i < nv; i++) {
+                    pole[v] = -(cs[v] / 2.) + cs[v] / (double)(2 * nv) + (double)i * (cs[v] / (double)nv);
+                    for (int j = 0; j < nw; j++) {
+                        pole[w] = -(cs[w] / 2.) + cs[w] / (double)(2 * nw) + (double)j * (cs[w] / (double)nw);
+                        for (int aa = 0; aa < nx; aa++) {
+                            double rx = R[0] - c0 / 2 + c0 / (double)(2 * nx) + (c0 / (double)nx) * (double)aa;
+                            for (int bb = 0; bb < ny; bb++) {
+                                double ry = R[1] - c1 / 2 + c1 / (double)(2 * ny) + (c1 / (double)ny) * (double)bb;
+                                for (int gg = 0; gg < nz; gg++) {
+                                    double rz = R[2] - c2 / 2 + c2 / (double)(2 * nz) + (c2 / (double)nz) * (double)gg;
+                                    pole[u] = pu1;
+                                    double q0 = rx - pole[0], q1 = ry - pole[1], q2 = rz - pole[2];
+                                    double r = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+                                    double qr = charge / (4 * HS_PI_D * r * r * r);
+                                    double bx = q0 * qr, by = q1 * qr, bz = q2 * qr;
+                                    pole[u] = pu2;
+                                    q0 = rx - pole[0]; q1 = ry - pole[1]; q2 = rz - pole[2];
+                                    r = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+                                    qr = -charge / (4 * HS_PI_D * r * r * r);
+                                    B[0] += (bx + q0 * qr);
+                                    B[1] += (by + q1 * qr);
+                                    B[2] += (bz + q2 * qr);
+                                }
+                            }
+                        }
+                    }
+                }
+                for (int dd = s; dd < 3; dd++) acc[dd] += B[dd];
+            }
+        }
+    }
+    (void)any;
+    // FullTensorIdx (common/index.go:44-47): XX=0 YY=1 ZZ=2 YZ=3 XZ=4 XY=5
+    const int idx[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+    for (int dd = s; dd < 3; dd++) oct[idx[s][dd] * noct + I] = acc[dd];
+    (void)size;
+}
+
+// Fill the full real (P0,P1,P2) kernel of component c from its octant with
+// the mirror signs of kernel_magnetostatic.go:190-234.
+__global__ void k_kernel_fill(double* __restrict__ full, const double* __restrict__ oct, int s0, int s1, int s2) {
+    const int h0 = s0 / 2 + 1, h1 = s1 / 2 + 1, h2 = s2 / 2 + 1;
+    const long noct = (long)h0 * h1 * h2;
+    const long n = (long)s0 * s1 * s2;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (g >= n) return;
+    int z = (int)(g % s2);
+    int y = (int)((g / s2) % s1);
+    int x = (int)(g / ((long)s1 * s2));
+    bool fx = x > s0 / 2, fy = y > s1 / 2, fz = z > s2 / 2;
+    int xm = fx ? s0 - x : x, ym = fy ? s1 - y : y, zm = fz ? s2 - z : z;
+    double sg = 1.0;
+    if (c == 5) sg = (fx != fy) ? -1.0 : 1.0;        // XY odd in x and y
+    else if (c == 4) sg = (fx != fz) ? -1.0 : 1.0;   // XZ odd in x and z
+    else if (c == 3) sg = (fy != fz) ? -1.0 : 1.0;   // YZ odd in y and z
+    full[c * n + g] = sg * oct[c * noct + ((long)xm * h1 + ym) * h2 + zm];
+}
+
+// Keep the real part of the first octant of the transformed kernel, scaled by
+// 1/(P0 P1 P2); record max|imag| and max|real| (extract(), maxwell_convolution.go:350-388).
+__global__ void k_kernel_extract(const cpx<double>* __restrict__ X, double* __restrict__ kern64,
+                                 float* __restrict__ kern32, int P0, int P1, int H2, int H2p, int K0, int K1,
+                                 double norm, unsigned long long* __restrict__ maxbits) {
+    long nk = (long)K0 * K1 * H2;
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    double mi = 0, mr = 0;
+    if (g < nk) {
+        int k = (int)(g % H2);
+        int j = (int)((g / H2) % K1);
+        int i = (int)(g / ((long)K1 * H2));
+        cpx<double> v = X[(((long)c * P0 + i) * P1 + j) * H2p + k];
+        double r = v.x * norm;
+        kern64[c * nk + g] = r;
+        kern32[c * nk + g] = (float)r;
+        mi = fabs(v.y);
+        mr = fabs(v.x);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mi = fmax(mi, __shfl_xor(mi, o, 64));
+        mr = fmax(mr, __shfl_xor(mr, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&maxbits[0], (unsigned long long)__double_as_longlong(mi));
+        atomicMax(&maxbits[1], (unsigned long long)__double_as_longlong(mr));
+    }
+}
+
+// ------------------------------------------------------------------ dispatch tables
+template <typename T>
+using r2c_fn = void (*)(const void*, const void*, long, cpx<T>*, long, int, int, const cpx<T>*, const cpx<T>*,
+                        hipStream_t, long);
+
+namespace {
+
+template <typename T, typename TI, int M>
+void launch_r2c(const void* x, const void* msk, long mrows, cpx<T>* out, long nrows, int n2, int H2p,
+                const cpx<T>* twM, const cpx<T>* twP, hipStream_t st) {
+    constexpr int W = row_w<M>();
+    unsigned g = (unsigned)((nrows + W - 1) / W);
+    hipLaunchKernelGGL((k_r2c_rows<T, TI, M>), dim3(g), dim3(row_nt<M>()), 0, st, (const TI*)x, (const TI*)msk,
+                       mrows, out, nrows, n2, H2p, twM, twP);
+}
+template <typename T, typename TO, int M>
+void launch_c2r(const cpx<T>* in, void* x, long nrows, int n2, int H2p, const cpx<T>* twM, const cpx<T>* twP,
+                hipStream_t st) {
+    constexpr int W = row_w<M>();
+    unsigned g = (unsigned)((nrows + W - 1) / W);
+    hipLaunchKernelGGL((k_c2r_rows<T, TO, M>), dim3(g), dim3(row_nt<M>()), 0, st, in, (TO*)x, nrows, n2, H2p, twM,
+                       twP);
+}
+template <typename T, int L, int S>
+void launch_cols(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int npencil, int nin, int nkeep, int ncol,
+                 const cpx<T>* tw, hipStream_t st) {
+    constexpr int W = col_w<T, L>();
+    dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
+    hipLaunchKernelGGL((k_cols<T, L, S>), g, dim3(col_nt<T, L>()), 0, st, in, out, gi, go, nin, nkeep, ncol, tw);
+}
+template <typename T, typename TK, int L>
+void launch_zconv(cpx<T>* X, const TK* kern, int N0, int P1, int H2, int H2p, int K1, T srcMul, const cpx<T>* tw,
+                  hipStream_t st) {
+    dim3 g((unsigned)((H2 + ZW - 1) / ZW), (unsigned)P1);
+    hipLaunchKernelGGL((k_zconv<T, TK, L>), g, dim3(z_nt<L>()), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul, tw);
+}
+
+template <typename T, typename TI>
+using r2c_t = void (*)(const void*, const void*, long, cpx<T>*, long, int, int, const cpx<T>*, const cpx<T>*,
+                       hipStream_t);
+template <typename T, typename TO>
+using c2r_t = void (*)(const cpx<T>*, void*, long, int, int, const cpx<T>*, const cpx<T>*, hipStream_t);
+template <typename T>
+using cols_t = void (*)(const cpx<T>*, cpx<T>*, Geo, Geo, int, int, int, int, const cpx<T>*, hipStream_t);
+template <typename T, typename TK>
+using zconv_t = void (*)(cpx<T>*, const TK*, int, int, int, int, int, T, const cpx<T>*, hipStream_t);
+
+constexpr long kMaxLds = 163840;
+template <typename T, int M>
+constexpr bool rows_fit() { return (long)(M + 1) * row_w<M>() * (long)sizeof(cpx<T>) <= kMaxLds; }
+template <typename T, int L>
+constexpr bool cols_fit() { return (long)L * col_w<T, L>() * (long)sizeof(cpx<T>) <= kMaxLds; }
+template <typename T, int L>
+constexpr bool z_fit() { return L <= 256 && (long)L * 3 * ZW * (long)sizeof(cpx<T>) <= kMaxLds; }
+
+template <typename T, typename TI>
+r2c_t<T, TI> pick_r2c(int M) {
+#define X(n) \
+    if constexpr (rows_fit<T, n>()) if (M == n) return &launch_r2c<T, TI, n>;
+    HS_FFT_SIZES(X)
+#undef X
+    return nullptr;
+}
+template <typename T, typename TO>
+c2r_t<T, TO> pick_c2r(int M) {
+#define X(n) \
+    if constexpr (rows_fit<T, n>()) if (M == n) return &launch_c2r<T, TO, n>;
+    HS_FFT_SIZES(X)
+#undef X
+    return nullptr;
+}
+template <typename T, int S>
+cols_t<T> pick_cols(int L) {
+#define X(n) \
+    if constexpr (cols_fit<T, n>()) if (L == n) return &launch_cols<T, n, S>;
+    HS_FFT_SIZES(X)
+#undef X
+    return nullptr;
+}
+template <typename T, typename TK>
+zconv_t<T, TK> pick_zconv(int L) {
+#define X(n) \
+    if constexpr (z_fit<T, n>()) if (L == n) return &launch_zconv<T, TK, n>;
+    HS_FFT_SIZES(X)
+#undef X
+    return nullptr;
+}
+
+template <typename T>
+cpx<T>* make_twiddles(int L) {
+    std::vector<cpx<T>> h(L > 0 ? L : 1);
+    for (int t = 0; t < L; t++) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)L;
+        h[t] = {(T)cosl(a), (T)sinl(a)};
+    }
+    cpx<T>* d = nullptr;
+    if (hipMalloc(&d, sizeof(cpx<T>) * h.size()) != hipSuccess) return nullptr;
+    hipMemcpy(d, h.data(), sizeof(cpx<T>) * h.size(), hipMemcpyHostToDevice);
+    return d;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ plan
+struct hs_demag {
+    int N[3], P[3], K[3], pbc[3];
+    double cell[3];
+    int M, H2, H2p;
+    long ncell;
+    double* kern64 = nullptr;
+    float* kern32 = nullptr;
+    cpx<float>*X1f = nullptr, *X2f = nullptr;
+    cpx<double>*X1d = nullptr, *X2d = nullptr;
+    cpx<float>*twMf = nullptr, *twP2f = nullptr, *tw1f = nullptr, *tw0f = nullptr;
+    cpx<double>*twMd = nullptr, *twP2d = nullptr, *tw1d = nullptr, *tw0d = nullptr;
+    double max_imag_ratio = 0;
+};
+
+namespace {
+template <typename T>
+struct Pipe {
+    r2c_t<T, T> r2c;
+    c2r_t<T, T> c2r;
+    cols_t<T> fwd1, inv1;
+    zconv_t<T, T> zc;
+};
+
+bool sizes_ok(const hs_demag* d) {
+    return (d->P[2] % 2 == 0) && fft_supported(d->M) && pick_r2c<float, float>(d->M) &&
+           pick_cols<float, -1>(d->P[1]) && pick_zconv<float, float>(d->P[0]);
+}
+
+template <typename T>
+int run_field(hs_demag* d, const T* m, const T* msk, double srcMul, T* B, hipStream_t st, cpx<T>* X1, cpx<T>* X2,
+              const T* kern, const cpx<T>* twM, const cpx<T>* twP2, const cpx<T>* tw1, const cpx<T>* tw0) {
+    const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1], P0 = d->P[0];
+    const int H2 = d->H2, H2p = d->H2p, M = d->M;
+    auto r2c = pick_r2c<T, T>(M);
+    auto c2r = pick_c2r<T, T>(M);
+    auto f1 = pick_cols<T, -1>(P1);
+    auto i1 = pick_cols<T, +1>(P1);
+    auto zc = pick_zconv<T, T>(P0);
+    if (!r2c || !c2r || !f1 || !i1 || !zc) {
+        hs::set_error(HS_ESIZE, "demag: unsupported FFT length");
+        return HS_ESIZE;
+    }
+    const long nrows = 3L * N0 * N1;
+    // F1: m .* mask, rows of N2 -> X1 (3, N0, N1, H2p); the mask repeats every N0*N1 rows
+    hs::prof_begin(HS_PROF_F1, st);
+    r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, twM, twP2, st);
+    hs::prof_end(HS_PROF_F1, st);
+    // F2: pencils a = (c, i), rows j < N1 of X1 -> P1 rows of X2
+    Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
+    Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
+    hs::prof_begin(HS_PROF_F2, st);
+    f1(X1, X2, g1, g2, 3 * N0, N1, P1, H2, tw1, st);
+    hs::prof_end(HS_PROF_F2, st);
+    // F3K: axis 0, kernel product, inverse axis 0
+    hs::prof_begin(HS_PROF_F3K, st);
+    zc(X2, kern, N0, P1, H2, H2p, d->K[1], (T)srcMul, tw0, st);
+    hs::prof_end(HS_PROF_F3K, st);
+    // I2: P1 rows of X2 -> rows j < N1 of X1
+    hs::prof_begin(HS_PROF_I2, st);
+    i1(X2, X1, g2, g1, 3 * N0, P1, N1, H2, tw1, st);
+    hs::prof_end(HS_PROF_I2, st);
+    // I1: C2R rows -> B
+    hs::prof_begin(HS_PROF_I1, st);
+    c2r(X1, B, nrows, N2, H2p, twM, twP2, st);
+    hs::prof_end(HS_PROF_I1, st);
+    HS_LAUNCH_CHECK("hs_demag_field");
+    return hs_last_error();
+}
+}  // namespace
+
+extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const int pbc[3], int accuracy,
+                                     CUstream stream) {
+    if (!N || !cell || N[0] < 1 || N[1] < 1 || N[2] < 1) {
+        hs::set_error(HS_EINVAL, "hs_demag_create: bad mesh");
+        return nullptr;
+    }
+    hipStream_t st = hs::S(stream);
+    hs_demag* d = new hs_demag();
+    for (int c = 0; c < 3; c++) {
+        d->N[c] = N[c];
+        d->pbc[c] = pbc ? pbc[c] : 0;
+        d->cell[c] = cell[c];
+        d->P[c] = (N[c] > 1 && d->pbc[c] == 0) ? 2 * N[c] : N[c];
+    }
+    d->M = d->P[2] / 2;
+    d->H2 = d->M + 1;
+    d->H2p = (d->H2 + 7) / 8 * 8;
+    d->K[0] = d->P[0] / 2 + 1;
+    d->K[1] = d->P[1] / 2 + 1;
+    d->K[2] = d->H2;
+    d->ncell = (long)N[0] * N[1] * N[2];
+    if (!sizes_ok(d)) {
+        hs::set_error(HS_ESIZE, "hs_demag_create: padded size needs even P2 and 2^a 3^b 5^c FFT lengths");
+        delete d;
+        return nullptr;
+    }
+    const int P0 = d->P[0], P1 = d->P[1], P2 = d->P[2], H2 = d->H2, H2p = d->H2p, M = d->M;
+    const long nk = (long)d->K[0] * d->K[1] * d->K[2];
+    // twiddles
+    d->twMf = make_twiddles<float>(M);
+    d->twP2f = make_twiddles<float>(P2);
+    d->tw1f = make_twiddles<float>(P1);
+    d->tw0f = make_twiddles<float>(P0);
+    d->twMd = make_twiddles<double>(M);
+    d->twP2d = make_twiddles<double>(P2);
+    d->tw1d = make_twiddles<double>(P1);
+    d->tw0d = make_twiddles<double>(P0);
+    hipMalloc(&d->kern64, sizeof(double) * 6 * nk);
+    hipMalloc(&d->kern32, sizeof(float) * 6 * nk);
+    hipMalloc(&d->X1f, sizeof(cpx<float>) * 3L * N[0] * N[1] * H2p);
+    hipMalloc(&d->X2f, sizeof(cpx<float>) * 3L * N[0] * P1 * H2p);
+    // ---- Kernel_Arne on the device (FP64)
+    int r1[3], r2[3];
+    for (int c = 0; c < 3; c++) {
+        if (d->pbc[c] == 0) {
+            r1[c] = -(d->P[c] - 1) / 2;
+            r2[c] = (d->P[c] - 1) / 2;
+        } else {
+            r1[c] = -(d->P[c] * d->pbc[c] - 1);
+            r2[c] = (d->P[c] * d->pbc[c] - 1);
+        }
+    }
+    if (d->P[2] == 1 && d->pbc[2] == 0) r2[2] = 0;
+    const long noct = (long)(P0 / 2 + 1) * (P1 / 2 + 1) * (P2 / 2 + 1);
+    double* oct = nullptr;
+    double* full = nullptr;
+    cpx<double>* X = nullptr;
+    unsigned long long* maxbits = nullptr;
+    const long P = (long)P0 * P1 * P2;
+    hipMalloc(&oct, sizeof(double) * 6 * noct);
+    hipMalloc(&full, sizeof(double) * 6 * P);
+    hipMalloc(&X, sizeof(cpx<double>) * 6L * P0 * P1 * H2p);
+    hipMalloc(&maxbits, 2 * sizeof(unsigned long long));
+    if (!oct || !full || !X || !maxbits || !d->kern64 || !d->kern32 || !d->X1f || !d->X2f) {
+        hs::set_error(HS_ENOMEM, "hs_demag_create: out of device memory");
+        hipFree(oct); hipFree(full); hipFree(X); hipFree(maxbits);
+        hs_demag_destroy(d);
+        return nullptr;
+    }
+    hipMemsetAsync(oct, 0, sizeof(double) * 6 * noct, st);
+    hipMemsetAsync(maxbits, 0, 2 * sizeof(unsigned long long), st);
+    hipLaunchKernelGGL(k_kernel_arne, dim3(hs::div_up(3 * noct, 64)), dim3(64), 0, st, oct, P0, P1, P2, cell[0],
+                       cell[1], cell[2], r1[0], r1[1], r1[2], r2[0], r2[1], r2[2], (double)(accuracy > 0 ? accuracy : 6));
+    hipLaunchKernelGGL(k_kernel_fill, dim3(hs::grid1d(P), 6), dim3(hs::kBlock), 0, st, full, oct, P0, P1, P2);
+    // IsZero (maxwell_convolution.go:227-232): components below 1e-12 of the
+    // global max are dropped.  Done on the host from the octant.
+    std::vector<double> hoct(6 * noct);
+    hipMemcpyAsync(hoct.data(), oct, sizeof(double) * 6 * noct, hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    double gmax = 0, cmax[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 6; c++)
+        for (long i = 0; i < noct; i++) cmax[c] = fmax(cmax[c], fabs(hoct[c * noct + i]));
+    for (int c = 0; c < 6; c++) gmax = fmax(gmax, cmax[c]);
+    // ---- FP64 transform of the 6 real components: R2C rows, then axes 1 and 0 in place
+    auto r2c = pick_r2c<double, double>(M);
+    auto f1 = pick_cols<double, -1>(P1);
+    auto f0 = pick_cols<double, -1>(P0);
+    r2c(full, nullptr, 1, X, 6L * P0 * P1, P2, H2p, d->twMd, d->twP2d, st);
+    Geo ga1 = {(long)P1 * H2p, 0, H2p, 1};
+    f1(X, X, ga1, ga1, 6 * P0, P1, P1, H2, d->tw1d, st);
+    Geo ga0 = {(long)P0 * P1 * H2p, (long)H2p, (long)P1 * H2p, P1};
+    f0(X, X, ga0, ga0, 6 * P1, P0, P0, H2, d->tw0d, st);
+    hipLaunchKernelGGL(k_kernel_extract, dim3(hs::grid1d(nk), 6), dim3(hs::kBlock), 0, st, X, d->kern64, d->kern32,
+                       P0, P1, H2, H2p, d->K[0], d->K[1], 1.0 / (double)P, maxbits);
+    for (int c = 0; c < 6; c++) {
+        if (gmax > 0 && cmax[c] / gmax < 1e-12) {
+            hipMemsetAsync(d->kern64 + c * nk, 0, sizeof(double) * nk, st);
+            hipMemsetAsync(d->kern32 + c * nk, 0, sizeof(float) * nk, st);
+        }
+    }
+    unsigned long long mb[2];
+    hipMemcpyAsync(mb, maxbits, sizeof(mb), hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    double mi, mr;
+    memcpy(&mi, &mb[0], 8);
+    memcpy(&mr, &mb[1], 8);
+    d->max_imag_ratio = mr > 0 ? mi / mr : 0;
+    hipFree(oct);
+    hipFree(full);
+    hipFree(X);
+    hipFree(maxbits);
+    HS_LAUNCH_CHECK("hs_demag_create");
+    if (hs_last_error()) {
+        hs_demag_destroy(d);
+        return nullptr;
+    }
+    return d;
+}
+
+extern "C" void hs_demag_destroy(hs_demag* d) {
+    if (!d) return;
+    void* ps[] = {d->kern64, d->kern32, d->X1f, d->X2f, d->X1d, d->X2d, d->twMf, d->twP2f,
+                  d->tw1f,   d->tw0f,   d->twMd, d->twP2d, d->tw1d, d->tw0d};
+    for (void* p : ps)
+        if (p) hipFree(p);
+    delete d;
+}
+
+extern "C" int hs_demag_sizes(const hs_demag* d, int P[3], int K[3]) {
+    if (!d) return HS_EINVAL;
+    for (int c = 0; c < 3; c++) {
+        if (P) P[c] = d->P[c];
+        if (K) K[c] = d->K[c];
+    }
+    return 0;
+}
+
+extern "C" int hs_demag_kernel_f64(const hs_demag* d, double* host_out) {
+    if (!d || !host_out) return HS_EINVAL;
+    long nk = (long)d->K[0] * d->K[1] * d->K[2];
+    hipError_t e = hipMemcpy(host_out, d->kern64, sizeof(double) * 6 * nk, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? 0 : (int)e;
+}
+
+extern "C" double hs_demag_max_imag_ratio(const hs_demag* d) { return d ? d->max_imag_ratio : -1; }
+
+extern "C" int hs_demag_field_f32(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
+                                  CUstream stream) {
+    if (!d || !m || !B) {
+        hs::set_error(HS_EINVAL, "hs_demag_field_f32");
+        return HS_EINVAL;
+    }
+    return run_field<float>(d, m, msk, HS_MU0 * msat0T0Mul, B, hs::S(stream), d->X1f, d->X2f, d->kern32, d->twMf,
+                            d->twP2f, d->tw1f, d->tw0f);
+}
+
+extern "C" int hs_demag_field_f64(hs_demag* d, const double* m, const double* msk, double msat0T0Mul, double* B,
+                                  CUstream stream) {
+    if (!d || !m || !B) {
+        hs::set_error(HS_EINVAL, "hs_demag_field_f64");
+        return HS_EINVAL;
+    }
+    if (!d->X1d) {
+        hipMalloc(&d->X1d, sizeof(cpx<double>) * 3L * d->N[0] * d->N[1] * d->H2p);
+        hipMalloc(&d->X2d, sizeof(cpx<double>) * 3L * d->N[0] * d->P[1] * d->H2p);
+        if (!d->X1d || !d->X2d) {
+            hs::set_error(HS_ENOMEM, "hs_demag_field_f64: out of device memory");
+            return HS_ENOMEM;
+        }
+    }
+    return run_field<double>(d, m, msk, HS_MU0 * msat0T0Mul, B, hs::S(stream), d->X1d, d->X2d, d->kern64, d->twMd,
+                             d->twP2d, d->tw1d, d->tw0d);
+}
+
+// Internal entry for the solver: FP32 field of the first `ncomp`-component
+// planes with an FP32 mask.
+int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
+                                hipStream_t st) {
+    return run_field<float>(d, m, msk, HS_MU0 * msat0T0Mul, B, st, d->X1f, d->X2f, d->kern32, d->twMf, d->twP2f,
+                            d->tw1f, d->tw0f);
+}
+long hs_demag_ncell(const hs_demag* d) { return d->ncell; }

@@ -1,0 +1,804 @@
+// solver.hip -- the resident LLB-r solver: FP32 fields, FP64 state.
+//
+// One RK stage = [demag passes on the FP32 stage input m_s -> B] + one fused
+// stage kernel.  The fused kernel marches a 64(k) x 4(j) tile through all
+// planes i, holding planes i-1, i, i+1 of m_s in registers and the plane's
+// (j,k) halo in LDS (6-neighbour exchange of exchange6.cu:11-127), adds
+// B/mu0, B_ext/mu0, the longitudinal field (long_field.cu:13-72) and the
+// Philox thermal field (temperature.cu:14-71 scaling, mod_brown.go:125-145
+// prefactor), forms k = gamma (m x H + alpha .* H) (llbar-torque.cu:39,
+// llbar-local00nc.cu:37-56, summed as update_sum.go:41-62 does) and applies
+// the RK stage update to the FP64 state in the same pass: accumulator,
+// next stage input N(y0 + a h k) in FP32, error norm max|e| in FP64.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hs_common.hpp"
+#include "philox.hpp"
+
+int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
+                                hipStream_t st);
+
+namespace {
+
+constexpr int TX = 64, TY = 4;
+
+struct Phys {
+    int N0, N1, N2, w0, w1, w2;
+    int exchange, demag, zeeman, longfield, thermal, torque, local, normalize;
+    float ex0, ex1, ex2;  // lexMul^2 / c_d^2
+    float ms0T0Mul;
+    const float* lex;
+    const float* msk;  // msat0T0 mask
+    float invMu0;
+    float bext[3];  // B_ext / mu0
+    float JMul, nMul, TcMul, TMul;
+    const float *J, *n, *Tc, *T;
+    float lfPre;  // kB / mu0
+    uint64_t seed;
+    uint32_t stream;
+    float gamma;
+    float aMul[3];
+    const float* alpha[3];
+};
+
+// Roles of the fused stage kernel.
+enum Role {
+    R_RHS = 0,      // k and H_eff out (testing)
+    R_RK4_1, R_RK4_2, R_RK4_3, R_RK4_4,
+    R_BS_1, R_BS_2, R_BS_3, R_BS_4,  // Bogacki-Shampine RK23
+    R_RK12_1, R_RK12_2,               // Heun-Euler (reference RK12 / Heun)
+    R_EULER
+};
+
+struct StageIO {
+    const float* m;   // stage input (3 planes)
+    const float* B;   // demag field (3 planes) or null
+    double* y0;       // FP64 state at step start
+    double* acc;      // FP64 accumulator
+    float* e;         // FP32 error accumulator / stored k1
+    float* mout;      // next stage input
+    float* kout;      // R_RHS: k
+    float* hout;      // R_RHS: H_eff
+    unsigned long long* errmax;  // max |e| as double bits
+    double h;
+    float thK;        // thermal prefactor (depends on dt)
+    uint32_t step, attempt;
+};
+
+__device__ __forceinline__ float clampf_idx(int v) { return (float)v; }
+
+template <int ROLE>
+__global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
+    __shared__ float tile[3][TY + 2][TX + 2];
+    __shared__ float ltile[TY + 2][TX + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int k = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
+    const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
+    const long N12 = (long)N1 * N2;
+    const long N = (long)N0 * N12;
+    const bool in = (k < N2) && (j < N1);
+    const float* m = io.m;
+    // neighbour indices along k and j (clamped = Neumann, or wrapped)
+    auto wrapc = [](int c, int n, int w) {
+        if (c < 0) c = w ? c + n : 0;
+        if (c >= n) c = w ? c - n : n - 1;
+        return c;
+    };
+    const int kc = min(k, N2 - 1), jc = min(j, N1 - 1);
+    double errloc = 0.0;
+    // registers: plane i-1, i, i+1 at (j,k)
+    float a[3][3];  // [plane][comp]
+    float la[3] = {1.f, 1.f, 1.f};
+    {
+        long I0 = (long)jc * N2 + kc;
+        int ip = wrapc(-1, N0, P.w0);
+        int in1 = wrapc(1, N0, P.w0);
+        for (int c = 0; c < 3; c++) {
+            a[1][c] = m[c * N + I0];
+            a[0][c] = m[c * N + ip * N12 + I0];
+            a[2][c] = m[c * N + in1 * N12 + I0];
+        }
+        if (P.lex) {
+            la[1] = P.lex[I0];
+            la[0] = P.lex[ip * N12 + I0];
+            la[2] = P.lex[in1 * N12 + I0];
+        }
+    }
+    for (int i = 0; i < N0; i++) {
+        const long I = i * N12 + (long)jc * N2 + kc;
+        // ---- plane tile with halo in LDS
+        for (int c = 0; c < 3; c++) tile[c][ty + 1][tx + 1] = a[1][c];
+        if (P.lex) ltile[ty + 1][tx + 1] = la[1];
+        if (P.exchange) {
+            const long base = i * N12;
+            if (ty == 0 || ty == TY - 1) {
+                int jn = (ty == 0) ? wrapc(blockIdx.y * TY - 1, N1, P.w1) : wrapc(min(blockIdx.y * TY + TY, N1), N1, P.w1);
+                int row = (ty == 0) ? 0 : TY + 1;
+                long a2 = base + (long)jn * N2 + kc;
+                for (int c = 0; c < 3; c++) tile[c][row][tx + 1] = m[c * N + a2];
+                if (P.lex) ltile[row][tx + 1] = P.lex[a2];
+            }
+            if (tx == 0 || tx == TX - 1) {
+                int kn = (tx == 0) ? wrapc(blockIdx.x * TX - 1, N2, P.w2) : wrapc(min(blockIdx.x * TX + TX, N2), N2, P.w2);
+                int col = (tx == 0) ? 0 : TX + 1;
+                long a2 = base + (long)jc * N2 + kn;
+                for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = m[c * N + a2];
+                if (P.lex) ltile[ty + 1][col] = P.lex[a2];
+            }
+        }
+        __syncthreads();
+        // ---- effective field
+        float m0[3] = {a[1][0], a[1][1], a[1][2]};
+        float H[3] = {0.f, 0.f, 0.f};
+        if (P.exchange && in) {
+            float l0 = la[1];
+            float l02 = l0 * l0;
+            // neighbours along k: inside the tile unless at the tile edge or the mesh edge
+            int km = k - 1, kp = k + 1;
+            int kmx = (km < 0) ? (P.w2 ? tx : tx + 1) : tx;          // LDS column of k-1
+            int kpx = (kp >= N2) ? (P.w2 ? -1 : tx + 1) : tx + 2;   // LDS column of k+1
+            if (kp >= N2 && P.w2) kpx = -1;
+            int jm = j - 1, jp = j + 1;
+            int jmy = (jm < 0) ? (P.w1 ? ty : ty + 1) : ty;
+            int jpy = (jp >= N1) ? (P.w1 ? -1 : ty + 1) : ty + 2;
+            float e0[3], e1[3], e2[3];
+            // axis 0 (X): planes i-1, i+1 (clamped at open ends: self)
+            {
+                bool lo_self = (i == 0 && !P.w0), hi_self = (i == N0 - 1 && !P.w0);
+                float l1 = lo_self ? l0 : la[0], l2 = hi_self ? l0 : la[2];
+                float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+                for (int c = 0; c < 3; c++) {
+                    float v1 = lo_self ? m0[c] : a[0][c], v2 = hi_self ? m0[c] : a[2][c];
+                    e0[c] = P.ex0 * (pre1 * (v1 - m0[c]) + pre2 * (v2 - m0[c]));
+                }
+            }
+            // axis 2 (Z, contiguous k)
+            {
+                float v1[3], v2[3], l1, l2;
+                if (km < 0 && P.w2) {  // wrapped: k-1 = N2-1 lives in another tile unless N2 <= TX
+                    long aa = i * N12 + (long)jc * N2 + (N2 - 1);
+                    for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
+                    l1 = P.lex ? P.lex[aa] : 1.f;
+                } else {
+                    for (int c = 0; c < 3; c++) v1[c] = tile[c][ty + 1][kmx];
+                    l1 = P.lex ? ltile[ty + 1][kmx] : 1.f;
+                }
+                if (kpx < 0) {
+                    long aa = i * N12 + (long)jc * N2 + 0;
+                    for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
+                    l2 = P.lex ? P.lex[aa] : 1.f;
+                } else {
+                    for (int c = 0; c < 3; c++) v2[c] = tile[c][ty + 1][kpx];
+                    l2 = P.lex ? ltile[ty + 1][kpx] : 1.f;
+                }
+                float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+                for (int c = 0; c < 3; c++) e2[c] = P.ex2 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
+            }
+            // axis 1 (Y)
+            {
+                float v1[3], v2[3], l1, l2;
+                if (jm < 0 && P.w1) {
+                    long aa = i * N12 + (long)(N1 - 1) * N2 + kc;
+                    for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
+                    l1 = P.lex ? P.lex[aa] : 1.f;
+                } else {
+                    for (int c = 0; c < 3; c++) v1[c] = tile[c][jmy][tx + 1];
+                    l1 = P.lex ? ltile[jmy][tx + 1] : 1.f;
+                }
+                if (jpy < 0) {
+                    long aa = i * N12 + (long)0 * N2 + kc;
+                    for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
+                    l2 = P.lex ? P.lex[aa] : 1.f;
+                } else {
+                    for (int c = 0; c < 3; c++) v2[c] = tile[c][jpy][tx + 1];
+                    l2 = P.lex ? ltile[jpy][tx + 1] : 1.f;
+                }
+                float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+                for (int c = 0; c < 3; c++) e1[c] = P.ex1 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
+            }
+            for (int c = 0; c < 3; c++) H[c] = P.ms0T0Mul * (e0[c] + e2[c] + e1[c]);
+        }
+        __syncthreads();  // tile reuse in the next plane
+        // roll the plane registers (load plane i+2)
+        float nxt[3] = {0.f, 0.f, 0.f};
+        float lnx = 1.f;
+        if (N0 > 1) {
+            int i2 = wrapc(i + 2, N0, P.w0);
+            if (!P.w0 && i + 2 >= N0) i2 = N0 - 1;
+            long a2 = i2 * N12 + (long)jc * N2 + kc;
+            for (int c = 0; c < 3; c++) nxt[c] = m[c * N + a2];
+            if (P.lex) lnx = P.lex[a2];
+        }
+        if (in) {
+            const float ms = P.msk ? P.msk[I] : 1.f;
+            if (P.demag) {
+                H[0] += P.invMu0 * io.B[I];
+                H[1] += P.invMu0 * io.B[N + I];
+                H[2] += P.invMu0 * io.B[2 * N + I];
+            }
+            if (P.zeeman) {
+                H[0] += P.bext[0];
+                H[1] += P.bext[1];
+                H[2] += P.bext[2];
+            }
+            if (P.longfield) {
+                float Ms0T0 = P.ms0T0Mul * ms;
+                float S = P.JMul * (P.J ? P.J[I] : 1.f);
+                float n = P.nMul * (P.n ? P.n[I] : 1.f);
+                float Tc = P.TcMul * (P.Tc ? P.Tc[I] : 1.f);
+                float Ts = P.TMul * (P.T ? P.T[I] : 1.f);
+                float am = sqrtf(m0[0] * m0[0] + m0[1] * m0[1] + m0[2] * m0[2]);
+                if (!(Ms0T0 == 0.f || n == 0.f || am <= 1e-30f)) {
+                    float preS = (S > (float)HS_INFSPIN) ? 1.f : S / (S + 1.f);
+                    float b = 3.f * preS * Tc / Ts;
+                    float W, dW;
+                    brillouin_w_dw<float>(S, am * b, W, dW);
+                    float mult = n * P.lfPre * Ts * (W - am) / (Ms0T0 * dW) / am;
+                    H[0] += mult * m0[0];
+                    H[1] += mult * m0[1];
+                    H[2] += mult * m0[2];
+                }
+            }
+            if (P.thermal && ms != 0.f) {
+                float x0, x1, x2;
+                thermal_normals_f((uint32_t)I, io.step, io.attempt, P.stream, P.seed, x0, x1, x2);
+                float Ti = P.T ? P.T[I] : 1.f;
+                float pre = sqrtf((Ti * io.thK) / ms);
+                float s0 = sqrtf(P.aMul[0] * (P.alpha[0] ? P.alpha[0][I] : 1.f));
+                float s1 = sqrtf(P.aMul[1] * (P.alpha[1] ? P.alpha[1][I] : 1.f));
+                float s2 = sqrtf(P.aMul[2] * (P.alpha[2] ? P.alpha[2][I] : 1.f));
+                H[0] += pre * (s0 * x0);
+                H[1] += pre * (s1 * x1);
+                H[2] += pre * (s2 * x2);
+            }
+            float kk[3] = {0.f, 0.f, 0.f};
+            if (ms != 0.f) {
+                if (P.torque) {  // cross(H, m) = m x H
+                    kk[0] = P.gamma * (-H[1] * m0[2] + H[2] * m0[1]);
+                    kk[1] = P.gamma * (-H[2] * m0[0] + H[0] * m0[2]);
+                    kk[2] = P.gamma * (-H[0] * m0[1] + H[1] * m0[0]);
+                }
+                if (P.local) {
+                    for (int c = 0; c < 3; c++) {
+                        float al = P.aMul[c] * (P.alpha[c] ? P.alpha[c][I] : 1.f);
+                        kk[c] += P.gamma * (al * H[c]);
+                    }
+                }
+            }
+            // ---- RK stage update (FP64 state)
+            const double h = io.h;
+            if constexpr (ROLE == R_RHS) {
+                for (int c = 0; c < 3; c++) {
+                    io.kout[c * N + I] = kk[c];
+                    io.hout[c * N + I] = H[c];
+                }
+            } else {
+                double y[3], acc[3];
+                auto nrm = [&](double* v) {
+                    if (P.normalize) {
+                        double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                        double s = (l == 0.0) ? 0.0 : 1.0 / l;
+                        v[0] *= s; v[1] *= s; v[2] *= s;
+                    }
+                };
+                auto store_m = [&](const double* v) {
+                    double w[3] = {v[0], v[1], v[2]};
+                    nrm(w);
+                    for (int c = 0; c < 3; c++) io.mout[c * N + I] = (float)w[c];
+                };
+                if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER) {
+                    for (int c = 0; c < 3; c++) y[c] = io.y0[c * N + I];
+                    if (P.normalize) {
+                        nrm(y);
+                        for (int c = 0; c < 3; c++) io.y0[c * N + I] = y[c];
+                    }
+                }
+                if constexpr (ROLE == R_RK4_1) {
+                    for (int c = 0; c < 3; c++) {
+                        io.acc[c * N + I] = y[c] + (h / 6.0) * (double)kk[c];
+                        acc[c] = y[c] + (0.5 * h) * (double)kk[c];
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_RK4_2 || ROLE == R_RK4_3) {
+                    const double cm = (ROLE == R_RK4_2) ? 0.5 * h : h;
+                    for (int c = 0; c < 3; c++) {
+                        io.acc[c * N + I] += (h / 3.0) * (double)kk[c];
+                        acc[c] = io.y0[c * N + I] + cm * (double)kk[c];
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_RK4_4) {
+                    for (int c = 0; c < 3; c++) {
+                        acc[c] = io.acc[c * N + I] + (h / 6.0) * (double)kk[c];
+                        io.y0[c * N + I] = acc[c];
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_BS_1) {
+                    for (int c = 0; c < 3; c++) {
+                        io.acc[c * N + I] = y[c] + (h * (2.0 / 9.0)) * (double)kk[c];
+                        io.e[c * N + I] = (float)((h * (-5.0 / 72.0)) * (double)kk[c]);
+                        acc[c] = y[c] + (0.5 * h) * (double)kk[c];
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_BS_2) {
+                    for (int c = 0; c < 3; c++) {
+                        io.acc[c * N + I] += (h / 3.0) * (double)kk[c];
+                        io.e[c * N + I] += (float)((h / 12.0) * (double)kk[c]);
+                        acc[c] = io.y0[c * N + I] + (0.75 * h) * (double)kk[c];
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_BS_3) {
+                    for (int c = 0; c < 3; c++) {
+                        acc[c] = io.acc[c * N + I] + (h * (4.0 / 9.0)) * (double)kk[c];
+                        io.acc[c * N + I] = acc[c];
+                        io.e[c * N + I] += (float)((h / 9.0) * (double)kk[c]);
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_BS_4) {
+                    for (int c = 0; c < 3; c++) {
+                        double ev = (double)io.e[c * N + I] - (h / 8.0) * (double)kk[c];
+                        errloc = fmax(errloc, fabs(ev));
+                    }
+                    if (P.normalize) {
+                        for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
+                        nrm(acc);
+                        for (int c = 0; c < 3; c++) io.acc[c * N + I] = acc[c];
+                    }
+                } else if constexpr (ROLE == R_RK12_1) {
+                    for (int c = 0; c < 3; c++) {
+                        io.e[c * N + I] = kk[c];
+                        acc[c] = y[c] + h * (double)kk[c];
+                        io.acc[c * N + I] = acc[c];
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_RK12_2) {
+                    for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
+                    nrm(acc);
+                    for (int c = 0; c < 3; c++) {
+                        float k1 = io.e[c * N + I];
+                        acc[c] += (0.5 * h) * (double)kk[c] + (-0.5 * h) * (double)k1;
+                        io.acc[c * N + I] = acc[c];
+                        errloc = fmax(errloc, fabs((double)kk[c] - (double)k1));
+                    }
+                    store_m(acc);
+                } else if constexpr (ROLE == R_EULER) {
+                    for (int c = 0; c < 3; c++) {
+                        acc[c] = y[c] + h * (double)kk[c];
+                        io.y0[c * N + I] = acc[c];
+                    }
+                    store_m(acc);
+                }
+            }
+        }
+        // roll
+        for (int c = 0; c < 3; c++) {
+            a[0][c] = a[1][c];
+            a[1][c] = a[2][c];
+            a[2][c] = nxt[c];
+        }
+        la[0] = la[1];
+        la[1] = la[2];
+        la[2] = lnx;
+        // wrapped/clamped plane i+1 for the next iteration is already in a[2]
+        if (P.w0 && N0 > 1) {
+            // a[0] must be plane i (it is), a[2] plane (i+2) mod N0 (loaded above)
+        }
+    }
+    if constexpr (ROLE == R_BS_4 || ROLE == R_RK12_2) {
+        for (int o = 32; o > 0; o >>= 1) errloc = fmax(errloc, __shfl_xor(errloc, o, 64));
+        if (((ty * TX + tx) & 63) == 0) atomicMax(io.errmax, (unsigned long long)__double_as_longlong(errloc));
+    }
+}
+
+// RK12 retry: predictor from the stored k1 without a new field evaluation
+// (solve_rk12.go:127-134 restores y0 and dy, then Madd).
+__global__ void k_rk12_predict(const double* y0, const float* k1, double* acc, float* mout, double h, int normalize,
+                               long N) {
+    long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    double v[3];
+    for (int c = 0; c < 3; c++) {
+        v[c] = y0[c * N + I] + h * (double)k1[c * N + I];
+        acc[c * N + I] = v[c];
+    }
+    if (normalize) {
+        double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double s = l == 0.0 ? 0.0 : 1.0 / l;
+        for (int c = 0; c < 3; c++) v[c] *= s;
+    }
+    for (int c = 0; c < 3; c++) mout[c * N + I] = (float)v[c];
+}
+
+__global__ void k_set_state(const double* y, float* mf, int normalize, long N) {
+    long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    double v[3] = {y[I], y[N + I], y[2 * N + I]};
+    if (normalize) {
+        double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double s = l == 0.0 ? 0.0 : 1.0 / l;
+        for (int c = 0; c < 3; c++) v[c] *= s;
+    }
+    for (int c = 0; c < 3; c++) mf[c * N + I] = (float)v[c];
+}
+
+__global__ void k_d2f(const double* a, float* b, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+}  // namespace
+
+struct hs_solver {
+    hs_llb_params p;
+    hipStream_t st;
+    long N;
+    hs_demag* demag = nullptr;
+    Phys phys;
+    // device buffers
+    double *y0 = nullptr, *acc = nullptr;
+    float *mA = nullptr, *mB = nullptr, *mC = nullptr;  // FP32 stage inputs
+    float *B0 = nullptr, *Bs = nullptr;
+    float* e = nullptr;
+    float *kout = nullptr, *hout = nullptr;
+    unsigned long long* errmax = nullptr;
+    std::vector<float*> masks;
+    bool b0_valid = false;  // B0 = demag(mA) is current
+};
+
+namespace {
+float* upload_mask(hs_solver* s, const double* h) {
+    if (!h) return nullptr;
+    double* tmp = nullptr;
+    float* d = nullptr;
+    hipMalloc(&tmp, sizeof(double) * s->N);
+    hipMalloc(&d, sizeof(float) * s->N);
+    hipMemcpy(tmp, h, sizeof(double) * s->N, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_d2f, hs::grid1d(s->N), hs::kBlock, 0, s->st, tmp, d, s->N);
+    hipStreamSynchronize(s->st);
+    hipFree(tmp);
+    s->masks.push_back(d);
+    return d;
+}
+
+template <int ROLE>
+void launch_stage(hs_solver* s, const StageIO& io) {
+    const Phys& P = s->phys;
+    dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY));
+    hs::prof_begin(HS_PROF_STAGE, s->st);
+    hipLaunchKernelGGL(k_stage<ROLE>, grd, blk, 0, s->st, P, io);
+    hs::prof_end(HS_PROF_STAGE, s->st);
+}
+
+void demag_into(hs_solver* s, const float* m, float* B) {
+    if (s->phys.demag) hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st);
+}
+
+float thermal_K(const hs_solver* s, double dt) {
+    // mod_brown.go:125-145: K = 2 kB TMul / (mu0 V gamma max(cutoff_dt, dt) Ms0T0Mul)
+    const hs_llb_params& p = s->p;
+    double V = p.cell[0] * p.cell[1] * p.cell[2];
+    double KB2TMul = HS_KB * 2.0 * p.TMul;
+    double deltat = fmax(p.cutoff_dt, dt);
+    double den = HS_MU0 * V * p.gammaLL * deltat * p.msat0T0Mul;
+    return (float)(KB2TMul / den);
+}
+
+StageIO base_io(hs_solver* s, double h, uint32_t step, uint32_t attempt) {
+    StageIO io{};
+    io.y0 = s->y0;
+    io.acc = s->acc;
+    io.e = s->e;
+    io.errmax = s->errmax;
+    io.h = h;
+    io.thK = s->p.thermal ? thermal_K(s, h) : 0.f;
+    io.step = step;
+    io.attempt = attempt;
+    return io;
+}
+
+double read_err(hs_solver* s) {
+    unsigned long long b = 0;
+    hipMemcpyAsync(&b, s->errmax, sizeof(b), hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    double v;
+    memcpy(&v, &b, sizeof(v));
+    return v;
+}
+void zero_err(hs_solver* s) { hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long), s->st); }
+
+void ensure_b0(hs_solver* s) {
+    if (!s->b0_valid) {
+        demag_into(s, s->mA, s->B0);
+        s->b0_valid = true;
+    }
+}
+
+// the reference's step-size controller, solve_rk12.go:164-198
+double controller(double dt, double err, double maxErr, bool bad, double expo_good, double expo_bad,
+                  double mindt, double maxdt) {
+    double factor = bad ? pow(maxErr / err, expo_bad) : pow(maxErr / err, expo_good);
+    factor *= 0.8;
+    if (factor > 1.5) factor = 1.5;
+    if (factor < 0.1) factor = 0.1;
+    double nd = dt * factor;
+    if (nd < mindt) nd = mindt;
+    if (nd > maxdt) nd = maxdt;
+    return nd;
+}
+}  // namespace
+
+extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) {
+    if (!p || p->N[0] < 1 || p->N[1] < 1 || p->N[2] < 1) {
+        hs::set_error(HS_EINVAL, "hs_solver_create: bad mesh");
+        return nullptr;
+    }
+    hs_solver* s = new hs_solver();
+    s->p = *p;
+    s->st = hs::S(stream);
+    s->N = (long)p->N[0] * p->N[1] * p->N[2];
+    const long N = s->N;
+    hipMalloc(&s->y0, sizeof(double) * 3 * N);
+    hipMalloc(&s->acc, sizeof(double) * 3 * N);
+    hipMalloc(&s->mA, sizeof(float) * 3 * N);
+    hipMalloc(&s->mB, sizeof(float) * 3 * N);
+    hipMalloc(&s->mC, sizeof(float) * 3 * N);
+    hipMalloc(&s->B0, sizeof(float) * 3 * N);
+    hipMalloc(&s->Bs, sizeof(float) * 3 * N);
+    hipMalloc(&s->e, sizeof(float) * 3 * N);
+    hipMalloc(&s->errmax, sizeof(unsigned long long));
+    if (!s->y0 || !s->acc || !s->mA || !s->mB || !s->mC || !s->B0 || !s->Bs || !s->e || !s->errmax) {
+        hs::set_error(HS_ENOMEM, "hs_solver_create: out of device memory");
+        hs_solver_destroy(s);
+        return nullptr;
+    }
+    hipMemsetAsync(s->B0, 0, sizeof(float) * 3 * N, s->st);
+    hipMemsetAsync(s->Bs, 0, sizeof(float) * 3 * N, s->st);
+    Phys& P = s->phys;
+    P.N0 = p->N[0];
+    P.N1 = p->N[1];
+    P.N2 = p->N[2];
+    P.w0 = p->pbc[0] != 0;
+    P.w1 = p->pbc[1] != 0;
+    P.w2 = p->pbc[2] != 0;
+    P.exchange = p->exchange;
+    P.demag = p->demag;
+    P.zeeman = p->zeeman;
+    P.longfield = p->longfield;
+    P.thermal = p->thermal;
+    P.torque = p->torque;
+    P.local = p->local00nc;
+    P.normalize = p->normalize;
+    double lex2 = p->lexMul * p->lexMul;
+    P.ex0 = (float)(lex2 / (p->cell[0] * p->cell[0]));
+    P.ex1 = (float)(lex2 / (p->cell[1] * p->cell[1]));
+    P.ex2 = (float)(lex2 / (p->cell[2] * p->cell[2]));
+    P.ms0T0Mul = (float)p->msat0T0Mul;
+    P.lex = upload_mask(s, p->lex);
+    P.msk = upload_mask(s, p->msat0T0);
+    P.invMu0 = (float)(1.0 / HS_MU0);
+    for (int c = 0; c < 3; c++) P.bext[c] = (float)(1.0 / HS_MU0 * p->Bext[c]);
+    P.JMul = (float)p->JMul;
+    P.nMul = (float)p->nMul;
+    P.TcMul = (float)p->TcMul;
+    P.TMul = (float)p->TMul;
+    P.J = upload_mask(s, p->J);
+    P.n = upload_mask(s, p->n);
+    P.Tc = upload_mask(s, p->Tc);
+    P.T = upload_mask(s, p->T);
+    P.lfPre = (float)(HS_KB / HS_MU0);
+    P.seed = p->seed;
+    P.stream = p->stream;
+    P.gamma = (float)p->gammaLL;
+    for (int c = 0; c < 3; c++) {
+        P.aMul[c] = (float)p->alphaMul[c];
+        P.alpha[c] = upload_mask(s, p->alpha[c]);
+    }
+    if (p->demag) {
+        s->demag = hs_demag_create(p->N, p->cell, p->pbc, p->demag_accuracy > 0 ? p->demag_accuracy : 6, stream);
+        if (!s->demag) {
+            hs_solver_destroy(s);
+            return nullptr;
+        }
+    }
+    HS_LAUNCH_CHECK("hs_solver_create");
+    return s;
+}
+
+extern "C" void hs_solver_destroy(hs_solver* s) {
+    if (!s) return;
+    if (s->demag) hs_demag_destroy(s->demag);
+    void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->kout, s->hout, s->errmax};
+    for (void* q : ps)
+        if (q) hipFree(q);
+    for (float* q : s->masks) hipFree(q);
+    delete s;
+}
+
+extern "C" CUstream hs_solver_stream(hs_solver* s) { return s ? (CUstream)s->st : nullptr; }
+
+extern "C" int hs_solver_set_m(hs_solver* s, const double* m) {
+    if (!s || !m) return HS_EINVAL;
+    hipMemcpyAsync(s->y0, m, sizeof(double) * 3 * s->N, hipMemcpyHostToDevice, s->st);
+    hipLaunchKernelGGL(k_set_state, hs::grid1d(s->N), hs::kBlock, 0, s->st, s->y0, s->mA, s->p.normalize, s->N);
+    s->b0_valid = false;
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_set_m");
+    return hs_last_error();
+}
+
+extern "C" int hs_solver_get_m(hs_solver* s, double* m) {
+    if (!s || !m) return HS_EINVAL;
+    hipMemcpyAsync(m, s->y0, sizeof(double) * 3 * s->N, hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_get_m");
+    return hs_last_error();
+}
+
+extern "C" int hs_solver_rhs(hs_solver* s, double* k_host, double* h_host, uint32_t step, uint32_t attempt,
+                             double dt) {
+    if (!s) return HS_EINVAL;
+    const long N = s->N;
+    if (!s->kout) {
+        hipMalloc(&s->kout, sizeof(float) * 3 * N);
+        hipMalloc(&s->hout, sizeof(float) * 3 * N);
+    }
+    ensure_b0(s);
+    StageIO io = base_io(s, dt, step, attempt);
+    io.m = s->mA;
+    io.B = s->B0;
+    io.kout = s->kout;
+    io.hout = s->hout;
+    launch_stage<R_RHS>(s, io);
+    std::vector<float> tmp(3 * N);
+    if (k_host) {
+        hipMemcpyAsync(tmp.data(), s->kout, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s->st);
+        hipStreamSynchronize(s->st);
+        for (long i = 0; i < 3 * N; i++) k_host[i] = tmp[i];
+    }
+    if (h_host) {
+        hipMemcpyAsync(tmp.data(), s->hout, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s->st);
+        hipStreamSynchronize(s->st);
+        for (long i = 0; i < 3 * N; i++) h_host[i] = tmp[i];
+    }
+    HS_LAUNCH_CHECK("hs_solver_rhs");
+    return hs_last_error();
+}
+
+extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, double* dtio, double maxErr,
+                             double mindt, double maxdt, hs_run_stats* stats) {
+    if (!s || !dtio || nsteps < 0) return HS_EINVAL;
+    hs_run_stats st{};
+    double dt = *dtio;
+    const int maxTry = 10;
+    for (long n = 0; n < nsteps; n++) {
+        uint32_t step = (uint32_t)(step0 + n);
+        if (method == HS_RK4) {
+            ensure_b0(s);
+            StageIO io = base_io(s, dt, step, 0);
+            io.m = s->mA; io.B = s->B0; io.mout = s->mB;
+            launch_stage<R_RK4_1>(s, io);
+            demag_into(s, s->mB, s->Bs);
+            io.m = s->mB; io.B = s->Bs; io.mout = s->mC;
+            launch_stage<R_RK4_2>(s, io);
+            demag_into(s, s->mC, s->Bs);
+            io.m = s->mC; io.B = s->Bs; io.mout = s->mB;
+            launch_stage<R_RK4_3>(s, io);
+            demag_into(s, s->mB, s->Bs);
+            io.m = s->mB; io.B = s->Bs; io.mout = s->mA;
+            launch_stage<R_RK4_4>(s, io);
+            s->b0_valid = false;
+            st.rhs_evals += 4;
+            st.t += dt;
+        } else if (method == HS_EULER) {
+            ensure_b0(s);
+            StageIO io = base_io(s, dt, step, 0);
+            io.m = s->mA; io.B = s->B0; io.mout = s->mB;
+            launch_stage<R_EULER>(s, io);
+            std::swap(s->mA, s->mB);
+            s->b0_valid = false;
+            st.rhs_evals += 1;
+            st.t += dt;
+        } else if (method == HS_RK23) {
+            int tr = 0;
+            double err = 0;
+            for (;;) {
+                ensure_b0(s);
+                StageIO io = base_io(s, dt, step, (uint32_t)tr);
+                io.m = s->mA; io.B = s->B0; io.mout = s->mB;
+                launch_stage<R_BS_1>(s, io);
+                demag_into(s, s->mB, s->Bs);
+                io.m = s->mB; io.B = s->Bs; io.mout = s->mC;
+                launch_stage<R_BS_2>(s, io);
+                demag_into(s, s->mC, s->Bs);
+                io.m = s->mC; io.B = s->Bs; io.mout = s->mB;
+                launch_stage<R_BS_3>(s, io);
+                demag_into(s, s->mB, s->Bs);
+                zero_err(s);
+                io.m = s->mB; io.B = s->Bs; io.mout = nullptr;
+                launch_stage<R_BS_4>(s, io);
+                st.rhs_evals += 3;
+                err = read_err(s);
+                bool bad = err > maxErr;
+                double nd = controller(dt, err, maxErr, bad, 1.0 / 3.0, 1.0 / 3.0, mindt, maxdt);
+                double used = dt;
+                dt = nd;
+                if (!bad || nd == mindt) {
+                    // accept: y1 (acc) becomes the state; its stage input and field are mB/Bs
+                    std::swap(s->y0, s->acc);
+                    std::swap(s->mA, s->mB);
+                    std::swap(s->B0, s->Bs);
+                    s->b0_valid = s->phys.demag != 0;
+                    st.t += used;
+                    break;
+                }
+                if (tr > maxTry) {
+                    hs::set_error(HS_ECONV, "hs_solver_run: RK23 did not converge");
+                    *dtio = dt;
+                    if (stats) *stats = st;
+                    return HS_ECONV;
+                }
+                tr++;
+                st.rejected++;
+            }
+            st.last_err = err;
+        } else if (method == HS_RK12 || method == HS_HEUN) {
+            int tr = 0;
+            double err = 0;
+            ensure_b0(s);
+            StageIO io = base_io(s, dt, step, 0);
+            io.m = s->mA; io.B = s->B0; io.mout = s->mB;
+            launch_stage<R_RK12_1>(s, io);
+            for (;;) {
+                if (tr > 0) {
+                    hipLaunchKernelGGL(k_rk12_predict, hs::grid1d(s->N), hs::kBlock, 0, s->st, s->y0, s->e, s->acc,
+                                       s->mB, dt, s->p.normalize, s->N);
+                }
+                demag_into(s, s->mB, s->Bs);
+                zero_err(s);
+                io = base_io(s, dt, step, (uint32_t)tr);
+                io.m = s->mB; io.B = s->Bs; io.mout = s->mC;
+                launch_stage<R_RK12_2>(s, io);
+                st.rhs_evals += 1;
+                if (method == HS_HEUN) {
+                    st.t += dt;
+                    break;
+                }
+                err = read_err(s) * dt;
+                bool bad = err > maxErr;
+                double nd = controller(dt, err, maxErr, bad, 0.5, 1.0 / 3.0, mindt, maxdt);
+                double used = dt;
+                dt = nd;
+                if (!bad || nd == mindt) {
+                    st.t += used;
+                    break;
+                }
+                if (tr > maxTry) {
+                    hs::set_error(HS_ECONV, "hs_solver_run: RK12 did not converge");
+                    *dtio = dt;
+                    if (stats) *stats = st;
+                    return HS_ECONV;
+                }
+                tr++;
+                st.rejected++;
+            }
+            std::swap(s->y0, s->acc);
+            std::swap(s->mA, s->mC);
+            s->b0_valid = false;
+            st.last_err = err;
+        } else {
+            hs::set_error(HS_EINVAL, "hs_solver_run: unknown method");
+            return HS_EINVAL;
+        }
+        st.steps++;
+    }
+    *dtio = dt;
+    st.dt = dt;
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_run");
+    if (stats) *stats = st;
+    return hs_last_error();
+}

@@ -1,0 +1,162 @@
+/*
+ * hotspin_amd.h -- entry points of libhotspin_amd.so beyond the legacy
+ * libhotspin ABI (libhotspin.h).  Plain C: pointers, sizes, scalars.
+ *
+ * They cover what the reference does outside libhotspin today and what the
+ * MI355X path adds:
+ *   - error reporting (the reference aborts or drops launch errors);
+ *   - the demag plan (replaces FFTPlanX + cuFFT and MaxwellPlan.LoadKernel,
+ *     src/hotspin-core/gpu/fftplanX.go:36-104,
+ *     src/hotspin-core/modules/maxwell_convolution.go:42-223);
+ *   - counter-based thermal noise (replaces the cuRAND Philox host generator,
+ *     src/hotspin-core/modules/mod_brown.go:98-122);
+ *   - the resident LLB-r solver: FP32 fields, FP64 state and FP64
+ *     reductions, hand-written HIP stages (replaces the engine's per-step
+ *     updater walk, src/hotspin-core/engine/solve_rk12.go:91-214 and
+ *     src/hotspin-core/engine/quant.go:417-445).
+ */
+#ifndef HOTSPIN_AMD_H
+#define HOTSPIN_AMD_H
+
+#include <stdint.h>
+
+#include "libhotspin.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+/* 0 = no error since the last hs_clear_error(); otherwise a hipError_t code
+ * or one of the HS_E* codes below.  Sticky: the first error is kept. */
+#define HS_EINVAL 10001 /* invalid argument (sizes, NULL where not allowed) */
+#define HS_ESIZE 10002  /* FFT length not supported (needs 2^a 3^b 5^c 7^d) */
+#define HS_ECONV 10003  /* adaptive step did not converge within 10 retries */
+#define HS_ENOMEM 10004
+HS_API int hs_last_error(void);
+HS_API const char* hs_last_error_string(void);
+HS_API void hs_clear_error(void);
+HS_API const char* hs_version(void);
+
+/* ---------------------------------------------------------------- demag plan */
+typedef struct hs_demag hs_demag;
+/* Build the demag plan of an N0 x N1 x N2 mesh (internal order), cell sizes
+ * (c0,c1,c2) in metres, periodic image counts pbc (0 = open, zero padded):
+ * Kernel_Arne (kernel_magnetostatic.go:25-241) integrated on the GPU in FP64,
+ * transformed in FP64 by the library's own FFT, real octant kept, scaled by
+ * 1/(P0 P1 P2) and stored in FP64 and FP32.  accuracy = 6 in the reference
+ * (maxwell_convolution.go:96).  Blocks until the kernel is ready.
+ * Returns NULL on error (hs_last_error). */
+HS_API hs_demag* hs_demag_create(const int N[3], const double cell[3], const int pbc[3], int accuracy,
+                                 CUstream stream);
+HS_API void hs_demag_destroy(hs_demag* d);
+/* P = logic (padded) size, K = kernel octant size (P0/2+1, P1/2+1, P2/2+1) */
+HS_API int hs_demag_sizes(const hs_demag* d, int P[3], int K[3]);
+/* Copy the FP64 kernel octants (6 planes XX,YY,ZZ,YZ,XZ,XY of K0*K1*K2) to host. */
+HS_API int hs_demag_kernel_f64(const hs_demag* d, double* host_out);
+/* max |imag| / max |real| of the transformed kernel (the reference panics
+ * above 1e-12, maxwell_convolution.go:383-385) */
+HS_API double hs_demag_max_imag_ratio(const hs_demag* d);
+/* B = mu0 * msat0T0Mul * (K conv (m .* msat0T0)), maxwell_convolution.go:103-131.
+ * m, B: 3 device planes of N cells (m and B may not alias); msat0T0: device
+ * mask or NULL.  FP64 variant: FP64 FFT passes; FP32 variant: FP32 passes. */
+HS_API int hs_demag_field_f64(hs_demag* d, const double* m, const double* msat0T0, double msat0T0Mul,
+                              double* B, CUstream stream);
+HS_API int hs_demag_field_f32(hs_demag* d, const float* m, const float* msat0T0, double msat0T0Mul,
+                              float* B, CUstream stream);
+
+/* ---------------------------------------------------------------- thermal noise */
+/* Philox4x32-10 (Salmon et al., SC'11) keyed by seed, counter
+ * (cell, step, attempt, stream); Box-Muller on 24-bit uniforms
+ * u = ((x>>8)+0.5)/2^24.  xi: 3 device planes of N standard normals
+ * (component c from Box-Muller pair c/2).  Replaces
+ * curandGenerateNormalDouble(3N) of mod_brown.go:122. */
+HS_API void hs_thermal_normals_f64(double* xi, int N, uint64_t seed, uint32_t stream, uint32_t step,
+                                   uint32_t attempt, CUstream s);
+/* raw Philox words for n counters (device arrays, 4 words each) */
+HS_API void hs_philox4x32_10(uint32_t* out, const uint32_t* ctr, int n, uint64_t key, CUstream s);
+
+/* ---------------------------------------------------------------- LLB-r solver */
+/* Module composition of the reference's LLB-r recipes (examples/p2f.py,
+ * tests/stdproblem4-llbr-modular.py): H_eff = H_ex + B/mu0 + B_ext/mu0 + H_lf
+ * + H_therm, dm/dt = gamma_LL (m x H_eff + alpha .* H_eff), optional
+ * normalizer (LLG limit).  Host pointers; masks may be NULL (uniform). */
+typedef struct {
+    int N[3];
+    double cell[3];
+    int pbc[3];
+    double msat0T0Mul;
+    const double* msat0T0;
+    int exchange;
+    double lexMul;
+    const double* lex;
+    int demag;
+    int demag_accuracy;
+    int zeeman;
+    double Bext[3];
+    int longfield;
+    double JMul, nMul, TcMul, TMul;
+    const double *J, *n, *Tc, *T;
+    int thermal;
+    uint64_t seed;
+    uint32_t stream;
+    double cutoff_dt;
+    double gammaLL;
+    int torque;
+    int local00nc;
+    double alphaMul[3];
+    const double* alpha[3];
+    int normalize;
+} hs_llb_params;
+
+#define HS_EULER 0
+#define HS_HEUN 1
+#define HS_RK4 2
+#define HS_RK12 3
+#define HS_RK23 4
+
+typedef struct {
+    long steps;          /* accepted steps */
+    long rejected;       /* retried attempts */
+    long rhs_evals;      /* field evaluations (demag convolutions) */
+    double t;            /* simulated time advanced */
+    double last_err;     /* error estimate of the last accepted step */
+    double dt;           /* dt for the next step */
+} hs_run_stats;
+
+typedef struct hs_solver hs_solver;
+HS_API hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream);
+HS_API void hs_solver_destroy(hs_solver* s);
+/* state m: 3 planes of N doubles on the host (internal order) */
+HS_API int hs_solver_set_m(hs_solver* s, const double* m_host);
+HS_API int hs_solver_get_m(hs_solver* s, double* m_host);
+/* Advance nsteps accepted steps.  Fixed-step methods use *dt; adaptive
+ * methods (RK12, RK23) update *dt with the reference's controller
+ * (solve_rk12.go:121-210).  step0 is the global step counter of the first
+ * step (it keys the thermal noise).  stats may be NULL. */
+HS_API int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, double* dt, double maxErr,
+                         double mindt, double maxdt, hs_run_stats* stats);
+/* One RHS evaluation at the current state (no step): k and H_eff to host. */
+HS_API int hs_solver_rhs(hs_solver* s, double* k_host, double* heff_host, uint32_t step, uint32_t attempt,
+                         double dt);
+HS_API CUstream hs_solver_stream(hs_solver* s);
+
+/* ---------------------------------------------------------------- timing */
+/* Per-kernel durations from HIP events recorded on the launching stream
+ * around each instrumented launch (off by default).  Categories: */
+#define HS_PROF_F1 0    /* demag R2C rows (axis 2) */
+#define HS_PROF_F2 1    /* demag forward columns (axis 1) */
+#define HS_PROF_F3K 2   /* demag axis-0 FFT + kernel product + inverse */
+#define HS_PROF_I2 3    /* demag inverse columns (axis 1) */
+#define HS_PROF_I1 4    /* demag C2R rows (axis 2) */
+#define HS_PROF_STAGE 5 /* fused LLB-r field + torque + RK stage kernel */
+#define HS_PROF_NCAT 8
+/* on != 0: start recording (clears earlier records); 0: stop */
+HS_API void hs_prof_enable(int on);
+/* synchronises the device; total milliseconds and launch counts per category */
+HS_API int hs_prof_read(double* total_ms, long* count, int ncat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOTSPIN_AMD_H */

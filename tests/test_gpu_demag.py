@@ -1,0 +1,88 @@
+"""Demag plan on the GPU vs the FP64 oracle and the reference's golden dumps.
+
+- Kernel octant (Kernel_Arne on the device in FP64, own FFT): relative 1e-12.
+- FP64 convolution: relative 1e-12 of max|B|.
+- FP32 convolution: relative 2e-6 of max|B| (FP32 FFT round-off).
+- Golden 64^3 (tests/test-demag/cmp-ref.py): mean over cells of sum_c |dB_c|
+  <= 1e-6 T, the reference's own criterion, on the committed samples."""
+import os
+
+import numpy as np
+import pytest
+
+from hotspin_amd.api import Demag
+from hotspin_amd._lib import check
+from oracle import pyoracle as po
+from tests.golden import regen
+from tests.gpu_util import P, dev, empty, host, relerr, require_gpu, torch
+
+pytestmark = pytest.mark.gpu
+FIX = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_dumps.npz"))
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    require_gpu()
+    yield
+    check("demag")
+
+
+CASES = [
+    ((4, 6, 8), (2e-9, 3e-9, 2.5e-9), (0, 0, 0)),
+    ((1, 32, 128), (3e-9, 125e-9 / 32, 500e-9 / 128), (0, 0, 0)),   # SP4-like film
+    ((8, 48, 96), (3e-9, 3e-9, 3e-9), (0, 0, 0)),                    # radix-3 sizes
+    ((5, 10, 20), (2e-9, 2e-9, 2e-9), (0, 0, 0)),                    # radix-5 sizes
+    ((4, 8, 16), (1e-9, 2e-9, 4e-9), (2, 0, 3)),                     # periodic images
+    ((3, 16, 10), (1e-9, 1e-9, 1e-9), (0, 1, 0)),
+]
+
+
+@pytest.mark.parametrize("N,cell,pbc", CASES)
+def test_kernel_octant_matches_oracle(N, cell, pbc):
+    d = Demag(N, cell, pbc)
+    o = po.Demag(N, cell, pbc)
+    assert d.P == o.P and d.K == o.K
+    got = d.kernel()
+    want = o.kernel()
+    assert relerr(got, want) <= 1e-12
+    assert d.max_imag_ratio() < 1e-12
+
+
+@pytest.mark.parametrize("N,cell,pbc", CASES)
+@pytest.mark.parametrize("masked", [False, True])
+def test_field_f64_and_f32_match_oracle(N, cell, pbc, masked):
+    rng = np.random.default_rng(sum(N) + masked)
+    n = N[0] * N[1] * N[2]
+    m = rng.standard_normal((3, n))
+    msk = rng.random(n) if masked else None
+    want = po.Demag(N, cell, pbc).B(m, msk, 8e5).reshape(3, n)
+    d = Demag(N, cell, pbc)
+    dm = dev(m)
+    dmsk = dev(msk) if masked else None
+    B = empty((3, n))
+    d.field_f64(P(dm), P(dmsk), 8e5, P(B))
+    assert relerr(host(B), want) <= 1e-12
+    m32 = dev(m, np.float32)
+    msk32 = dev(msk, np.float32) if masked else None
+    B32 = empty((3, n), torch.float32)
+    d.field_f32(P(m32), P(msk32), 8e5, P(B32))
+    assert relerr(host(B32), want) <= 2e-6
+
+
+@pytest.mark.parametrize("pbc,name", [((0, 0, 0), "B_ref"), ((2, 4, 8), "B_ref_pbc")])
+def test_golden_demag_fp32(pbc, name):
+    """tests/test-demag/cmp-ref.py through the FP32 production path."""
+    msk, m = regen.demag_inputs()
+    d = Demag((64, 64, 64), regen.CELL_INTERNAL, pbc)
+    n = 64 ** 3
+    m32 = dev(m.reshape(3, n), np.float32)
+    msk32 = dev(msk.reshape(n), np.float32)
+    B = empty((3, n), torch.float32)
+    d.field_f32(P(m32), P(msk32), 800e3, P(B))
+    Bh = host(B).astype(np.float64)
+    idx = FIX[name + "_idx"]
+    ref = FIX[name + "_val"].astype(np.float64)
+    per_cell = np.abs(Bh[:, idx] - ref).sum(axis=0)
+    assert per_cell.mean() <= 1e-6, per_cell.mean()
+    dsum = np.abs(Bh.sum(axis=1) - FIX[name + "_sum"]) / FIX[name + "_sumabs"]
+    assert dsum.max() <= 2e-6

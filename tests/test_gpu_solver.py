@@ -1,0 +1,130 @@
+"""Resident FP32-field / FP64-state solver on the GPU vs the FP64 oracle.
+
+Same module composition and inputs on both sides; the thermal noise is the
+same Philox stream.  Bars: RHS relative 2e-5 of max|H| / max|k| (FP32
+fields); trajectories max|dm| <= 1e-5 (the drop-in target)."""
+import math
+
+import numpy as np
+import pytest
+
+from hotspin_amd.api import Solver
+from hotspin_amd._lib import check
+from oracle import pyoracle as po
+from tests.gpu_util import relerr, require_gpu
+
+pytestmark = pytest.mark.gpu
+
+MU0 = 4e-7 * math.pi
+GAMMA0 = 2.211e5
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    require_gpu()
+    yield
+    check("solver")
+
+
+def llb_problem(N, cell, seed=1, thermal=True, pbc=(0, 0, 0), masks=False, T=300.0):
+    """LLB-r composition of examples/p2f.py: exchange6, demag, mfa/longfield,
+    temperature/brown, llbar torque + local00nc damping."""
+    Ms0, Tc, A, lam, J = 1.7e6, 1044.0, 2.1e-11, 0.01, 0.5
+    n_spin = Ms0 / (2.0 * 9.27400968202020202e-24 * J)
+    lex = math.sqrt(2 * A / (MU0 * Ms0 * Ms0))
+    ncell = N[0] * N[1] * N[2]
+    rng = np.random.default_rng(seed)
+    kw = dict(msat0T0Mul=Ms0, exchange=1, lexMul=lex, demag=1, longfield=1, JMul=J, nMul=n_spin, TcMul=Tc,
+              TMul=T, thermal=1 if thermal else 0, seed=1234, stream=0, cutoff_dt=0.0, gammaLL=GAMMA0,
+              torque=1, local00nc=1, alphaMul=(lam, lam, lam), zeeman=1, Bext=(0.01, -0.02, 0.05))
+    if masks:
+        kw["msat0T0"] = 0.5 + 0.5 * rng.random(ncell)
+        kw["lex"] = 0.8 + 0.4 * rng.random(ncell)
+        kw["T"] = 0.9 + 0.2 * rng.random(ncell)
+    me = 0.8
+    m = rng.standard_normal((3, ncell))
+    m *= me / np.linalg.norm(m, axis=0)
+    return kw, m
+
+
+def llg_problem(N, cell, seed=2):
+    """LLG limit of the reference's SP4 recipe (tests/stdproblem4-llbr-modular.py)."""
+    Ms0, A, alpha = 8.0e5, 1.3e-11, 0.02
+    lex = math.sqrt(2 * A / (MU0 * Ms0 * Ms0))
+    kw = dict(msat0T0Mul=Ms0, exchange=1, lexMul=lex, demag=1, zeeman=1, Bext=(0.0, 0.0043, -0.0246),
+              gammaLL=GAMMA0 / (1 + alpha ** 2), torque=1, local00nc=1, alphaMul=(alpha,) * 3, normalize=1)
+    ncell = N[0] * N[1] * N[2]
+    rng = np.random.default_rng(seed)
+    m = np.zeros((3, ncell))
+    m[2] = 1.0
+    m += 0.2 * rng.standard_normal((3, ncell))
+    m /= np.linalg.norm(m, axis=0)
+    return kw, m
+
+
+def _normalized(m):
+    return m / np.linalg.norm(m, axis=0)
+
+
+@pytest.mark.parametrize("masks", [False, True])
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (0, 1, 1)])
+def test_rhs_matches_oracle(masks, pbc):
+    N, cell = (4, 12, 70), (2e-9, 2e-9, 2e-9)
+    kw, m = llb_problem(N, cell, masks=masks, pbc=pbc)
+    o = po.LLB(N, cell, pbc, **kw)
+    mo, ko = o.rhs(m, step=5, attempt=1, dt=1e-15)
+    Ho = o.heff()
+    g = Solver(N, cell, pbc, **kw)
+    g.set_m(m)
+    kg, Hg = g.rhs(step=5, attempt=1, dt=1e-15)
+    assert relerr(Hg, Ho) <= 2e-5
+    assert relerr(kg, ko) <= 2e-5
+
+
+@pytest.mark.parametrize("method,steps,dt", [("rk4", 200, 1e-13), ("rk12", 100, 1e-14), ("heun", 100, 5e-14),
+                                             ("euler", 100, 1e-14), ("rk23", 100, 1e-14)])
+def test_llg_trajectory_matches_oracle(method, steps, dt):
+    N, cell = (1, 16, 64), (3e-9, 3.90625e-9, 3.90625e-9)
+    kw, m = llg_problem(N, cell)
+    o = po.LLB(N, cell, **kw)
+    mo = m.copy()
+    dto = dt
+    for i in range(steps):
+        dto, _ = o.step(method, mo, dto, i, maxErr=1e-4, mindt=1e-16, maxdt=1e-12)
+    g = Solver(N, cell, **kw)
+    g.set_m(m)
+    dtg, st = g.run(method, steps, dt, maxErr=1e-4, mindt=1e-16, maxdt=1e-12)
+    mg = g.get_m()
+    assert np.abs(_normalized(mg) - _normalized(mo)).max() <= 1e-5
+    if method in ("rk12", "rk23"):
+        assert abs(dtg - dto) <= 1e-6 * dto
+
+
+@pytest.mark.parametrize("thermal", [False, True])
+def test_llb_rk4_trajectory_matches_oracle(thermal):
+    N, cell = (2, 16, 32), (2e-9, 2e-9, 2e-9)
+    kw, m = llb_problem(N, cell, thermal=thermal)
+    o = po.LLB(N, cell, **kw)
+    mo = m.copy()
+    for i in range(50):
+        o.step("rk4", mo, 2e-15, i)
+    g = Solver(N, cell, **kw)
+    g.set_m(m)
+    g.run("rk4", 50, 2e-15)
+    mg = g.get_m()
+    assert np.abs(mg - mo).max() <= 1e-5
+
+
+def test_llb_rk23_adaptive_matches_oracle():
+    N, cell = (2, 16, 32), (2e-9, 2e-9, 2e-9)
+    kw, m = llb_problem(N, cell, thermal=False, masks=True)
+    o = po.LLB(N, cell, **kw)
+    mo = m.copy()
+    dto = 1e-15
+    for i in range(40):
+        dto, _ = o.step("rk23", mo, dto, i, maxErr=1e-5, mindt=1e-18, maxdt=1e-12)
+    g = Solver(N, cell, **kw)
+    g.set_m(m)
+    dtg, st = g.run("rk23", 40, 1e-15, maxErr=1e-5, mindt=1e-18, maxdt=1e-12)
+    assert np.abs(g.get_m() - mo).max() <= 1e-5
+    assert abs(dtg / dto - 1) <= 1e-3
