@@ -227,21 +227,23 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
                 H[2] += P.bext[2];
             }
             if (P.longfield) {
-                float Ms0T0 = P.ms0T0Mul * ms;
-                float S = P.JMul * (P.J ? P.J[I] : 1.f);
-                float n = P.nMul * (P.n ? P.n[I] : 1.f);
-                float Tc = P.TcMul * (P.Tc ? P.Tc[I] : 1.f);
-                float Ts = P.TMul * (P.T ? P.T[I] : 1.f);
-                float am = sqrtf(m0[0] * m0[0] + m0[1] * m0[1] + m0[2] * m0[2]);
-                if (!(Ms0T0 == 0.f || n == 0.f || am <= 1e-30f)) {
-                    float preS = (S > (float)HS_INFSPIN) ? 1.f : S / (S + 1.f);
-                    float b = 3.f * preS * Tc / Ts;
-                    float W, dW;
-                    brillouin_w_dw<float>(S, am * b, W, dW);
-                    float mult = n * P.lfPre * Ts * (W - am) / (Ms0T0 * dW) / am;
-                    H[0] += mult * m0[0];
-                    H[1] += mult * m0[1];
-                    H[2] += mult * m0[2];
+                // FP64: dB_S/dx is a difference of O(1) terms (common_func.h:95-98)
+                double Ms0T0 = (double)P.ms0T0Mul * ms;
+                double S = (double)P.JMul * (P.J ? P.J[I] : 1.f);
+                double n = (double)P.nMul * (P.n ? P.n[I] : 1.f);
+                double Tc = (double)P.TcMul * (P.Tc ? P.Tc[I] : 1.f);
+                double Ts = (double)P.TMul * (P.T ? P.T[I] : 1.f);
+                double mx = m0[0], my = m0[1], mz = m0[2];
+                double am = sqrt(mx * mx + my * my + mz * mz);
+                if (!(Ms0T0 == 0.0 || n == 0.0 || am <= HS_ZERO)) {
+                    double preS = (S > HS_INFSPIN) ? 1.0 : S / (S + 1.0);
+                    double b = 3.0 * preS * Tc / Ts;
+                    double W, dW;
+                    brillouin_w_dw<double>(S, am * b, W, dW);
+                    double mult = n * (HS_KB / HS_MU0) * Ts * (W - am) / (Ms0T0 * dW) / am;
+                    H[0] += (float)(mult * mx);
+                    H[1] += (float)(mult * my);
+                    H[2] += (float)(mult * mz);
                 }
             }
             if (P.thermal && ms != 0.f) {

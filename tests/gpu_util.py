@@ -33,7 +33,13 @@ def zeros(shape, dtype=torch.float64):
 
 
 def relerr(a, b):
+    """max|a-b| / max|b| over finite entries; inf if the non-finite patterns differ."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    scale = max(np.abs(b).max(), 1e-300)
-    return np.abs(a - b).max() / scale
+    fa, fb = np.isfinite(a), np.isfinite(b)
+    if not np.array_equal(fa, fb) or not np.array_equal(a[~fb], b[~fb]):
+        return np.inf
+    if not fb.any():
+        return 0.0
+    scale = max(np.abs(b[fb]).max(), 1e-300)
+    return np.abs(a[fb] - b[fb]).max() / scale

@@ -83,6 +83,15 @@ def test_golden_demag_fp32(pbc, name):
     idx = FIX[name + "_idx"]
     ref = FIX[name + "_val"].astype(np.float64)
     per_cell = np.abs(Bh[:, idx] - ref).sum(axis=0)
-    assert per_cell.mean() <= 1e-6, per_cell.mean()
     dsum = np.abs(Bh.sum(axis=1) - FIX[name + "_sum"]) / FIX[name + "_sumabs"]
-    assert dsum.max() <= 2e-6
+    if pbc == (0, 0, 0):
+        assert per_cell.mean() <= 1e-6, per_cell.mean()
+        assert dsum.max() <= 2e-6, dsum
+    else:
+        # the PBC fixture differs from the FP64 oracle by a uniform ~3e-7 T per
+        # component (oracle: 9.3e-7 T/cell); FP32 round-off adds on top of it
+        assert per_cell.mean() <= 1.2e-6, per_cell.mean()
+        assert dsum.max() <= 2e-5, dsum
+    # and the FP32 field against the FP64 oracle on the same inputs
+    want = po.Demag((64, 64, 64), regen.CELL_INTERNAL, pbc).B(m, msk, 800e3).reshape(3, n)
+    assert relerr(Bh, want) <= 2e-6
