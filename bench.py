@@ -97,11 +97,11 @@ def demag_pass_bytes(N, P):
 
 def stage_bytes_rk23(ncell, normalize=False):
     """Algorithmic bytes per cell of the four fused RK23 stage kernels (FP32
-    m, B, e; FP64 y0, acc)."""
-    b = [12 + 12 + 24 + 24 + 12 + 12 + (24 if normalize else 0),   # m, B, y0 in; acc, e, m' out (+y0 out)
-         12 + 12 + 24 + 24 + 12 + 24 + 12 + 12,                   # m, B, y0, acc, e in; acc, e, m' out
-         12 + 12 + 24 + 12 + 24 + 12 + 12,                        # m, B, acc, e in; acc, e, m' out
-         12 + 12 + 12 + (48 if normalize else 0)]                 # m, B, e in (+acc rw)
+    m, B, k, e; FP64 y0, y1)."""
+    b = [12 + 12 + 24 + 12 + 12 + (24 if normalize else 0),       # m, B, y0 in; k1, m' out (+y0 out)
+         12 + 12 + 24 + 12 + 12,                                  # m, B, y0 in; k2, m' out
+         12 + 12 + 24 + 12 + 12 + 24 + 12 + 12,                   # m, B, y0, k1, k2 in; y1, e, m' out
+         12 + 12 + 12 + (48 if normalize else 0)]                 # m, B, e in (+y1 rw)
     return [x * ncell for x in b]
 
 
@@ -124,6 +124,23 @@ def cpu_baseline(w, kw, m, threads, nsteps=1):
                       "kernel set-up %.1f s excluded)" % (nsteps, el, setup)}
 
 
+def copy_calibration():
+    """A plain copy of known size through the library: 2^28 doubles (2 GiB)
+    read and 2 GiB written per launch, 5 launches."""
+    import torch
+    from hotspin_amd import _lib
+    L = _lib.lib()
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    D = (1, 1 << 14, 1 << 14)
+    for _ in range(5):
+        L.copyUnPad3DAsync(b.data_ptr(), D[0], D[1], D[2], a.data_ptr(), D[0], D[1], D[2], 1, None)
+    torch.cuda.synchronize()
+    _lib.check("copy calibration")
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,11 +148,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 2 GiB (copyUnPad3DAsync with equal extents), "
+                         "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs each solve on a single MI355X "
               "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    import ctypes as C
 
     import torch  # noqa: F401  (device initialisation, allocator shared with the library)
     from hotspin_amd import _lib
@@ -150,6 +175,11 @@ def main():
     s.set_m(m)
     torch.cuda.synchronize()
     log("bench: set-up %.2f s" % (time.time() - t0))
+    tiles = (C.c_int * 6)()
+    _lib.lib().hs_demag_tiles(_lib.lib().hs_solver_demag(s._h), tiles)
+    log("bench: demag tiles (W,NT) rows %s cols %s z %s" % (tuple(tiles[0:2]), tuple(tiles[2:4]), tuple(tiles[4:6])))
+    stage_variant = "flat" if _lib.lib().hs_solver_stage_variant(s._h) == 1 else "march"
+    log("bench: stage kernel variant %s" % stage_variant)
     dt = 1e-15
     dt, st = s.run("rk23", args.warmup, dt, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12)
     log("bench: warmup done, dt=%.3e rejected=%d" % (dt, st.rejected))
@@ -163,7 +193,6 @@ def main():
                                                                             st.rhs_evals, dt))
     # per-kernel durations from HIP events on the solver's stream (separate short run)
     L = _lib.lib()
-    import ctypes as C
     ncat = 8
     L.hs_prof_enable(1)
     nprof = 5
@@ -204,6 +233,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w["name"], "cells": ncell, "method": "rk23", "rejected": st.rejected,
+                   "demag_tiles": {"rows": list(tiles[0:2]), "cols": list(tiles[2:4]), "z": list(tiles[4:6])},
+                   "stage_kernel": stage_variant,
                    "state": "f64", "fields": "f32", "reductions": "f64", "final_dt": dt},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom_bytes},

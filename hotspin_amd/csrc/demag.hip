@@ -25,225 +25,16 @@
 #include <map>
 #include <vector>
 
-#include "fft.hpp"
 #include "hs_common.hpp"
 
 #define HS_PI_D 3.141592653589793
 
-// ------------------------------------------------------------------ tiling
-template <typename T, int L>
-constexpr int col_w() {  // adjacent columns per block: <= 128 KB of LDS
-    int w = 16;
-    while (w > 1 && (long)L * w * (long)sizeof(cpx<T>) > 131072) w /= 2;
-    return w;
-}
-constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-template <typename T, int L>
-constexpr int col_nt() {
-    return clampi(L * col_w<T, L>() / 16, 64, 1024);
-}
-template <int M>
-constexpr int row_w() {
-    return clampi(4096 / (M > 0 ? M : 1), 1, 16);
-}
-template <int M>
-constexpr int row_nt() {
-    return clampi(M * row_w<M>() / 8, 64, 512);
-}
-constexpr int ZW = 16;  // columns per block of the axis-0 convolution pass
-template <int L>
-constexpr int z_nt() {
-    return clampi(L * 3 * ZW / 16, 64, 1024);
-}
+#include "demag_kernels.hpp"
+#include "demag_tune.hpp"
 
-struct Geo {  // pencil a starts at (a / A2) * S1 + (a % A2) * S2; element t at + t * ts
-    long S1, S2, ts;
-    int A2;
-};
-__device__ __forceinline__ long geo_base(const Geo& g, int a) { return (long)(a / g.A2) * g.S1 + (long)(a % g.A2) * g.S2; }
-
-// ------------------------------------------------------------------ F1: R2C rows
-// x: rows of n2 reals (row r at x + r*n2), optional mask of the same layout
-// (applied per cell, rows taken modulo mrows), out: rows of H2 complex at
-// stride H2p.  z[n] = x[2n] + i x[2n+1]; X[k] = E + w^k O with
-// E = (Z[k] + conj Z[M-k])/2, O = -i (Z[k] - conj Z[M-k])/2, w = exp(-2 pi i/P2).
-template <typename T, typename TI, int M>
-__global__ void __launch_bounds__(row_nt<M>()) k_r2c_rows(const TI* __restrict__ x, const TI* __restrict__ msk,
-                                                         long mrows, cpx<T>* __restrict__ out, long nrows, int n2,
-                                                         int H2p, const cpx<T>* __restrict__ twM,
-                                                         const cpx<T>* __restrict__ twP) {
-    constexpr int W = row_w<M>();
-    constexpr int NT = row_nt<M>();
-    __shared__ cpx<T> lds[M * W];
-    const int tid = threadIdx.x;
-    long row0 = (long)blockIdx.x * W;
-    for (int e = tid; e < M * W; e += NT) {
-        int p = e / M, n = e % M;
-        long r = row0 + p;
-        cpx<T> v = {T(0), T(0)};
-        if (r < nrows) {
-            long base = r * n2;
-            long mb = (r % mrows) * n2;
-            int i0 = 2 * n, i1 = 2 * n + 1;
-            if (i0 < n2) v.x = T(x[base + i0]) * (msk ? T(msk[mb + i0]) : T(1));
-            if (i1 < n2) v.y = T(x[base + i1]) * (msk ? T(msk[mb + i1]) : T(1));
-        }
-        lds[n * W + p] = v;
-    }
-    __syncthreads();
-    fft_lds<T, M, W, NT, -1>(lds, twM, tid);
-    for (int e = tid; e < (M + 1) * W; e += NT) {
-        int p = e / (M + 1), k = e % (M + 1);
-        long r = row0 + p;
-        if (r >= nrows) continue;
-        cpx<T> zk = lds[(k % M) * W + p];
-        cpx<T> zm = lds[((M - k) % M) * W + p];
-        cpx<T> E = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-        cpx<T> O = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
-        cpx<T> w = twP[k];
-        out[r * H2p + k] = cadd(E, cmul(w, O));
-    }
-}
-
-// ------------------------------------------------------------------ I1: C2R rows
-// in: rows of H2 = M+1 complex (stride H2p), out: rows of n2 reals keeping
-// the first n2 of P2 = 2M.  Z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]),
-// z = IFFT_M(Z), x[2n] = Re z[n], x[2n+1] = Im z[n] (unnormalised C2R).
-template <typename T, typename TO, int M>
-__global__ void __launch_bounds__(row_nt<M>()) k_c2r_rows(const cpx<T>* __restrict__ in, TO* __restrict__ x,
-                                                         long nrows, int n2, int H2p, const cpx<T>* __restrict__ twM,
-                                                         const cpx<T>* __restrict__ twP) {
-    constexpr int W = row_w<M>();
-    constexpr int NT = row_nt<M>();
-    __shared__ cpx<T> lds[(M + 1) * W];
-    const int tid = threadIdx.x;
-    long row0 = (long)blockIdx.x * W;
-    for (int e = tid; e < (M + 1) * W; e += NT) {
-        int p = e / (M + 1), k = e % (M + 1);
-        long r = row0 + p;
-        cpx<T> v = {T(0), T(0)};
-        if (r < nrows) v = in[r * H2p + k];
-        lds[k * W + p] = v;
-    }
-    __syncthreads();
-    constexpr int PER = (M * W + NT - 1) / NT;
-    cpx<T> z[PER];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int e = tid + q * NT;
-        if (e < M * W) {
-            int k = e / W, p = e % W;
-            cpx<T> a = lds[k * W + p];
-            cpx<T> b = lds[(M - k) * W + p];
-            cpx<T> s = {a.x + b.x, a.y - b.y};      // X[k] + conj X[M-k]
-            cpx<T> d = {a.x - b.x, a.y + b.y};      // X[k] - conj X[M-k]
-            cpx<T> w = twP[k];
-            w.y = -w.y;                            // w^-k
-            cpx<T> wd = cmul(w, d);
-            z[q] = {s.x - wd.y, s.y + wd.x};       // s + i w^-k d
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int e = tid + q * NT;
-        if (e < M * W) lds[e] = z[q];
-    }
-    __syncthreads();
-    fft_lds<T, M, W, NT, +1>(lds, twM, tid);
-    for (int e = tid; e < M * W; e += NT) {
-        int p = e / M, n = e % M;
-        long r = row0 + p;
-        if (r >= nrows) continue;
-        cpx<T> v = lds[n * W + p];
-        int i0 = 2 * n, i1 = 2 * n + 1;
-        if (i0 < n2) x[r * n2 + i0] = TO(v.x);
-        if (i1 < n2) x[r * n2 + i1] = TO(v.y);
-    }
-}
-
-// ------------------------------------------------------------------ F2 / I2: columns
-// Pencil a (blockIdx.y), columns col0..col0+W (blockIdx.x); input rows t < nin
-// (zero beyond), output rows t < nkeep.  in and out may be the same buffer.
-template <typename T, int L, int S>
-__global__ void __launch_bounds__((col_nt<T, L>())) k_cols(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin,
-                                                        int nkeep, int ncol, const cpx<T>* __restrict__ tw) {
-    constexpr int W = col_w<T, L>();
-    constexpr int NT = col_nt<T, L>();
-    __shared__ cpx<T> lds[L * W];
-    const int tid = threadIdx.x;
-    const int a = blockIdx.y;
-    const int col0 = blockIdx.x * W;
-    const long bi = geo_base(gi, a) + col0;
-    const long bo = geo_base(go, a) + col0;
-    for (int e = tid; e < L * W; e += NT) {
-        int t = e / W, p = e % W;
-        cpx<T> v = {T(0), T(0)};
-        if (t < nin && col0 + p < ncol) v = in[bi + t * gi.ts + p];
-        lds[e] = v;
-    }
-    __syncthreads();
-    fft_lds<T, L, W, NT, S>(lds, tw, tid);
-    for (int e = tid; e < nkeep * W; e += NT) {
-        int t = e / W, p = e % W;
-        if (col0 + p < ncol) out[bo + t * go.ts + p] = lds[e];
-    }
-}
-
-// ------------------------------------------------------------------ F3K: axis 0 + kernel product
-// X2 layout (3, N0, P1, H2p); block = (column tile, j'), all 3 components.
-// Kernel octants: 6 planes (XX,YY,ZZ,YZ,XZ,XY) of (K0, K1, H2).
-template <typename T, typename TK, int L>
-__global__ void __launch_bounds__(z_nt<L>()) k_zconv(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1,
-                                                     int H2, int H2p, int K1, T srcMul,
-                                                     const cpx<T>* __restrict__ tw) {
-    constexpr int W = ZW;
-    constexpr int WP = 3 * W;
-    constexpr int NT = z_nt<L>();
-    __shared__ cpx<T> lds[L * WP];
-    const int tid = threadIdx.x;
-    const int jp = blockIdx.y;
-    const int col0 = blockIdx.x * W;
-    const long cstride = (long)N0 * P1 * H2p;
-    const long tstride = (long)P1 * H2p;
-    const long base = (long)jp * H2p + col0;
-    for (int e = tid; e < L * WP; e += NT) {
-        int t = e / WP, q = e % WP;
-        int c = q / W, p = q % W;
-        cpx<T> v = {T(0), T(0)};
-        if (t < N0 && col0 + p < H2) v = X[c * cstride + base + t * tstride + p];
-        lds[e] = v;
-    }
-    __syncthreads();
-    fft_lds<T, L, WP, NT, -1>(lds, tw, tid);
-    const int jj = (jp > P1 / 2) ? P1 - jp : jp;
-    const T sj = (jp > P1 / 2) ? T(-1) : T(1);
-    const long kplane = (long)(L / 2 + 1) * K1 * H2;
-    for (int e = tid; e < L * W; e += NT) {
-        int t = e / W, p = e % W;
-        int col = col0 + p;
-        if (col >= H2) continue;
-        int ii = (t > L / 2) ? L - t : t;
-        T si = (t > L / 2) ? T(-1) : T(1);
-        long K = ((long)ii * K1 + jj) * H2 + col;
-        T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
-        T kyz = sj * T(kern[3 * kplane + K]);
-        T kxz = si * T(kern[4 * kplane + K]);
-        T kxy = si * sj * T(kern[5 * kplane + K]);
-        cpx<T> mx = cscale(lds[t * WP + p], srcMul);
-        cpx<T> my = cscale(lds[t * WP + W + p], srcMul);
-        cpx<T> mz = cscale(lds[t * WP + 2 * W + p], srcMul);
-        lds[t * WP + p] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
-        lds[t * WP + W + p] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
-        lds[t * WP + 2 * W + p] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
-    }
-    __syncthreads();
-    fft_lds<T, L, WP, NT, +1>(lds, tw, tid);
-    for (int e = tid; e < N0 * WP; e += NT) {
-        int t = e / WP, q = e % WP;
-        int c = q / W, p = q % W;
-        if (col0 + p < H2) X[c * cstride + base + t * tstride + p] = lds[e];
-    }
+hsdm::Variants& hsdm::variants(int L) {
+    static std::map<int, hsdm::Variants> reg;
+    return reg[L];
 }
 
 // ------------------------------------------------------------------ Kernel_Arne (FP64)
@@ -512,6 +303,11 @@ struct hs_demag {
     cpx<float>*twMf = nullptr, *twP2f = nullptr, *tw1f = nullptr, *tw0f = nullptr;
     cpx<double>*twMd = nullptr, *twP2d = nullptr, *tw1d = nullptr, *tw0d = nullptr;
     double max_imag_ratio = 0;
+    // FP32 pass variants chosen by timing at plan creation
+    hsdm::RowV row32{};
+    hsdm::ColV col32{};
+    hsdm::ZV z32{};
+    bool tuned = false;
 };
 
 namespace {
@@ -525,7 +321,97 @@ struct Pipe {
 
 bool sizes_ok(const hs_demag* d) {
     return (d->P[2] % 2 == 0) && fft_supported(d->M) && pick_r2c<float, float>(d->M) &&
-           pick_cols<float, -1>(d->P[1]) && pick_zconv<float, float>(d->P[0]);
+           pick_cols<float, -1>(d->P[1]) && pick_zconv<float, float>(d->P[0]) &&
+           !hsdm::variants(d->M).rows.empty() && !hsdm::variants(d->P[1]).cols.empty() &&
+           !hsdm::variants(d->P[0]).z.empty();
+}
+
+int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, float* B, hipStream_t st) {
+    const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1];
+    const int H2 = d->H2, H2p = d->H2p;
+    cpx<float>*X1 = d->X1f, *X2 = d->X2f;
+    const long nrows = 3L * N0 * N1;
+    Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
+    Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
+    hs::prof_begin(HS_PROF_F1, st);
+    d->row32.r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, d->twMf, d->twP2f, st);
+    hs::prof_end(HS_PROF_F1, st);
+    hs::prof_begin(HS_PROF_F2, st);
+    d->col32.fwd(X1, X2, g1, g2, 3 * N0, N1, P1, H2, d->tw1f, st);
+    hs::prof_end(HS_PROF_F2, st);
+    hs::prof_begin(HS_PROF_F3K, st);
+    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], (float)srcMul, d->tw0f, st);
+    hs::prof_end(HS_PROF_F3K, st);
+    hs::prof_begin(HS_PROF_I2, st);
+    d->col32.inv(X2, X1, g2, g1, 3 * N0, P1, N1, H2, d->tw1f, st);
+    hs::prof_end(HS_PROF_I2, st);
+    hs::prof_begin(HS_PROF_I1, st);
+    d->row32.c2r(X1, B, nrows, N2, H2p, d->twMf, d->twP2f, st);
+    hs::prof_end(HS_PROF_I1, st);
+    HS_LAUNCH_CHECK("hs_demag_field");
+    return hs_last_error();
+}
+
+// Time each registered FP32 variant of every pass on the plan's own buffers
+// and keep the fastest (the tile autotuner).
+void tune32(hs_demag* d, hipStream_t st) {
+    hsdm::Variants& vr = hsdm::variants(d->M);
+    hsdm::Variants& vc = hsdm::variants(d->P[1]);
+    hsdm::Variants& vz = hsdm::variants(d->P[0]);
+    d->row32 = vr.rows.empty() ? hsdm::RowV{} : vr.rows[0];
+    d->col32 = vc.cols.empty() ? hsdm::ColV{} : vc.cols[0];
+    d->z32 = vz.z.empty() ? hsdm::ZV{} : vz.z[0];
+    if (!d->row32.r2c || !d->col32.fwd || !d->z32.fn) return;
+    const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1];
+    const int H2 = d->H2, H2p = d->H2p;
+    const long nrows = 3L * N0 * N1;
+    float *m = nullptr, *B = nullptr;
+    hipMalloc(&m, sizeof(float) * 3 * d->ncell);
+    hipMalloc(&B, sizeof(float) * 3 * d->ncell);
+    if (!m || !B) {
+        hipFree(m);
+        hipFree(B);
+        return;
+    }
+    hipMemsetAsync(m, 0, sizeof(float) * 3 * d->ncell, st);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    auto timeit = [&](auto&& launch) {
+        launch();
+        hipEventRecord(e0, st);
+        for (int r = 0; r < 3; r++) launch();
+        hipEventRecord(e1, st);
+        hipEventSynchronize(e1);
+        float ms = 1e30f;
+        hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
+    Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
+    float best = 1e30f;
+    for (auto& v : vr.rows) {
+        float t = timeit([&] { v.r2c(m, nullptr, (long)N0 * N1, d->X1f, nrows, N2, H2p, d->twMf, d->twP2f, st); }) +
+                  timeit([&] { v.c2r(d->X1f, B, nrows, N2, H2p, d->twMf, d->twP2f, st); });
+        if (t < best) { best = t; d->row32 = v; }
+    }
+    best = 1e30f;
+    for (auto& v : vc.cols) {
+        float t = timeit([&] { v.fwd(d->X1f, d->X2f, g1, g2, 3 * N0, N1, P1, H2, d->tw1f, st); }) +
+                  timeit([&] { v.inv(d->X2f, d->X1f, g2, g1, 3 * N0, P1, N1, H2, d->tw1f, st); });
+        if (t < best) { best = t; d->col32 = v; }
+    }
+    best = 1e30f;
+    for (auto& v : vz.z) {
+        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], 1.0f, d->tw0f, st); });
+        if (t < best) { best = t; d->z32 = v; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamSynchronize(st);
+    hipFree(m);
+    hipFree(B);
+    d->tuned = true;
 }
 
 template <typename T>
@@ -681,6 +567,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     hipFree(full);
     hipFree(X);
     hipFree(maxbits);
+    tune32(d, st);
     HS_LAUNCH_CHECK("hs_demag_create");
     if (hs_last_error()) {
         hs_demag_destroy(d);
@@ -716,14 +603,24 @@ extern "C" int hs_demag_kernel_f64(const hs_demag* d, double* host_out) {
 
 extern "C" double hs_demag_max_imag_ratio(const hs_demag* d) { return d ? d->max_imag_ratio : -1; }
 
+extern "C" int hs_demag_tiles(const hs_demag* d, int* out6) {
+    if (!d || !out6) return HS_EINVAL;
+    out6[0] = d->row32.W;
+    out6[1] = d->row32.NT;
+    out6[2] = d->col32.W;
+    out6[3] = d->col32.NT;
+    out6[4] = d->z32.W;
+    out6[5] = d->z32.NT;
+    return 0;
+}
+
 extern "C" int hs_demag_field_f32(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
                                   CUstream stream) {
     if (!d || !m || !B) {
         hs::set_error(HS_EINVAL, "hs_demag_field_f32");
         return HS_EINVAL;
     }
-    return run_field<float>(d, m, msk, HS_MU0 * msat0T0Mul, B, hs::S(stream), d->X1f, d->X2f, d->kern32, d->twMf,
-                            d->twP2f, d->tw1f, d->tw0f);
+    return run_field32(d, m, msk, HS_MU0 * msat0T0Mul, B, hs::S(stream));
 }
 
 extern "C" int hs_demag_field_f64(hs_demag* d, const double* m, const double* msk, double msat0T0Mul, double* B,
@@ -748,7 +645,6 @@ extern "C" int hs_demag_field_f64(hs_demag* d, const double* m, const double* ms
 // planes with an FP32 mask.
 int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
                                 hipStream_t st) {
-    return run_field<float>(d, m, msk, HS_MU0 * msat0T0Mul, B, st, d->X1f, d->X2f, d->kern32, d->twMf, d->twP2f,
-                            d->tw1f, d->tw0f);
+    return run_field32(d, m, msk, HS_MU0 * msat0T0Mul, B, st);
 }
 long hs_demag_ncell(const hs_demag* d) { return d->ncell; }

@@ -1,0 +1,214 @@
+// demag_kernels.hpp -- the five FFT pass kernels of the demag convolution
+// (see demag.hip for the pipeline).  Tile shapes (pencils per block W,
+// threads NT) are template parameters so the plan can time variants.
+#pragma once
+
+#include "fft.hpp"
+
+// ------------------------------------------------------------------ tiling
+template <typename T, int L>
+constexpr int col_w() {  // adjacent columns per block: <= 128 KB of LDS
+    int w = 16;
+    while (w > 1 && (long)L * w * (long)sizeof(cpx<T>) > 131072) w /= 2;
+    return w;
+}
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+template <typename T, int L>
+constexpr int col_nt() {
+    return clampi(L * col_w<T, L>() / 16, 64, 1024);
+}
+template <int M>
+constexpr int row_w() {
+    return clampi(4096 / (M > 0 ? M : 1), 1, 16);
+}
+template <int M>
+constexpr int row_nt() {
+    return clampi(M * row_w<M>() / 8, 64, 512);
+}
+constexpr int ZW = 16;  // columns per block of the axis-0 convolution pass
+template <int L>
+constexpr int z_nt() {
+    return clampi(L * 3 * ZW / 16, 64, 1024);
+}
+
+struct Geo {  // pencil a starts at (a / A2) * S1 + (a % A2) * S2; element t at + t * ts
+    long S1, S2, ts;
+    int A2;
+};
+__device__ __forceinline__ long geo_base(const Geo& g, int a) { return (long)(a / g.A2) * g.S1 + (long)(a % g.A2) * g.S2; }
+
+// ------------------------------------------------------------------ F1: R2C rows
+// x: rows of n2 reals (row r at x + r*n2), optional mask of the same layout
+// (applied per cell, rows taken modulo mrows), out: rows of H2 complex at
+// stride H2p.  z[n] = x[2n] + i x[2n+1]; X[k] = E + w^k O with
+// E = (Z[k] + conj Z[M-k])/2, O = -i (Z[k] - conj Z[M-k])/2, w = exp(-2 pi i/P2).
+template <typename T, typename TI, int M, int W = row_w<M>(), int NT = row_nt<M>()>
+__global__ void __launch_bounds__(NT) k_r2c_rows(const TI* __restrict__ x, const TI* __restrict__ msk,
+                                                         long mrows, cpx<T>* __restrict__ out, long nrows, int n2,
+                                                         int H2p, const cpx<T>* __restrict__ twM,
+                                                         const cpx<T>* __restrict__ twP) {
+    __shared__ cpx<T> lds[M * W];
+    const int tid = threadIdx.x;
+    long row0 = (long)blockIdx.x * W;
+    for (int e = tid; e < M * W; e += NT) {
+        int p = e / M, n = e % M;
+        long r = row0 + p;
+        cpx<T> v = {T(0), T(0)};
+        if (r < nrows) {
+            long base = r * n2;
+            long mb = (r % mrows) * n2;
+            int i0 = 2 * n, i1 = 2 * n + 1;
+            if (i0 < n2) v.x = T(x[base + i0]) * (msk ? T(msk[mb + i0]) : T(1));
+            if (i1 < n2) v.y = T(x[base + i1]) * (msk ? T(msk[mb + i1]) : T(1));
+        }
+        lds[n * W + p] = v;
+    }
+    __syncthreads();
+    fft_lds<T, M, W, NT, -1>(lds, twM, tid);
+    for (int e = tid; e < (M + 1) * W; e += NT) {
+        int p = e / (M + 1), k = e % (M + 1);
+        long r = row0 + p;
+        if (r >= nrows) continue;
+        cpx<T> zk = lds[(k % M) * W + p];
+        cpx<T> zm = lds[((M - k) % M) * W + p];
+        cpx<T> E = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+        cpx<T> O = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
+        cpx<T> w = twP[k];
+        out[r * H2p + k] = cadd(E, cmul(w, O));
+    }
+}
+
+// ------------------------------------------------------------------ I1: C2R rows
+// in: rows of H2 = M+1 complex (stride H2p), out: rows of n2 reals keeping
+// the first n2 of P2 = 2M.  Z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]),
+// z = IFFT_M(Z), x[2n] = Re z[n], x[2n+1] = Im z[n] (unnormalised C2R).
+template <typename T, typename TO, int M, int W = row_w<M>(), int NT = row_nt<M>()>
+__global__ void __launch_bounds__(NT) k_c2r_rows(const cpx<T>* __restrict__ in, TO* __restrict__ x,
+                                                         long nrows, int n2, int H2p, const cpx<T>* __restrict__ twM,
+                                                         const cpx<T>* __restrict__ twP) {
+    __shared__ cpx<T> lds[(M + 1) * W];
+    const int tid = threadIdx.x;
+    long row0 = (long)blockIdx.x * W;
+    for (int e = tid; e < (M + 1) * W; e += NT) {
+        int p = e / (M + 1), k = e % (M + 1);
+        long r = row0 + p;
+        cpx<T> v = {T(0), T(0)};
+        if (r < nrows) v = in[r * H2p + k];
+        lds[k * W + p] = v;
+    }
+    __syncthreads();
+    constexpr int PER = (M * W + NT - 1) / NT;
+    cpx<T> z[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int e = tid + q * NT;
+        if (e < M * W) {
+            int k = e / W, p = e % W;
+            cpx<T> a = lds[k * W + p];
+            cpx<T> b = lds[(M - k) * W + p];
+            cpx<T> s = {a.x + b.x, a.y - b.y};      // X[k] + conj X[M-k]
+            cpx<T> d = {a.x - b.x, a.y + b.y};      // X[k] - conj X[M-k]
+            cpx<T> w = twP[k];
+            w.y = -w.y;                            // w^-k
+            cpx<T> wd = cmul(w, d);
+            z[q] = {s.x - wd.y, s.y + wd.x};       // s + i w^-k d
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        int e = tid + q * NT;
+        if (e < M * W) lds[e] = z[q];
+    }
+    __syncthreads();
+    fft_lds<T, M, W, NT, +1>(lds, twM, tid);
+    for (int e = tid; e < M * W; e += NT) {
+        int p = e / M, n = e % M;
+        long r = row0 + p;
+        if (r >= nrows) continue;
+        cpx<T> v = lds[n * W + p];
+        int i0 = 2 * n, i1 = 2 * n + 1;
+        if (i0 < n2) x[r * n2 + i0] = TO(v.x);
+        if (i1 < n2) x[r * n2 + i1] = TO(v.y);
+    }
+}
+
+// ------------------------------------------------------------------ F2 / I2: columns
+// Pencil a (blockIdx.y), columns col0..col0+W (blockIdx.x); input rows t < nin
+// (zero beyond), output rows t < nkeep.  in and out may be the same buffer.
+template <typename T, int L, int S, int W = col_w<T, L>(), int NT = col_nt<T, L>()>
+__global__ void __launch_bounds__(NT) k_cols(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
+                                             int ncol, const cpx<T>* __restrict__ tw) {
+    __shared__ cpx<T> lds[L * W];
+    const int tid = threadIdx.x;
+    const int a = blockIdx.y;
+    const int col0 = blockIdx.x * W;
+    const long bi = geo_base(gi, a) + col0;
+    const long bo = geo_base(go, a) + col0;
+    for (int e = tid; e < L * W; e += NT) {
+        int t = e / W, p = e % W;
+        cpx<T> v = {T(0), T(0)};
+        if (t < nin && col0 + p < ncol) v = in[bi + t * gi.ts + p];
+        lds[e] = v;
+    }
+    __syncthreads();
+    fft_lds<T, L, W, NT, S>(lds, tw, tid);
+    for (int e = tid; e < nkeep * W; e += NT) {
+        int t = e / W, p = e % W;
+        if (col0 + p < ncol) out[bo + t * go.ts + p] = lds[e];
+    }
+}
+
+// ------------------------------------------------------------------ F3K: axis 0 + kernel product
+// X2 layout (3, N0, P1, H2p); block = (column tile, j'), all 3 components.
+// Kernel octants: 6 planes (XX,YY,ZZ,YZ,XZ,XY) of (K0, K1, H2).
+template <typename T, typename TK, int L, int W = ZW, int NT = z_nt<L>()>
+__global__ void __launch_bounds__(NT) k_zconv(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+                                              int H2p, int K1, T srcMul, const cpx<T>* __restrict__ tw) {
+    constexpr int WP = 3 * W;
+    __shared__ cpx<T> lds[L * WP];
+    const int tid = threadIdx.x;
+    const int jp = blockIdx.y;
+    const int col0 = blockIdx.x * W;
+    const long cstride = (long)N0 * P1 * H2p;
+    const long tstride = (long)P1 * H2p;
+    const long base = (long)jp * H2p + col0;
+    for (int e = tid; e < L * WP; e += NT) {
+        int t = e / WP, q = e % WP;
+        int c = q / W, p = q % W;
+        cpx<T> v = {T(0), T(0)};
+        if (t < N0 && col0 + p < H2) v = X[c * cstride + base + t * tstride + p];
+        lds[e] = v;
+    }
+    __syncthreads();
+    fft_lds<T, L, WP, NT, -1>(lds, tw, tid);
+    const int jj = (jp > P1 / 2) ? P1 - jp : jp;
+    const T sj = (jp > P1 / 2) ? T(-1) : T(1);
+    const long kplane = (long)(L / 2 + 1) * K1 * H2;
+    for (int e = tid; e < L * W; e += NT) {
+        int t = e / W, p = e % W;
+        int col = col0 + p;
+        if (col >= H2) continue;
+        int ii = (t > L / 2) ? L - t : t;
+        T si = (t > L / 2) ? T(-1) : T(1);
+        long K = ((long)ii * K1 + jj) * H2 + col;
+        T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
+        T kyz = sj * T(kern[3 * kplane + K]);
+        T kxz = si * T(kern[4 * kplane + K]);
+        T kxy = si * sj * T(kern[5 * kplane + K]);
+        cpx<T> mx = cscale(lds[t * WP + p], srcMul);
+        cpx<T> my = cscale(lds[t * WP + W + p], srcMul);
+        cpx<T> mz = cscale(lds[t * WP + 2 * W + p], srcMul);
+        lds[t * WP + p] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
+        lds[t * WP + W + p] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
+        lds[t * WP + 2 * W + p] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
+    }
+    __syncthreads();
+    fft_lds<T, L, WP, NT, +1>(lds, tw, tid);
+    for (int e = tid; e < N0 * WP; e += NT) {
+        int t = e / WP, q = e % WP;
+        int c = q / W, p = q % W;
+        if (col0 + p < H2) X[c * cstride + base + t * tstride + p] = lds[e];
+    }
+}
+

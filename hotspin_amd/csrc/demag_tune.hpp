@@ -1,0 +1,105 @@
+// demag_tune.hpp -- registry of FP32 tile variants of the demag FFT passes.
+// Each supported length registers several (pencils per block W, threads NT)
+// instantiations; the demag plan times them on its own buffers at creation
+// and keeps the fastest per pass.
+#pragma once
+
+#include <vector>
+
+#include "demag_kernels.hpp"
+
+namespace hsdm {
+using r2c32_t = void (*)(const float*, const float*, long, cpx<float>*, long, int, int, const cpx<float>*,
+                         const cpx<float>*, hipStream_t);
+using c2r32_t = void (*)(const cpx<float>*, float*, long, int, int, const cpx<float>*, const cpx<float>*,
+                         hipStream_t);
+using cols32_t = void (*)(const cpx<float>*, cpx<float>*, Geo, Geo, int, int, int, int, const cpx<float>*,
+                          hipStream_t);
+using z32_t = void (*)(cpx<float>*, const float*, int, int, int, int, int, float, const cpx<float>*, hipStream_t);
+struct RowV {
+    int W, NT;
+    r2c32_t r2c;
+    c2r32_t c2r;
+};
+struct ColV {
+    int W, NT;
+    cols32_t fwd, inv;
+};
+struct ZV {
+    int W, NT;
+    z32_t fn;
+};
+struct Variants {
+    std::vector<RowV> rows;
+    std::vector<ColV> cols;
+    std::vector<ZV> z;
+};
+Variants& variants(int L);  // registry (defined in demag.hip)
+
+constexpr long kLdsMax = 163840;
+constexpr int round64(int v) { return (v + 63) / 64 * 64; }
+constexpr int clamp_nt(long v) { return round64((int)(v < 64 ? 64 : (v > 1024 ? 1024 : v))); }
+
+template <int M, int W, int NT>
+void launch_r2c32(const float* x, const float* msk, long mrows, cpx<float>* out, long nrows, int n2, int H2p,
+                  const cpx<float>* twM, const cpx<float>* twP, hipStream_t st) {
+    hipLaunchKernelGGL((k_r2c_rows<float, float, M, W, NT>), dim3((unsigned)((nrows + W - 1) / W)), dim3(NT), 0,
+                       st, x, msk, mrows, out, nrows, n2, H2p, twM, twP);
+}
+template <int M, int W, int NT>
+void launch_c2r32(const cpx<float>* in, float* x, long nrows, int n2, int H2p, const cpx<float>* twM,
+                  const cpx<float>* twP, hipStream_t st) {
+    hipLaunchKernelGGL((k_c2r_rows<float, float, M, W, NT>), dim3((unsigned)((nrows + W - 1) / W)), dim3(NT), 0,
+                       st, in, x, nrows, n2, H2p, twM, twP);
+}
+template <int L, int S, int W, int NT>
+void launch_cols32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int npencil, int nin, int nkeep,
+                   int ncol, const cpx<float>* tw, hipStream_t st) {
+    dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
+    hipLaunchKernelGGL((k_cols<float, L, S, W, NT>), g, dim3(NT), 0, st, in, out, gi, go, nin, nkeep, ncol, tw);
+}
+template <int L, int W, int NT>
+void launch_z32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, float srcMul,
+                const cpx<float>* tw, hipStream_t st) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)P1);
+    hipLaunchKernelGGL((k_zconv<float, float, L, W, NT>), g, dim3(NT), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul,
+                       tw);
+}
+
+template <int M, int W, int E>
+void add_row(Variants& v) {
+    constexpr int NT = clamp_nt((long)M * W / E);
+    if constexpr ((long)(M + 1) * W * 8 <= kLdsMax) v.rows.push_back({W, NT, &launch_r2c32<M, W, NT>, &launch_c2r32<M, W, NT>});
+}
+template <int L, int W, int E>
+void add_col(Variants& v) {
+    constexpr int NT = clamp_nt((long)L * W / E);
+    if constexpr ((long)L * W * 8 <= kLdsMax) v.cols.push_back({W, NT, &launch_cols32<L, -1, W, NT>, &launch_cols32<L, 1, W, NT>});
+}
+template <int L, int W, int E>
+void add_z(Variants& v) {
+    constexpr int NT = clamp_nt((long)L * 3 * W / E);
+    if constexpr (L <= 256 && (long)L * 3 * W * 8 <= kLdsMax) v.z.push_back({W, NT, &launch_z32<L, W, NT>});
+}
+
+template <int L>
+void register_size() {
+    Variants& v = variants(L);
+    add_row<L, 2, 8>(v);
+    add_row<L, 4, 8>(v);
+    add_row<L, 8, 8>(v);
+    add_row<L, 4, 4>(v);
+    add_row<L, 16, 8>(v);
+    add_col<L, 8, 8>(v);
+    add_col<L, 8, 16>(v);
+    add_col<L, 16, 16>(v);
+    add_col<L, 16, 8>(v);
+    add_col<L, 4, 8>(v);
+    add_col<L, 4, 4>(v);
+    add_z<L, 16, 12>(v);
+    add_z<L, 32, 12>(v);
+    add_z<L, 64, 12>(v);
+    add_z<L, 32, 6>(v);
+    add_z<L, 16, 6>(v);
+}
+}  // namespace hsdm

@@ -12,6 +12,7 @@
 // next stage input N(y0 + a h k) in FP32, error norm max|e| in FP64.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -29,6 +30,7 @@ constexpr int TX = 64, TY = 4;
 
 struct Phys {
     int N0, N1, N2, w0, w1, w2;
+    int chunk;  // planes marched by one block (grid.z = ceil(N0 / chunk))
     int exchange, demag, zeeman, longfield, thermal, torque, local, normalize;
     float ex0, ex1, ex2;  // lexMul^2 / c_d^2
     float ms0T0Mul;
@@ -60,7 +62,8 @@ struct StageIO {
     const float* B;   // demag field (3 planes) or null
     double* y0;       // FP64 state at step start
     double* acc;      // FP64 accumulator
-    float* e;         // FP32 error accumulator / stored k1
+    float* e;         // FP32 error partial (RK23) / stored k1 (RK12)
+    float *k1, *k2, *k3;  // stored stage slopes (RK4, RK23)
     float* mout;      // next stage input
     float* kout;      // R_RHS: k
     float* hout;      // R_RHS: H_eff
@@ -70,8 +73,279 @@ struct StageIO {
     uint32_t step, attempt;
 };
 
-__device__ __forceinline__ float clampf_idx(int v) { return (float)v; }
 
+__device__ __forceinline__ int wrapc(int c, int n, int w) {
+    if (c < 0) c = w ? c + n : 0;
+    if (c >= n) c = w ? c - n : n - 1;
+    return c;
+}
+
+// Load the (j,k) tile of plane i with its one-cell halo into LDS (the halo
+// rows/columns are wrapped or clamped as the stencil needs).
+__device__ __forceinline__ void load_tile(const Phys& P, const float* m, long N, int i, int jc, int kc, int tx, int ty,
+                                          const float own[3], float lown, float (*tile)[TY + 2][TX + 2],
+                                          float (*ltile)[TX + 2]) {
+    const int N1 = P.N1, N2 = P.N2;
+    const long N12 = (long)N1 * N2;
+    for (int c = 0; c < 3; c++) tile[c][ty + 1][tx + 1] = own[c];
+    if (P.lex) ltile[ty + 1][tx + 1] = lown;
+    if (!P.exchange) return;
+    const long base = i * N12;
+    if (ty == 0 || ty == TY - 1) {
+        int jn = (ty == 0) ? wrapc(blockIdx.y * TY - 1, N1, P.w1) : wrapc(min(blockIdx.y * TY + TY, N1), N1, P.w1);
+        int row = (ty == 0) ? 0 : TY + 1;
+        long a2 = base + (long)jn * N2 + kc;
+        for (int c = 0; c < 3; c++) tile[c][row][tx + 1] = m[c * N + a2];
+        if (P.lex) ltile[row][tx + 1] = P.lex[a2];
+    }
+    if (tx == 0 || tx == TX - 1) {
+        int kn = (tx == 0) ? wrapc(blockIdx.x * TX - 1, N2, P.w2) : wrapc(min(blockIdx.x * TX + TX, N2), N2, P.w2);
+        int col = (tx == 0) ? 0 : TX + 1;
+        long a2 = base + (long)jc * N2 + kn;
+        for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = m[c * N + a2];
+        if (P.lex) ltile[ty + 1][col] = P.lex[a2];
+    }
+}
+
+// 6-neighbour exchange field of cell (i,j,k), exchange6.cu:11-127 in FP32
+// with the differences formed first: pre_d (pre1 (m1-m0) + pre2 (m2-m0)).
+// lo/hi: planes i-1, i+1 (registers); the j,k neighbours come from the tile.
+__device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, long N, int i, int j, int k, int jc,
+                                              int kc, int tx, int ty, const float m0[3], const float lo[3],
+                                              const float hi[3], float l0, float llo, float lhi,
+                                              float (*tile)[TY + 2][TX + 2], float (*ltile)[TX + 2], float H[3]) {
+    const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
+    const long N12 = (long)N1 * N2;
+    const float l02 = l0 * l0;
+    float e0[3], e1[3], e2[3];
+    {  // axis 0 (X): planes i-1, i+1; clamped at open ends = self
+        bool lo_self = (i == 0 && !P.w0), hi_self = (i == N0 - 1 && !P.w0);
+        float l1 = lo_self ? l0 : llo, l2 = hi_self ? l0 : lhi;
+        float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+        for (int c = 0; c < 3; c++) {
+            float v1 = lo_self ? m0[c] : lo[c], v2 = hi_self ? m0[c] : hi[c];
+            e0[c] = P.ex0 * (pre1 * (v1 - m0[c]) + pre2 * (v2 - m0[c]));
+        }
+    }
+    {  // axis 2 (Z, contiguous k)
+        float v1[3], v2[3], l1, l2;
+        if (k - 1 < 0 && P.w2) {
+            long aa = i * N12 + (long)jc * N2 + (N2 - 1);
+            for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
+            l1 = P.lex ? P.lex[aa] : 1.f;
+        } else {
+            int col = (k - 1 < 0) ? tx + 1 : tx;
+            for (int c = 0; c < 3; c++) v1[c] = tile[c][ty + 1][col];
+            l1 = P.lex ? ltile[ty + 1][col] : 1.f;
+        }
+        if (k + 1 >= N2 && P.w2) {
+            long aa = i * N12 + (long)jc * N2;
+            for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
+            l2 = P.lex ? P.lex[aa] : 1.f;
+        } else {
+            int col = (k + 1 >= N2) ? tx + 1 : tx + 2;
+            for (int c = 0; c < 3; c++) v2[c] = tile[c][ty + 1][col];
+            l2 = P.lex ? ltile[ty + 1][col] : 1.f;
+        }
+        float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+        for (int c = 0; c < 3; c++) e2[c] = P.ex2 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
+    }
+    {  // axis 1 (Y)
+        float v1[3], v2[3], l1, l2;
+        if (j - 1 < 0 && P.w1) {
+            long aa = i * N12 + (long)(N1 - 1) * N2 + kc;
+            for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
+            l1 = P.lex ? P.lex[aa] : 1.f;
+        } else {
+            int row = (j - 1 < 0) ? ty + 1 : ty;
+            for (int c = 0; c < 3; c++) v1[c] = tile[c][row][tx + 1];
+            l1 = P.lex ? ltile[row][tx + 1] : 1.f;
+        }
+        if (j + 1 >= N1 && P.w1) {
+            long aa = i * N12 + kc;
+            for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
+            l2 = P.lex ? P.lex[aa] : 1.f;
+        } else {
+            int row = (j + 1 >= N1) ? ty + 1 : ty + 2;
+            for (int c = 0; c < 3; c++) v2[c] = tile[c][row][tx + 1];
+            l2 = P.lex ? ltile[row][tx + 1] : 1.f;
+        }
+        float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+        for (int c = 0; c < 3; c++) e1[c] = P.ex1 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
+    }
+    for (int c = 0; c < 3; c++) H[c] = P.ms0T0Mul * (e0[c] + e2[c] + e1[c]);
+}
+
+// Everything after the exchange term for one cell: B/mu0, B_ext/mu0, H_lf,
+// H_therm, k = gamma (m x H + alpha H) and the RK stage update (FP64 state).
+template <int ROLE>
+__device__ __forceinline__ void finish_cell(const Phys& P, const StageIO& io, long I, long N, const float m0[3],
+                                            float H[3], double& errloc) {
+    const float ms = P.msk ? P.msk[I] : 1.f;
+    if (P.demag) {
+        H[0] += P.invMu0 * io.B[I];
+        H[1] += P.invMu0 * io.B[N + I];
+        H[2] += P.invMu0 * io.B[2 * N + I];
+    }
+    if (P.zeeman) {
+        H[0] += P.bext[0];
+        H[1] += P.bext[1];
+        H[2] += P.bext[2];
+    }
+    if (P.longfield) {
+        // FP64: dB_S/dx is a difference of O(1) terms (common_func.h:95-98)
+        double Ms0T0 = (double)P.ms0T0Mul * ms;
+        double S = (double)P.JMul * (P.J ? P.J[I] : 1.f);
+        double n = (double)P.nMul * (P.n ? P.n[I] : 1.f);
+        double Tc = (double)P.TcMul * (P.Tc ? P.Tc[I] : 1.f);
+        double Ts = (double)P.TMul * (P.T ? P.T[I] : 1.f);
+        double mx = m0[0], my = m0[1], mz = m0[2];
+        double am = sqrt(mx * mx + my * my + mz * mz);
+        if (!(Ms0T0 == 0.0 || n == 0.0 || am <= HS_ZERO)) {
+            double preS = (S > HS_INFSPIN) ? 1.0 : S / (S + 1.0);
+            double b = 3.0 * preS * Tc / Ts;
+            double W, dW;
+            brillouin_w_dw<double>(S, am * b, W, dW);
+            double mult = n * (HS_KB / HS_MU0) * Ts * (W - am) / (Ms0T0 * dW) / am;
+            H[0] += (float)(mult * mx);
+            H[1] += (float)(mult * my);
+            H[2] += (float)(mult * mz);
+        }
+    }
+    if (P.thermal && ms != 0.f) {
+        float x0, x1, x2;
+        thermal_normals_f((uint32_t)I, io.step, io.attempt, P.stream, P.seed, x0, x1, x2);
+        float Ti = P.T ? P.T[I] : 1.f;
+        float pre = sqrtf((Ti * io.thK) / ms);
+        float s0 = sqrtf(P.aMul[0] * (P.alpha[0] ? P.alpha[0][I] : 1.f));
+        float s1 = sqrtf(P.aMul[1] * (P.alpha[1] ? P.alpha[1][I] : 1.f));
+        float s2 = sqrtf(P.aMul[2] * (P.alpha[2] ? P.alpha[2][I] : 1.f));
+        H[0] += pre * (s0 * x0);
+        H[1] += pre * (s1 * x1);
+        H[2] += pre * (s2 * x2);
+    }
+    float kk[3] = {0.f, 0.f, 0.f};
+    if (ms != 0.f) {
+        if (P.torque) {  // cross(H, m) = m x H
+            kk[0] = P.gamma * (-H[1] * m0[2] + H[2] * m0[1]);
+            kk[1] = P.gamma * (-H[2] * m0[0] + H[0] * m0[2]);
+            kk[2] = P.gamma * (-H[0] * m0[1] + H[1] * m0[0]);
+        }
+        if (P.local) {
+            for (int c = 0; c < 3; c++) {
+                float al = P.aMul[c] * (P.alpha[c] ? P.alpha[c][I] : 1.f);
+                kk[c] += P.gamma * (al * H[c]);
+            }
+        }
+    }
+    const double h = io.h;
+    if constexpr (ROLE == R_RHS) {
+        for (int c = 0; c < 3; c++) {
+            io.kout[c * N + I] = kk[c];
+            io.hout[c * N + I] = H[c];
+        }
+        return;
+    } else {
+        double y[3], acc[3];
+        auto nrm = [&](double* v) {
+            if (P.normalize) {
+                double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                double s = (l == 0.0) ? 0.0 : 1.0 / l;
+                v[0] *= s;
+                v[1] *= s;
+                v[2] *= s;
+            }
+        };
+        auto store_m = [&](const double* v) {
+            double w[3] = {v[0], v[1], v[2]};
+            nrm(w);
+            for (int c = 0; c < 3; c++) io.mout[c * N + I] = (float)w[c];
+        };
+        if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER) {
+            for (int c = 0; c < 3; c++) y[c] = io.y0[c * N + I];
+            if (P.normalize) {
+                nrm(y);
+                for (int c = 0; c < 3; c++) io.y0[c * N + I] = y[c];
+            }
+        }
+        if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1) {
+            for (int c = 0; c < 3; c++) {
+                io.k1[c * N + I] = kk[c];
+                acc[c] = y[c] + (0.5 * h) * (double)kk[c];
+            }
+            store_m(acc);
+        } else if constexpr (ROLE == R_RK4_2 || ROLE == R_RK4_3 || ROLE == R_BS_2) {
+            const double cm = (ROLE == R_RK4_3) ? h : ((ROLE == R_BS_2) ? 0.75 * h : 0.5 * h);
+            float* kst = (ROLE == R_RK4_3) ? io.k3 : io.k2;
+            for (int c = 0; c < 3; c++) {
+                kst[c * N + I] = kk[c];
+                acc[c] = io.y0[c * N + I] + cm * (double)kk[c];
+            }
+            store_m(acc);
+        } else if constexpr (ROLE == R_RK4_4) {
+            const double h6 = h / 6.0;
+            for (int c = 0; c < 3; c++) {
+                double k1 = io.k1[c * N + I], k2 = io.k2[c * N + I], k3 = io.k3[c * N + I];
+                acc[c] = io.y0[c * N + I] + h6 * (k1 + 2.0 * k2 + 2.0 * k3 + (double)kk[c]);
+                io.y0[c * N + I] = acc[c];
+            }
+            store_m(acc);
+        } else if constexpr (ROLE == R_BS_3) {
+            for (int c = 0; c < 3; c++) {
+                double k1 = io.k1[c * N + I], k2 = io.k2[c * N + I], k3 = kk[c];
+                acc[c] = io.y0[c * N + I] + h * ((2.0 / 9.0) * k1 + (1.0 / 3.0) * k2 + (4.0 / 9.0) * k3);
+                io.acc[c * N + I] = acc[c];
+                io.e[c * N + I] = (float)(h * ((-5.0 / 72.0) * k1 + (1.0 / 12.0) * k2 + (1.0 / 9.0) * k3));
+            }
+            store_m(acc);
+        } else if constexpr (ROLE == R_BS_4) {
+            for (int c = 0; c < 3; c++) {
+                double ev = (double)io.e[c * N + I] - (h / 8.0) * (double)kk[c];
+                errloc = fmax(errloc, fabs(ev));
+            }
+            if (P.normalize) {
+                for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
+                nrm(acc);
+                for (int c = 0; c < 3; c++) io.acc[c * N + I] = acc[c];
+            }
+        } else if constexpr (ROLE == R_RK12_1) {
+            for (int c = 0; c < 3; c++) {
+                io.e[c * N + I] = kk[c];
+                acc[c] = y[c] + h * (double)kk[c];
+                io.acc[c * N + I] = acc[c];
+            }
+            store_m(acc);
+        } else if constexpr (ROLE == R_RK12_2) {
+            for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
+            nrm(acc);
+            for (int c = 0; c < 3; c++) {
+                float k1 = io.e[c * N + I];
+                acc[c] += (0.5 * h) * (double)kk[c] + (-0.5 * h) * (double)k1;
+                io.acc[c * N + I] = acc[c];
+                errloc = fmax(errloc, fabs((double)kk[c] - (double)k1));
+            }
+            store_m(acc);
+        } else if constexpr (ROLE == R_EULER) {
+            for (int c = 0; c < 3; c++) {
+                acc[c] = y[c] + h * (double)kk[c];
+                io.y0[c * N + I] = acc[c];
+            }
+            store_m(acc);
+        }
+    }
+}
+
+template <int ROLE>
+__device__ __forceinline__ void flush_err(double errloc, const StageIO& io) {
+    if constexpr (ROLE == R_BS_4 || ROLE == R_RK12_2) {
+        for (int o = 32; o > 0; o >>= 1) errloc = fmax(errloc, __shfl_xor(errloc, o, 64));
+        if (((threadIdx.y * TX + threadIdx.x) & 63) == 0)
+            atomicMax(io.errmax, (unsigned long long)__double_as_longlong(errloc));
+    }
+}
+
+// Marching variant: a 64x4 tile walks planes [i0, i0+chunk) keeping planes
+// i-1, i, i+1 of m in registers (each m value is read from HBM once).
 template <int ROLE>
 __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
     __shared__ float tile[3][TY + 2][TX + 2];
@@ -83,300 +357,45 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
     const long N = (long)N0 * N12;
     const bool in = (k < N2) && (j < N1);
     const float* m = io.m;
-    // neighbour indices along k and j (clamped = Neumann, or wrapped)
-    auto wrapc = [](int c, int n, int w) {
-        if (c < 0) c = w ? c + n : 0;
-        if (c >= n) c = w ? c - n : n - 1;
-        return c;
-    };
     const int kc = min(k, N2 - 1), jc = min(j, N1 - 1);
     double errloc = 0.0;
-    // registers: plane i-1, i, i+1 at (j,k)
-    float a[3][3];  // [plane][comp]
+    const int i0 = blockIdx.z * P.chunk, i1 = min(N0, i0 + P.chunk);
+    float a[3][3];
     float la[3] = {1.f, 1.f, 1.f};
     {
         long I0 = (long)jc * N2 + kc;
-        int ip = wrapc(-1, N0, P.w0);
-        int in1 = wrapc(1, N0, P.w0);
+        int ip = wrapc(i0 - 1, N0, P.w0);
+        int in1 = wrapc(i0 + 1, N0, P.w0);
         for (int c = 0; c < 3; c++) {
-            a[1][c] = m[c * N + I0];
+            a[1][c] = m[c * N + i0 * N12 + I0];
             a[0][c] = m[c * N + ip * N12 + I0];
             a[2][c] = m[c * N + in1 * N12 + I0];
         }
         if (P.lex) {
-            la[1] = P.lex[I0];
+            la[1] = P.lex[i0 * N12 + I0];
             la[0] = P.lex[ip * N12 + I0];
             la[2] = P.lex[in1 * N12 + I0];
         }
     }
-    for (int i = 0; i < N0; i++) {
+    for (int i = i0; i < i1; i++) {
         const long I = i * N12 + (long)jc * N2 + kc;
-        // ---- plane tile with halo in LDS
-        for (int c = 0; c < 3; c++) tile[c][ty + 1][tx + 1] = a[1][c];
-        if (P.lex) ltile[ty + 1][tx + 1] = la[1];
-        if (P.exchange) {
-            const long base = i * N12;
-            if (ty == 0 || ty == TY - 1) {
-                int jn = (ty == 0) ? wrapc(blockIdx.y * TY - 1, N1, P.w1) : wrapc(min(blockIdx.y * TY + TY, N1), N1, P.w1);
-                int row = (ty == 0) ? 0 : TY + 1;
-                long a2 = base + (long)jn * N2 + kc;
-                for (int c = 0; c < 3; c++) tile[c][row][tx + 1] = m[c * N + a2];
-                if (P.lex) ltile[row][tx + 1] = P.lex[a2];
-            }
-            if (tx == 0 || tx == TX - 1) {
-                int kn = (tx == 0) ? wrapc(blockIdx.x * TX - 1, N2, P.w2) : wrapc(min(blockIdx.x * TX + TX, N2), N2, P.w2);
-                int col = (tx == 0) ? 0 : TX + 1;
-                long a2 = base + (long)jc * N2 + kn;
-                for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = m[c * N + a2];
-                if (P.lex) ltile[ty + 1][col] = P.lex[a2];
-            }
-        }
-        __syncthreads();
-        // ---- effective field
-        float m0[3] = {a[1][0], a[1][1], a[1][2]};
-        float H[3] = {0.f, 0.f, 0.f};
-        if (P.exchange && in) {
-            float l0 = la[1];
-            float l02 = l0 * l0;
-            // neighbours along k: inside the tile unless at the tile edge or the mesh edge
-            int km = k - 1, kp = k + 1;
-            int kmx = (km < 0) ? (P.w2 ? tx : tx + 1) : tx;          // LDS column of k-1
-            int kpx = (kp >= N2) ? (P.w2 ? -1 : tx + 1) : tx + 2;   // LDS column of k+1
-            if (kp >= N2 && P.w2) kpx = -1;
-            int jm = j - 1, jp = j + 1;
-            int jmy = (jm < 0) ? (P.w1 ? ty : ty + 1) : ty;
-            int jpy = (jp >= N1) ? (P.w1 ? -1 : ty + 1) : ty + 2;
-            float e0[3], e1[3], e2[3];
-            // axis 0 (X): planes i-1, i+1 (clamped at open ends: self)
-            {
-                bool lo_self = (i == 0 && !P.w0), hi_self = (i == N0 - 1 && !P.w0);
-                float l1 = lo_self ? l0 : la[0], l2 = hi_self ? l0 : la[2];
-                float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
-                for (int c = 0; c < 3; c++) {
-                    float v1 = lo_self ? m0[c] : a[0][c], v2 = hi_self ? m0[c] : a[2][c];
-                    e0[c] = P.ex0 * (pre1 * (v1 - m0[c]) + pre2 * (v2 - m0[c]));
-                }
-            }
-            // axis 2 (Z, contiguous k)
-            {
-                float v1[3], v2[3], l1, l2;
-                if (km < 0 && P.w2) {  // wrapped: k-1 = N2-1 lives in another tile unless N2 <= TX
-                    long aa = i * N12 + (long)jc * N2 + (N2 - 1);
-                    for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
-                    l1 = P.lex ? P.lex[aa] : 1.f;
-                } else {
-                    for (int c = 0; c < 3; c++) v1[c] = tile[c][ty + 1][kmx];
-                    l1 = P.lex ? ltile[ty + 1][kmx] : 1.f;
-                }
-                if (kpx < 0) {
-                    long aa = i * N12 + (long)jc * N2 + 0;
-                    for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
-                    l2 = P.lex ? P.lex[aa] : 1.f;
-                } else {
-                    for (int c = 0; c < 3; c++) v2[c] = tile[c][ty + 1][kpx];
-                    l2 = P.lex ? ltile[ty + 1][kpx] : 1.f;
-                }
-                float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
-                for (int c = 0; c < 3; c++) e2[c] = P.ex2 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
-            }
-            // axis 1 (Y)
-            {
-                float v1[3], v2[3], l1, l2;
-                if (jm < 0 && P.w1) {
-                    long aa = i * N12 + (long)(N1 - 1) * N2 + kc;
-                    for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
-                    l1 = P.lex ? P.lex[aa] : 1.f;
-                } else {
-                    for (int c = 0; c < 3; c++) v1[c] = tile[c][jmy][tx + 1];
-                    l1 = P.lex ? ltile[jmy][tx + 1] : 1.f;
-                }
-                if (jpy < 0) {
-                    long aa = i * N12 + (long)0 * N2 + kc;
-                    for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
-                    l2 = P.lex ? P.lex[aa] : 1.f;
-                } else {
-                    for (int c = 0; c < 3; c++) v2[c] = tile[c][jpy][tx + 1];
-                    l2 = P.lex ? ltile[jpy][tx + 1] : 1.f;
-                }
-                float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
-                for (int c = 0; c < 3; c++) e1[c] = P.ex1 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
-            }
-            for (int c = 0; c < 3; c++) H[c] = P.ms0T0Mul * (e0[c] + e2[c] + e1[c]);
-        }
-        __syncthreads();  // tile reuse in the next plane
-        // roll the plane registers (load plane i+2)
         float nxt[3] = {0.f, 0.f, 0.f};
         float lnx = 1.f;
-        if (N0 > 1) {
-            int i2 = wrapc(i + 2, N0, P.w0);
-            if (!P.w0 && i + 2 >= N0) i2 = N0 - 1;
+        if (i + 1 < i1) {  // prefetch plane i+2 (wrapped, or clamped at an open end)
+            int i2 = i + 2;
+            if (i2 >= N0) i2 = P.w0 ? i2 - N0 : N0 - 1;
             long a2 = i2 * N12 + (long)jc * N2 + kc;
             for (int c = 0; c < 3; c++) nxt[c] = m[c * N + a2];
             if (P.lex) lnx = P.lex[a2];
         }
-        if (in) {
-            const float ms = P.msk ? P.msk[I] : 1.f;
-            if (P.demag) {
-                H[0] += P.invMu0 * io.B[I];
-                H[1] += P.invMu0 * io.B[N + I];
-                H[2] += P.invMu0 * io.B[2 * N + I];
-            }
-            if (P.zeeman) {
-                H[0] += P.bext[0];
-                H[1] += P.bext[1];
-                H[2] += P.bext[2];
-            }
-            if (P.longfield) {
-                // FP64: dB_S/dx is a difference of O(1) terms (common_func.h:95-98)
-                double Ms0T0 = (double)P.ms0T0Mul * ms;
-                double S = (double)P.JMul * (P.J ? P.J[I] : 1.f);
-                double n = (double)P.nMul * (P.n ? P.n[I] : 1.f);
-                double Tc = (double)P.TcMul * (P.Tc ? P.Tc[I] : 1.f);
-                double Ts = (double)P.TMul * (P.T ? P.T[I] : 1.f);
-                double mx = m0[0], my = m0[1], mz = m0[2];
-                double am = sqrt(mx * mx + my * my + mz * mz);
-                if (!(Ms0T0 == 0.0 || n == 0.0 || am <= HS_ZERO)) {
-                    double preS = (S > HS_INFSPIN) ? 1.0 : S / (S + 1.0);
-                    double b = 3.0 * preS * Tc / Ts;
-                    double W, dW;
-                    brillouin_w_dw<double>(S, am * b, W, dW);
-                    double mult = n * (HS_KB / HS_MU0) * Ts * (W - am) / (Ms0T0 * dW) / am;
-                    H[0] += (float)(mult * mx);
-                    H[1] += (float)(mult * my);
-                    H[2] += (float)(mult * mz);
-                }
-            }
-            if (P.thermal && ms != 0.f) {
-                float x0, x1, x2;
-                thermal_normals_f((uint32_t)I, io.step, io.attempt, P.stream, P.seed, x0, x1, x2);
-                float Ti = P.T ? P.T[I] : 1.f;
-                float pre = sqrtf((Ti * io.thK) / ms);
-                float s0 = sqrtf(P.aMul[0] * (P.alpha[0] ? P.alpha[0][I] : 1.f));
-                float s1 = sqrtf(P.aMul[1] * (P.alpha[1] ? P.alpha[1][I] : 1.f));
-                float s2 = sqrtf(P.aMul[2] * (P.alpha[2] ? P.alpha[2][I] : 1.f));
-                H[0] += pre * (s0 * x0);
-                H[1] += pre * (s1 * x1);
-                H[2] += pre * (s2 * x2);
-            }
-            float kk[3] = {0.f, 0.f, 0.f};
-            if (ms != 0.f) {
-                if (P.torque) {  // cross(H, m) = m x H
-                    kk[0] = P.gamma * (-H[1] * m0[2] + H[2] * m0[1]);
-                    kk[1] = P.gamma * (-H[2] * m0[0] + H[0] * m0[2]);
-                    kk[2] = P.gamma * (-H[0] * m0[1] + H[1] * m0[0]);
-                }
-                if (P.local) {
-                    for (int c = 0; c < 3; c++) {
-                        float al = P.aMul[c] * (P.alpha[c] ? P.alpha[c][I] : 1.f);
-                        kk[c] += P.gamma * (al * H[c]);
-                    }
-                }
-            }
-            // ---- RK stage update (FP64 state)
-            const double h = io.h;
-            if constexpr (ROLE == R_RHS) {
-                for (int c = 0; c < 3; c++) {
-                    io.kout[c * N + I] = kk[c];
-                    io.hout[c * N + I] = H[c];
-                }
-            } else {
-                double y[3], acc[3];
-                auto nrm = [&](double* v) {
-                    if (P.normalize) {
-                        double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                        double s = (l == 0.0) ? 0.0 : 1.0 / l;
-                        v[0] *= s; v[1] *= s; v[2] *= s;
-                    }
-                };
-                auto store_m = [&](const double* v) {
-                    double w[3] = {v[0], v[1], v[2]};
-                    nrm(w);
-                    for (int c = 0; c < 3; c++) io.mout[c * N + I] = (float)w[c];
-                };
-                if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER) {
-                    for (int c = 0; c < 3; c++) y[c] = io.y0[c * N + I];
-                    if (P.normalize) {
-                        nrm(y);
-                        for (int c = 0; c < 3; c++) io.y0[c * N + I] = y[c];
-                    }
-                }
-                if constexpr (ROLE == R_RK4_1) {
-                    for (int c = 0; c < 3; c++) {
-                        io.acc[c * N + I] = y[c] + (h / 6.0) * (double)kk[c];
-                        acc[c] = y[c] + (0.5 * h) * (double)kk[c];
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_RK4_2 || ROLE == R_RK4_3) {
-                    const double cm = (ROLE == R_RK4_2) ? 0.5 * h : h;
-                    for (int c = 0; c < 3; c++) {
-                        io.acc[c * N + I] += (h / 3.0) * (double)kk[c];
-                        acc[c] = io.y0[c * N + I] + cm * (double)kk[c];
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_RK4_4) {
-                    for (int c = 0; c < 3; c++) {
-                        acc[c] = io.acc[c * N + I] + (h / 6.0) * (double)kk[c];
-                        io.y0[c * N + I] = acc[c];
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_BS_1) {
-                    for (int c = 0; c < 3; c++) {
-                        io.acc[c * N + I] = y[c] + (h * (2.0 / 9.0)) * (double)kk[c];
-                        io.e[c * N + I] = (float)((h * (-5.0 / 72.0)) * (double)kk[c]);
-                        acc[c] = y[c] + (0.5 * h) * (double)kk[c];
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_BS_2) {
-                    for (int c = 0; c < 3; c++) {
-                        io.acc[c * N + I] += (h / 3.0) * (double)kk[c];
-                        io.e[c * N + I] += (float)((h / 12.0) * (double)kk[c]);
-                        acc[c] = io.y0[c * N + I] + (0.75 * h) * (double)kk[c];
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_BS_3) {
-                    for (int c = 0; c < 3; c++) {
-                        acc[c] = io.acc[c * N + I] + (h * (4.0 / 9.0)) * (double)kk[c];
-                        io.acc[c * N + I] = acc[c];
-                        io.e[c * N + I] += (float)((h / 9.0) * (double)kk[c]);
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_BS_4) {
-                    for (int c = 0; c < 3; c++) {
-                        double ev = (double)io.e[c * N + I] - (h / 8.0) * (double)kk[c];
-                        errloc = fmax(errloc, fabs(ev));
-                    }
-                    if (P.normalize) {
-                        for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
-                        nrm(acc);
-                        for (int c = 0; c < 3; c++) io.acc[c * N + I] = acc[c];
-                    }
-                } else if constexpr (ROLE == R_RK12_1) {
-                    for (int c = 0; c < 3; c++) {
-                        io.e[c * N + I] = kk[c];
-                        acc[c] = y[c] + h * (double)kk[c];
-                        io.acc[c * N + I] = acc[c];
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_RK12_2) {
-                    for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
-                    nrm(acc);
-                    for (int c = 0; c < 3; c++) {
-                        float k1 = io.e[c * N + I];
-                        acc[c] += (0.5 * h) * (double)kk[c] + (-0.5 * h) * (double)k1;
-                        io.acc[c * N + I] = acc[c];
-                        errloc = fmax(errloc, fabs((double)kk[c] - (double)k1));
-                    }
-                    store_m(acc);
-                } else if constexpr (ROLE == R_EULER) {
-                    for (int c = 0; c < 3; c++) {
-                        acc[c] = y[c] + h * (double)kk[c];
-                        io.y0[c * N + I] = acc[c];
-                    }
-                    store_m(acc);
-                }
-            }
-        }
-        // roll
+        load_tile(P, m, N, i, jc, kc, tx, ty, a[1], la[1], tile, ltile);
+        __syncthreads();
+        float m0[3] = {a[1][0], a[1][1], a[1][2]};
+        float H[3] = {0.f, 0.f, 0.f};
+        if (P.exchange && in)
+            exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, a[0], a[2], la[1], la[0], la[2], tile, ltile, H);
+        __syncthreads();
+        if (in) finish_cell<ROLE>(P, io, I, N, m0, H, errloc);
         for (int c = 0; c < 3; c++) {
             a[0][c] = a[1][c];
             a[1][c] = a[2][c];
@@ -385,15 +404,48 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
         la[0] = la[1];
         la[1] = la[2];
         la[2] = lnx;
-        // wrapped/clamped plane i+1 for the next iteration is already in a[2]
-        if (P.w0 && N0 > 1) {
-            // a[0] must be plane i (it is), a[2] plane (i+2) mod N0 (loaded above)
+    }
+    flush_err<ROLE>(errloc, io);
+}
+
+// Flat variant: one plane per block, planes i-1 and i+1 read directly (they
+// are the interior of other blocks and come from L2 / Infinity Cache).
+template <int ROLE>
+__global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io) {
+    __shared__ float tile[3][TY + 2][TX + 2];
+    __shared__ float ltile[TY + 2][TX + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int k = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty, i = blockIdx.z;
+    const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
+    const long N12 = (long)N1 * N2;
+    const long N = (long)N0 * N12;
+    const bool in = (k < N2) && (j < N1);
+    const float* m = io.m;
+    const int kc = min(k, N2 - 1), jc = min(j, N1 - 1);
+    const long I = i * N12 + (long)jc * N2 + kc;
+    double errloc = 0.0;
+    float m0[3], lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+    float l0 = 1.f, llo = 1.f, lhi = 1.f;
+    for (int c = 0; c < 3; c++) m0[c] = m[c * N + I];
+    if (P.lex) l0 = P.lex[I];
+    if (P.exchange) {
+        long Ilo = wrapc(i - 1, N0, P.w0) * N12 + (long)jc * N2 + kc;
+        long Ihi = wrapc(i + 1, N0, P.w0) * N12 + (long)jc * N2 + kc;
+        for (int c = 0; c < 3; c++) {
+            lo[c] = m[c * N + Ilo];
+            hi[c] = m[c * N + Ihi];
+        }
+        if (P.lex) {
+            llo = P.lex[Ilo];
+            lhi = P.lex[Ihi];
         }
     }
-    if constexpr (ROLE == R_BS_4 || ROLE == R_RK12_2) {
-        for (int o = 32; o > 0; o >>= 1) errloc = fmax(errloc, __shfl_xor(errloc, o, 64));
-        if (((ty * TX + tx) & 63) == 0) atomicMax(io.errmax, (unsigned long long)__double_as_longlong(errloc));
-    }
+    load_tile(P, m, N, i, jc, kc, tx, ty, m0, l0, tile, ltile);
+    __syncthreads();
+    float H[3] = {0.f, 0.f, 0.f};
+    if (P.exchange && in) exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, lo, hi, l0, llo, lhi, tile, ltile, H);
+    if (in) finish_cell<ROLE>(P, io, I, N, m0, H, errloc);
+    flush_err<ROLE>(errloc, io);
 }
 
 // RK12 retry: predictor from the stored k1 without a new field evaluation
@@ -445,10 +497,12 @@ struct hs_solver {
     float *mA = nullptr, *mB = nullptr, *mC = nullptr;  // FP32 stage inputs
     float *B0 = nullptr, *Bs = nullptr;
     float* e = nullptr;
+    float *k1 = nullptr, *k2 = nullptr, *k3 = nullptr;
     float *kout = nullptr, *hout = nullptr;
     unsigned long long* errmax = nullptr;
     std::vector<float*> masks;
     bool b0_valid = false;  // B0 = demag(mA) is current
+    bool stage_flat = true;  // flat (one plane per block) or marching stage kernel
 };
 
 namespace {
@@ -467,12 +521,51 @@ float* upload_mask(hs_solver* s, const double* h) {
 }
 
 template <int ROLE>
+void launch_stage(hs_solver* s, const StageIO& io);
+void launch_stage_rk4_2(hs_solver* s, const StageIO& io) { launch_stage<R_RK4_2>(s, io); }
+template <int ROLE>
 void launch_stage(hs_solver* s, const StageIO& io) {
     const Phys& P = s->phys;
-    dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY));
     hs::prof_begin(HS_PROF_STAGE, s->st);
-    hipLaunchKernelGGL(k_stage<ROLE>, grd, blk, 0, s->st, P, io);
+    if (s->stage_flat) {
+        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), P.N0);
+        hipLaunchKernelGGL(k_stage_flat<ROLE>, grd, blk, 0, s->st, P, io);
+    } else {
+        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), hs::div_up(P.N0, P.chunk));
+        hipLaunchKernelGGL(k_stage<ROLE>, grd, blk, 0, s->st, P, io);
+    }
     hs::prof_end(HS_PROF_STAGE, s->st);
+}
+
+// Pick the faster stage-kernel variant (flat / marching) by timing one
+// representative role on the solver's own buffers.
+void tune_stage(hs_solver* s) {
+    const char* env = getenv("HS_STAGE_KERNEL");
+    if (env && !strcmp(env, "flat")) { s->stage_flat = true; return; }
+    if (env && !strcmp(env, "march")) { s->stage_flat = false; return; }
+    if (s->phys.N0 < 2) { s->stage_flat = true; return; }
+    StageIO io{};
+    io.m = s->mA; io.B = s->B0; io.y0 = s->y0; io.acc = s->acc; io.e = s->e;
+    io.k1 = s->k1; io.k2 = s->k2; io.k3 = s->k3; io.mout = s->mB; io.errmax = s->errmax;
+    io.h = 1e-15; io.thK = 0.f;
+    hipMemsetAsync(s->mA, 0, sizeof(float) * 3 * s->N, s->st);
+    hipMemsetAsync(s->y0, 0, sizeof(double) * 3 * s->N, s->st);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float t[2];
+    for (int v = 0; v < 2; v++) {
+        s->stage_flat = (v == 0);
+        launch_stage_rk4_2(s, io);
+        hipEventRecord(e0, s->st);
+        for (int r = 0; r < 3; r++) launch_stage_rk4_2(s, io);
+        hipEventRecord(e1, s->st);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&t[v], e0, e1);
+    }
+    s->stage_flat = t[0] <= t[1];
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
 }
 
 void demag_into(hs_solver* s, const float* m, float* B) {
@@ -494,6 +587,9 @@ StageIO base_io(hs_solver* s, double h, uint32_t step, uint32_t attempt) {
     io.y0 = s->y0;
     io.acc = s->acc;
     io.e = s->e;
+    io.k1 = s->k1;
+    io.k2 = s->k2;
+    io.k3 = s->k3;
     io.errmax = s->errmax;
     io.h = h;
     io.thK = s->p.thermal ? thermal_K(s, h) : 0.f;
@@ -551,8 +647,12 @@ extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) 
     hipMalloc(&s->B0, sizeof(float) * 3 * N);
     hipMalloc(&s->Bs, sizeof(float) * 3 * N);
     hipMalloc(&s->e, sizeof(float) * 3 * N);
+    hipMalloc(&s->k1, sizeof(float) * 3 * N);
+    hipMalloc(&s->k2, sizeof(float) * 3 * N);
+    hipMalloc(&s->k3, sizeof(float) * 3 * N);
     hipMalloc(&s->errmax, sizeof(unsigned long long));
-    if (!s->y0 || !s->acc || !s->mA || !s->mB || !s->mC || !s->B0 || !s->Bs || !s->e || !s->errmax) {
+    if (!s->y0 || !s->acc || !s->mA || !s->mB || !s->mC || !s->B0 || !s->Bs || !s->e || !s->k1 || !s->k2 ||
+        !s->k3 || !s->errmax) {
         hs::set_error(HS_ENOMEM, "hs_solver_create: out of device memory");
         hs_solver_destroy(s);
         return nullptr;
@@ -564,6 +664,14 @@ extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) 
     P.N1 = p->N[1];
     P.N2 = p->N[2];
     P.w0 = p->pbc[0] != 0;
+    {
+        // enough blocks to fill 256 CUs several times over: split the planes into chunks
+        long blocks2d = (long)hs::div_up(P.N2, TX) * hs::div_up(P.N1, TY);
+        long chunks = (4096 + blocks2d - 1) / blocks2d;
+        if (chunks < 1) chunks = 1;
+        if (chunks > P.N0) chunks = P.N0;
+        P.chunk = (int)((P.N0 + chunks - 1) / chunks);
+    }
     P.w1 = p->pbc[1] != 0;
     P.w2 = p->pbc[2] != 0;
     P.exchange = p->exchange;
@@ -606,6 +714,8 @@ extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) 
             return nullptr;
         }
     }
+    tune_stage(s);
+    hipStreamSynchronize(s->st);
     HS_LAUNCH_CHECK("hs_solver_create");
     return s;
 }
@@ -613,7 +723,8 @@ extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) 
 extern "C" void hs_solver_destroy(hs_solver* s) {
     if (!s) return;
     if (s->demag) hs_demag_destroy(s->demag);
-    void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->kout, s->hout, s->errmax};
+    void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
+                  s->errmax};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
@@ -621,6 +732,8 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
 }
 
 extern "C" CUstream hs_solver_stream(hs_solver* s) { return s ? (CUstream)s->st : nullptr; }
+extern "C" hs_demag* hs_solver_demag(hs_solver* s) { return s ? s->demag : nullptr; }
+extern "C" int hs_solver_stage_variant(hs_solver* s) { return s ? (s->stage_flat ? 1 : 0) : -1; }
 
 extern "C" int hs_solver_set_m(hs_solver* s, const double* m) {
     if (!s || !m) return HS_EINVAL;

@@ -57,6 +57,9 @@ HS_API int hs_demag_kernel_f64(const hs_demag* d, double* host_out);
 /* max |imag| / max |real| of the transformed kernel (the reference panics
  * above 1e-12, maxwell_convolution.go:383-385) */
 HS_API double hs_demag_max_imag_ratio(const hs_demag* d);
+/* FP32 tile shapes the plan's autotuner picked: (W, NT) of the row passes
+ * (F1/I1), the column passes (F2/I2) and the axis-0 product pass (F3K) */
+HS_API int hs_demag_tiles(const hs_demag* d, int* out6);
 /* B = mu0 * msat0T0Mul * (K conv (m .* msat0T0)), maxwell_convolution.go:103-131.
  * m, B: 3 device planes of N cells (m and B may not alias); msat0T0: device
  * mask or NULL.  FP64 variant: FP64 FFT passes; FP32 variant: FP32 passes. */
@@ -140,6 +143,11 @@ HS_API int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, doub
 HS_API int hs_solver_rhs(hs_solver* s, double* k_host, double* heff_host, uint32_t step, uint32_t attempt,
                          double dt);
 HS_API CUstream hs_solver_stream(hs_solver* s);
+/* the solver's demag plan (NULL when demag is off) */
+HS_API hs_demag* hs_solver_demag(hs_solver* s);
+/* stage-kernel variant picked by timing at creation: 1 = one plane per
+ * block, 0 = marching planes (env HS_STAGE_KERNEL=flat|march overrides) */
+HS_API int hs_solver_stage_variant(hs_solver* s);
 
 /* ---------------------------------------------------------------- timing */
 /* Per-kernel durations from HIP events recorded on the launching stream
