@@ -113,6 +113,15 @@ class Solver:
         check("hs_solver_rhs")
         return k, h
 
+    def run_graph(self, nsteps, dt, step0=None):
+        """Fixed-step RK4 from a captured hipGraph (same arithmetic as run("rk4"))."""
+        s0 = self.step_count if step0 is None else int(step0)
+        rc = lib().hs_solver_run_graph(self._h, int(nsteps), s0, float(dt))
+        check("hs_solver_run_graph")
+        if rc:
+            raise HotspinError("hs_solver_run_graph returned %d" % rc)
+        self.step_count = s0 + int(nsteps)
+
     def run(self, method, nsteps, dt, maxErr=1e-5, mindt=1e-38, maxdt=1e38, step0=None):
         meth = METHODS[method] if isinstance(method, str) else int(method)
         d = C.c_double(dt)
@@ -129,6 +138,175 @@ class Solver:
     def close(self):
         if getattr(self, "_h", None):
             lib().hs_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch(Solver):
+    """nsolve independent solves of one mesh stepped by the same launches
+    (per-solve temperature multiplier, thermal seed and applied field)."""
+
+    def __init__(self, N, cell, nsolve, pbc=(0, 0, 0), TMuls=None, seeds=None, Bexts=None, stream=None, **kw):
+        self.N = tuple(int(x) for x in N)
+        self.ncell = self.N[0] * self.N[1] * self.N[2]
+        self.nsolve = int(nsolve)
+        self.params, self._keep = make_params(N, cell, pbc, **kw)
+        arrT = None if TMuls is None else np.ascontiguousarray(TMuls, dtype=np.float64)
+        arrS = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint64)
+        arrB = None if Bexts is None else np.ascontiguousarray(Bexts, dtype=np.float64).reshape(-1)
+        for a, n in ((arrT, self.nsolve), (arrS, self.nsolve), (arrB, 3 * self.nsolve)):
+            if a is not None and a.size != n:
+                raise ValueError("per-solve array has %d entries, expected %d" % (a.size, n))
+        self._keep += [a for a in (arrT, arrS, arrB) if a is not None]
+        self._h = lib().hs_batch_create(C.byref(self.params), self.nsolve,
+                                        None if arrT is None else arrT.ctypes.data,
+                                        None if arrS is None else arrS.ctypes.data,
+                                        None if arrB is None else arrB.ctypes.data, stream)
+        if not self._h:
+            check("hs_batch_create")
+            raise HotspinError("hs_batch_create failed")
+        self.step_count = 0
+
+    def set_m(self, m):
+        m = _f64(m).reshape(self.nsolve, 3, self.ncell)
+        lib().hs_solver_set_m(self._h, m.ctypes.data)
+        check("hs_solver_set_m")
+
+    def get_m(self):
+        out = np.empty((self.nsolve, 3, self.ncell), np.float64)
+        lib().hs_solver_get_m(self._h, out.ctypes.data)
+        check("hs_solver_get_m")
+        return out
+
+    def rhs(self, step=0, attempt=0, dt=0.0):
+        k = np.empty((self.nsolve, 3, self.ncell), np.float64)
+        h = np.empty((self.nsolve, 3, self.ncell), np.float64)
+        lib().hs_solver_rhs(self._h, k.ctypes.data, h.ctypes.data, int(step), int(attempt), float(dt))
+        check("hs_solver_rhs")
+        return k, h
+
+    def run_graph(self, nsteps, dt, step0=None):
+        """Fixed-step RK4 of every solve from a captured hipGraph."""
+        s0 = self.step_count if step0 is None else int(step0)
+        rc = lib().hs_solver_run_graph(self._h, int(nsteps), s0, float(dt))
+        check("hs_solver_run_graph")
+        if rc:
+            raise HotspinError("hs_solver_run_graph returned %d" % rc)
+        self.step_count = s0 + int(nsteps)
+
+
+class Slot(C.Structure):
+    """hs_slot of include/hotspin_amd.h."""
+    _fields_ = [("step", C.c_uint32), ("attempt", C.c_uint32), ("h", C.c_double), ("t", C.c_double),
+                ("err", C.c_double), ("maxErr", C.c_double), ("mindt", C.c_double), ("maxdt", C.c_double),
+                ("end", C.c_uint32), ("state", C.c_int32), ("rejected", C.c_uint32), ("accepted", C.c_uint32),
+                ("home", C.c_uint32), ("retry", C.c_uint32)]
+
+
+SLOT_IDLE, SLOT_RUNNING, SLOT_DONE, SLOT_FAILED = 0, 1, 2, 3
+
+
+class SlotBatch:
+    """A batch whose nsolve solves are slots filled, stepped (adaptive RK23,
+    per-slot dt decided on the device) and emptied independently."""
+
+    def __init__(self, N, cell, capacity, pbc=(0, 0, 0), stream=None, **kw):
+        self.N = tuple(int(x) for x in N)
+        self.ncell = self.N[0] * self.N[1] * self.N[2]
+        self.capacity = int(capacity)
+        self.params, self._keep = make_params(N, cell, pbc, **kw)
+        self._h = lib().hs_batch_create(C.byref(self.params), self.capacity, None, None, None, stream)
+        if not self._h:
+            check("hs_batch_create")
+            raise HotspinError("hs_batch_create failed")
+        self._status = (Slot * self.capacity)()
+
+    def admit(self, slot, m, TMul, seed, nsteps, dt, maxErr=1e-5, mindt=1e-20, maxdt=1e-12, Bext=None,
+              stream=0, step0=0):
+        m = _f64(m).reshape(3, self.ncell)
+        b = None if Bext is None else (C.c_double * 3)(*[float(x) for x in Bext])
+        rc = lib().hs_slot_admit(self._h, int(slot), m.ctypes.data, float(TMul), int(seed), int(stream), b,
+                                 int(step0), int(nsteps), float(dt), float(maxErr), float(mindt), float(maxdt))
+        check("hs_slot_admit")
+        if rc:
+            raise HotspinError("hs_slot_admit returned %d" % rc)
+
+    def advance(self, iters):
+        rc = lib().hs_slot_advance(self._h, int(iters))
+        check("hs_slot_advance")
+        if rc:
+            raise HotspinError("hs_slot_advance returned %d" % rc)
+
+    def ready(self):
+        return bool(lib().hs_slot_ready(self._h))
+
+    def status(self):
+        lib().hs_slot_status(self._h, self._status)
+        check("hs_slot_status")
+        return self._status
+
+    def get_m(self, slot):
+        out = np.empty((3, self.ncell), np.float64)
+        lib().hs_slot_get_m(self._h, int(slot), out.ctypes.data)
+        check("hs_slot_get_m")
+        return out
+
+    def release(self, slot):
+        lib().hs_slot_release(self._h, int(slot))
+        check("hs_slot_release")
+
+    def footprint(self):
+        return int(lib().hs_solver_footprint(self._h))
+
+    def run_to_end(self, chunk=8):
+        """Advance until no slot is running; returns the slot records."""
+        while True:
+            self.advance(chunk)
+            st = self.status()
+            if not any(x.state == SLOT_RUNNING for x in st):
+                return st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hs_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Batches of uneven meshes stepped together: one RK4 step of all of them
+    is one hipGraph launch (branches on the batches' streams)."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)
+        arr = (C.c_void_p * len(self.batches))(*[b._h for b in self.batches])
+        self._h = lib().hs_ensemble_create(arr, len(self.batches))
+        if not self._h:
+            check("hs_ensemble_create")
+            raise HotspinError("hs_ensemble_create failed")
+        self.step_count = 0
+
+    def run_graph(self, nsteps, dt, step0=None):
+        s0 = self.step_count if step0 is None else int(step0)
+        rc = lib().hs_ensemble_run_graph(self._h, int(nsteps), s0, float(dt))
+        check("hs_ensemble_run_graph")
+        if rc:
+            raise HotspinError("hs_ensemble_run_graph returned %d" % rc)
+        self.step_count = s0 + int(nsteps)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hs_ensemble_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -308,6 +308,7 @@ struct hs_demag {
     hsdm::ColV col32{};
     hsdm::ZV z32{};
     bool tuned = false;
+    int cap = 1;  // solves the X1/X2 buffers hold
 };
 
 namespace {
@@ -326,24 +327,29 @@ bool sizes_ok(const hs_demag* d) {
            !hsdm::variants(d->P[0]).z.empty();
 }
 
-int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, float* B, hipStream_t st) {
+int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, float* B, hipStream_t st,
+                int nsolve = 1) {
     const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1];
     const int H2 = d->H2, H2p = d->H2p;
     cpx<float>*X1 = d->X1f, *X2 = d->X2f;
-    const long nrows = 3L * N0 * N1;
+    if (nsolve > d->cap) {
+        hs::set_error(HS_EINVAL, "hs_demag_field: batch larger than reserved");
+        return HS_EINVAL;
+    }
+    const long nrows = 3L * N0 * N1 * nsolve;
     Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
     Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
     hs::prof_begin(HS_PROF_F1, st);
     d->row32.r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, d->twMf, d->twP2f, st);
     hs::prof_end(HS_PROF_F1, st);
     hs::prof_begin(HS_PROF_F2, st);
-    d->col32.fwd(X1, X2, g1, g2, 3 * N0, N1, P1, H2, d->tw1f, st);
+    d->col32.fwd(X1, X2, g1, g2, 3 * N0 * nsolve, N1, P1, H2, d->tw1f, st);
     hs::prof_end(HS_PROF_F2, st);
     hs::prof_begin(HS_PROF_F3K, st);
-    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], (float)srcMul, d->tw0f, st);
+    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], (float)srcMul, d->tw0f, st, nsolve);
     hs::prof_end(HS_PROF_F3K, st);
     hs::prof_begin(HS_PROF_I2, st);
-    d->col32.inv(X2, X1, g2, g1, 3 * N0, P1, N1, H2, d->tw1f, st);
+    d->col32.inv(X2, X1, g2, g1, 3 * N0 * nsolve, P1, N1, H2, d->tw1f, st);
     hs::prof_end(HS_PROF_I2, st);
     hs::prof_begin(HS_PROF_I1, st);
     d->row32.c2r(X1, B, nrows, N2, H2p, d->twMf, d->twP2f, st);
@@ -403,7 +409,7 @@ void tune32(hs_demag* d, hipStream_t st) {
     }
     best = 1e30f;
     for (auto& v : vz.z) {
-        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], 1.0f, d->tw0f, st); });
+        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], 1.0f, d->tw0f, st, 1); });
         if (t < best) { best = t; d->z32 = v; }
     }
     hipEventDestroy(e0);
@@ -464,6 +470,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     }
     hipStream_t st = hs::S(stream);
     hs_demag* d = new hs_demag();
+    d->cap = 1;
     for (int c = 0; c < 3; c++) {
         d->N[c] = N[c];
         d->pbc[c] = pbc ? pbc[c] : 0;
@@ -644,7 +651,38 @@ extern "C" int hs_demag_field_f64(hs_demag* d, const double* m, const double* ms
 // Internal entry for the solver: FP32 field of the first `ncomp`-component
 // planes with an FP32 mask.
 int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
-                                hipStream_t st) {
-    return run_field32(d, m, msk, HS_MU0 * msat0T0Mul, B, st);
+                                hipStream_t st, int nsolve) {
+    return run_field32(d, m, msk, HS_MU0 * msat0T0Mul, B, st, nsolve);
+}
+
+// Grow the FP32 spectrum buffers to hold `nsolve` solves of this mesh.
+extern "C" int hs_demag_reserve(hs_demag* d, int nsolve) {
+    if (!d || nsolve < 1) return HS_EINVAL;
+    if (nsolve <= d->cap) return 0;
+    cpx<float>*X1 = nullptr, *X2 = nullptr;
+    hipMalloc(&X1, sizeof(cpx<float>) * 3L * d->N[0] * d->N[1] * d->H2p * nsolve);
+    hipMalloc(&X2, sizeof(cpx<float>) * 3L * d->N[0] * d->P[1] * d->H2p * nsolve);
+    if (!X1 || !X2) {
+        hipFree(X1);
+        hipFree(X2);
+        hs::set_error(HS_ENOMEM, "hs_demag_reserve: out of device memory");
+        return HS_ENOMEM;
+    }
+    hipFree(d->X1f);
+    hipFree(d->X2f);
+    d->X1f = X1;
+    d->X2f = X2;
+    d->cap = nsolve;
+    return 0;
 }
 long hs_demag_ncell(const hs_demag* d) { return d->ncell; }
+// device bytes of the plan: kernel octant (FP64 + FP32), FP32 spectra of
+// `cap` solves, FP64 spectra when allocated, twiddle tables
+long hs_demag_footprint(const hs_demag* d) {
+    const long nk = (long)d->K[0] * d->K[1] * d->K[2];
+    long b = 6 * nk * (sizeof(double) + sizeof(float));
+    b += (long)sizeof(cpx<float>) * 3L * d->N[0] * (d->N[1] + d->P[1]) * d->H2p * d->cap;
+    if (d->X1d) b += (long)sizeof(cpx<double>) * 3L * d->N[0] * (d->N[1] + d->P[1]) * d->H2p;
+    b += (long)(sizeof(cpx<float>) + sizeof(cpx<double>)) * (d->M + d->P[2] + d->P[1] + d->P[0]);
+    return b;
+}

@@ -166,6 +166,7 @@ template <typename T, typename TK, int L, int W = ZW, int NT = z_nt<L>()>
 __global__ void __launch_bounds__(NT) k_zconv(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
                                               int H2p, int K1, T srcMul, const cpx<T>* __restrict__ tw) {
     constexpr int WP = 3 * W;
+    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;  // solve of a batch
     __shared__ cpx<T> lds[L * WP];
     const int tid = threadIdx.x;
     const int jp = blockIdx.y;

@@ -15,7 +15,8 @@ using c2r32_t = void (*)(const cpx<float>*, float*, long, int, int, const cpx<fl
                          hipStream_t);
 using cols32_t = void (*)(const cpx<float>*, cpx<float>*, Geo, Geo, int, int, int, int, const cpx<float>*,
                           hipStream_t);
-using z32_t = void (*)(cpx<float>*, const float*, int, int, int, int, int, float, const cpx<float>*, hipStream_t);
+using z32_t = void (*)(cpx<float>*, const float*, int, int, int, int, int, float, const cpx<float>*, hipStream_t,
+                       int);
 struct RowV {
     int W, NT;
     r2c32_t r2c;
@@ -60,8 +61,8 @@ void launch_cols32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int np
 }
 template <int L, int W, int NT>
 void launch_z32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, float srcMul,
-                const cpx<float>* tw, hipStream_t st) {
-    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)P1);
+                const cpx<float>* tw, hipStream_t st, int nsolve) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)P1, (unsigned)nsolve);
     hipLaunchKernelGGL((k_zconv<float, float, L, W, NT>), g, dim3(NT), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul,
                        tw);
 }

@@ -22,27 +22,42 @@
 #include "philox.hpp"
 
 int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
-                                hipStream_t st);
+                                hipStream_t st, int nsolve);
+extern "C" int hs_demag_reserve(hs_demag* d, int nsolve);
 
 namespace {
 
 constexpr int TX = 64, TY = 4;
 
+// Per-solve parameters of a batch (solves share the mesh and materials).
+struct SolveP {
+    float TMul;     // temperature multiplier
+    float bext[3];  // B_ext / mu0
+    double thC;     // 2 kB TMul / (mu0 V gamma Ms0T0Mul); thermal K = thC / max(cutoff_dt, dt)
+    uint64_t seed;
+    uint32_t stream;
+    uint32_t pad;
+};
+
+// Per-solve clock and slot record (hs_slot): read by the stage kernels when
+// the step, attempt and dt live on the device (graph replay, slots).
+using Clock = hs_slot;
+constexpr int kMaxTry = 10;  // solve_rk12.go:44 (maxTry), retries of one step
+
 struct Phys {
     int N0, N1, N2, w0, w1, w2;
-    int chunk;  // planes marched by one block (grid.z = ceil(N0 / chunk))
+    int chunk;  // planes marched by one block (grid.z = nsolve * ceil(N0 / chunk))
+    int nsolve;
+    const SolveP* sp;
+    float cutoff_dt;
     int exchange, demag, zeeman, longfield, thermal, torque, local, normalize;
     float ex0, ex1, ex2;  // lexMul^2 / c_d^2
     float ms0T0Mul;
     const float* lex;
     const float* msk;  // msat0T0 mask
     float invMu0;
-    float bext[3];  // B_ext / mu0
-    float JMul, nMul, TcMul, TMul;
+    float JMul, nMul, TcMul;
     const float *J, *n, *Tc, *T;
-    float lfPre;  // kB / mu0
-    uint64_t seed;
-    uint32_t stream;
     float gamma;
     float aMul[3];
     const float* alpha[3];
@@ -67,11 +82,40 @@ struct StageIO {
     float* mout;      // next stage input
     float* kout;      // R_RHS: k
     float* hout;      // R_RHS: H_eff
-    unsigned long long* errmax;  // max |e| as double bits
+    unsigned long long* errmax;  // max |e| as double bits, one per solve
     double h;
-    float thK;        // thermal prefactor (depends on dt)
     uint32_t step, attempt;
+    const Clock* clk;  // when set, step/attempt/h of each solve are read from here
 };
+
+// Offset the buffers of `io` to solve `sol` (layout [solve][3][N]).
+__device__ __forceinline__ StageIO solve_io(const StageIO& io, int sol, long N) {
+    StageIO r = io;
+    const long b = (long)sol * 3 * N;
+    r.m += b;
+    if (r.B) r.B += b;
+    if (r.y0) r.y0 += b;
+    if (r.acc) r.acc += b;
+    if (r.e) r.e += b;
+    if (r.k1) r.k1 += b;
+    if (r.k2) r.k2 += b;
+    if (r.k3) r.k3 += b;
+    if (r.mout) r.mout += b;
+    if (r.kout) r.kout += b;
+    if (r.hout) r.hout += b;
+    r.errmax += sol;
+    if (io.clk) {
+        r.step = io.clk[sol].step;
+        r.attempt = io.clk[sol].attempt;
+        r.h = io.clk[sol].h;
+    }
+    return r;
+}
+
+// Blocks of a solve that is not running (idle / done / failed slot) exit.
+__device__ __forceinline__ bool solve_idle(const StageIO& io, int sol) {
+    return io.clk && io.clk[sol].state != HS_SLOT_RUNNING;
+}
 
 
 __device__ __forceinline__ int wrapc(int c, int n, int w) {
@@ -179,8 +223,8 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
 // Everything after the exchange term for one cell: B/mu0, B_ext/mu0, H_lf,
 // H_therm, k = gamma (m x H + alpha H) and the RK stage update (FP64 state).
 template <int ROLE>
-__device__ __forceinline__ void finish_cell(const Phys& P, const StageIO& io, long I, long N, const float m0[3],
-                                            float H[3], double& errloc) {
+__device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, const StageIO& io, long I, long N,
+                                            const float m0[3], float H[3], double& errloc) {
     const float ms = P.msk ? P.msk[I] : 1.f;
     if (P.demag) {
         H[0] += P.invMu0 * io.B[I];
@@ -188,9 +232,9 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const StageIO& io, lo
         H[2] += P.invMu0 * io.B[2 * N + I];
     }
     if (P.zeeman) {
-        H[0] += P.bext[0];
-        H[1] += P.bext[1];
-        H[2] += P.bext[2];
+        H[0] += sp.bext[0];
+        H[1] += sp.bext[1];
+        H[2] += sp.bext[2];
     }
     if (P.longfield) {
         // FP64: dB_S/dx is a difference of O(1) terms (common_func.h:95-98)
@@ -198,7 +242,7 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const StageIO& io, lo
         double S = (double)P.JMul * (P.J ? P.J[I] : 1.f);
         double n = (double)P.nMul * (P.n ? P.n[I] : 1.f);
         double Tc = (double)P.TcMul * (P.Tc ? P.Tc[I] : 1.f);
-        double Ts = (double)P.TMul * (P.T ? P.T[I] : 1.f);
+        double Ts = (double)sp.TMul * (P.T ? P.T[I] : 1.f);
         double mx = m0[0], my = m0[1], mz = m0[2];
         double am = sqrt(mx * mx + my * my + mz * mz);
         if (!(Ms0T0 == 0.0 || n == 0.0 || am <= HS_ZERO)) {
@@ -214,9 +258,10 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const StageIO& io, lo
     }
     if (P.thermal && ms != 0.f) {
         float x0, x1, x2;
-        thermal_normals_f((uint32_t)I, io.step, io.attempt, P.stream, P.seed, x0, x1, x2);
+        thermal_normals_f((uint32_t)I, io.step, io.attempt, sp.stream, sp.seed, x0, x1, x2);
         float Ti = P.T ? P.T[I] : 1.f;
-        float pre = sqrtf((Ti * io.thK) / ms);
+        float thK = (float)(sp.thC / fmax((double)P.cutoff_dt, io.h));
+        float pre = sqrtf((Ti * thK) / ms);
         float s0 = sqrtf(P.aMul[0] * (P.alpha[0] ? P.alpha[0][I] : 1.f));
         float s1 = sqrtf(P.aMul[1] * (P.alpha[1] ? P.alpha[1][I] : 1.f));
         float s2 = sqrtf(P.aMul[2] * (P.alpha[2] ? P.alpha[2][I] : 1.f));
@@ -347,7 +392,7 @@ __device__ __forceinline__ void flush_err(double errloc, const StageIO& io) {
 // Marching variant: a 64x4 tile walks planes [i0, i0+chunk) keeping planes
 // i-1, i, i+1 of m in registers (each m value is read from HBM once).
 template <int ROLE>
-__global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
+__global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
     __shared__ float tile[3][TY + 2][TX + 2];
     __shared__ float ltile[TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -355,11 +400,16 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
     const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
     const long N12 = (long)N1 * N2;
     const long N = (long)N0 * N12;
+    const int nchunk = (N0 + P.chunk - 1) / P.chunk;
+    const int sol = blockIdx.z / nchunk;
+    if (solve_idle(io0, sol)) return;
+    const StageIO io = solve_io(io0, sol, N);
+    const SolveP sp = P.sp[sol];
     const bool in = (k < N2) && (j < N1);
     const float* m = io.m;
     const int kc = min(k, N2 - 1), jc = min(j, N1 - 1);
     double errloc = 0.0;
-    const int i0 = blockIdx.z * P.chunk, i1 = min(N0, i0 + P.chunk);
+    const int i0 = (blockIdx.z % nchunk) * P.chunk, i1 = min(N0, i0 + P.chunk);
     float a[3][3];
     float la[3] = {1.f, 1.f, 1.f};
     {
@@ -395,7 +445,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
         if (P.exchange && in)
             exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, a[0], a[2], la[1], la[0], la[2], tile, ltile, H);
         __syncthreads();
-        if (in) finish_cell<ROLE>(P, io, I, N, m0, H, errloc);
+        if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, errloc);
         for (int c = 0; c < 3; c++) {
             a[0][c] = a[1][c];
             a[1][c] = a[2][c];
@@ -411,14 +461,18 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io) {
 // Flat variant: one plane per block, planes i-1 and i+1 read directly (they
 // are the interior of other blocks and come from L2 / Infinity Cache).
 template <int ROLE>
-__global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io) {
+__global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
     __shared__ float tile[3][TY + 2][TX + 2];
     __shared__ float ltile[TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int k = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty, i = blockIdx.z;
     const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
+    const int k = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty, i = blockIdx.z % N0;
+    const int sol = blockIdx.z / N0;
+    if (solve_idle(io0, sol)) return;
     const long N12 = (long)N1 * N2;
     const long N = (long)N0 * N12;
+    const StageIO io = solve_io(io0, sol, N);
+    const SolveP sp = P.sp[sol];
     const bool in = (k < N2) && (j < N1);
     const float* m = io.m;
     const int kc = min(k, N2 - 1), jc = min(j, N1 - 1);
@@ -444,7 +498,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io) {
     __syncthreads();
     float H[3] = {0.f, 0.f, 0.f};
     if (P.exchange && in) exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, lo, hi, l0, llo, lhi, tile, ltile, H);
-    if (in) finish_cell<ROLE>(P, io, I, N, m0, H, errloc);
+    if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, errloc);
     flush_err<ROLE>(errloc, io);
 }
 
@@ -454,6 +508,11 @@ __global__ void k_rk12_predict(const double* y0, const float* k1, double* acc, f
                                long N) {
     long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
+    const long b = (long)blockIdx.y * 3 * N;
+    y0 += b;
+    k1 += b;
+    acc += b;
+    mout += b;
     double v[3];
     for (int c = 0; c < 3; c++) {
         v[c] = y0[c * N + I] + h * (double)k1[c * N + I];
@@ -470,6 +529,9 @@ __global__ void k_rk12_predict(const double* y0, const float* k1, double* acc, f
 __global__ void k_set_state(const double* y, float* mf, int normalize, long N) {
     long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
+    const long b = (long)blockIdx.y * 3 * N;
+    y += b;
+    mf += b;
     double v[3] = {y[I], y[N + I], y[2 * N + I]};
     if (normalize) {
         double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -484,12 +546,106 @@ __global__ void k_d2f(const double* a, float* b, long n) {
     if (i < n) b[i] = (float)a[i];
 }
 
+__global__ void k_clock_step(Clock* clk, int nsolve) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nsolve) clk[q].step += 1;
+}
+
+// the reference's step-size controller, solve_rk12.go:164-198
+__host__ __device__ inline double controller(double dt, double err, double maxErr, bool bad, double expo_good,
+                                             double expo_bad, double mindt, double maxdt) {
+    double factor = bad ? pow(maxErr / err, expo_bad) : pow(maxErr / err, expo_good);
+    factor *= 0.8;
+    if (factor > 1.5) factor = 1.5;
+    if (factor < 0.1) factor = 0.1;
+    double nd = dt * factor;
+    if (nd < mindt) nd = mindt;
+    if (nd > maxdt) nd = maxdt;
+    return nd;
+}
+
+// Per-slot accept/reject after an RK23 attempt (one thread per slot).  The
+// state of iteration `par` is in buffer set par; an accepted attempt leaves
+// the new state in set par^1, a rejected one is copied there by k_fixup.
+__global__ void k_control(Clock* clk, unsigned long long* errmax, int nsolve, uint32_t par) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nsolve) return;
+    Clock c = clk[q];
+    if (c.state != HS_SLOT_RUNNING) return;
+    double err = __longlong_as_double((long long)errmax[q]);
+    errmax[q] = 0ull;
+    c.err = err;
+    if (!isfinite(err)) {  // the state has diverged: stop this slot
+        c.state = HS_SLOT_FAILED;
+        c.home = par;
+        c.retry = 0;
+        clk[q] = c;
+        return;
+    }
+    bool bad = err > c.maxErr;
+    double nd = controller(c.h, err, c.maxErr, bad, 1.0 / 3.0, 1.0 / 3.0, c.mindt, c.maxdt);
+    if (!bad || nd == c.mindt) {
+        c.t += c.h;
+        c.h = nd;
+        c.step += 1;
+        c.attempt = 0;
+        c.accepted += 1;
+        c.retry = 0;
+        if (c.step >= c.end) {
+            c.state = HS_SLOT_DONE;
+            c.home = par ^ 1u;
+        }
+    } else {
+        c.h = nd;
+        if (c.attempt >= (uint32_t)kMaxTry + 1) {
+            c.state = HS_SLOT_FAILED;
+            c.home = par;
+            c.retry = 0;
+        } else {
+            c.attempt += 1;
+            c.rejected += 1;
+            c.retry = 1;
+        }
+    }
+    clk[q] = c;
+}
+
+// A rejected slot restarts its step from the state of set `cur`: copy it
+// (FP64 state, FP32 stage input and its field) into set `nxt`.
+__global__ void k_fixup(const Clock* clk, const double* ycur, const float* mcur, const float* Bcur, double* ynxt,
+                        float* mnxt, float* Bnxt, long N, int demag) {
+    const int q = blockIdx.y;
+    if (clk[q].state != HS_SLOT_RUNNING || !clk[q].retry) return;
+    long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const long b = (long)q * 3 * N;
+    for (int c = 0; c < 3; c++) {
+        ynxt[b + c * N + I] = ycur[b + c * N + I];
+        mnxt[b + c * N + I] = mcur[b + c * N + I];
+        if (demag) Bnxt[b + c * N + I] = Bcur[b + c * N + I];
+    }
+}
+
 }  // namespace
 
 struct hs_solver {
     hs_llb_params p;
     hipStream_t st;
-    long N;
+    bool own_stream = false;
+    long N;       // cells per solve
+    int nsolve = 1;
+    SolveP* sp = nullptr;      // device, nsolve entries
+    Clock* clk = nullptr;      // device, nsolve entries (graph replay, slots)
+    hipGraphExec_t graph = nullptr;
+    // slots: one captured RK23 iteration per buffer-set parity
+    bool slot_mode = false;
+    long slot_iter = 0;                 // iterations enqueued (parity = slot_iter & 1)
+    hipGraphExec_t slot_graph[2] = {nullptr, nullptr};
+    hs_slot* slot_host = nullptr;       // pinned copy of the slot records
+    hipEvent_t slot_ev = nullptr;
+    std::vector<SolveP> sp_host;
+    int graph_method = -1;
+    double graph_dt = 0;
     hs_demag* demag = nullptr;
     Phys phys;
     // device buffers
@@ -521,65 +677,17 @@ float* upload_mask(hs_solver* s, const double* h) {
 }
 
 template <int ROLE>
-void launch_stage(hs_solver* s, const StageIO& io);
-void launch_stage_rk4_2(hs_solver* s, const StageIO& io) { launch_stage<R_RK4_2>(s, io); }
-template <int ROLE>
 void launch_stage(hs_solver* s, const StageIO& io) {
     const Phys& P = s->phys;
     hs::prof_begin(HS_PROF_STAGE, s->st);
     if (s->stage_flat) {
-        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), P.N0);
+        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), P.N0 * s->nsolve);
         hipLaunchKernelGGL(k_stage_flat<ROLE>, grd, blk, 0, s->st, P, io);
     } else {
-        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), hs::div_up(P.N0, P.chunk));
+        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), hs::div_up(P.N0, P.chunk) * s->nsolve);
         hipLaunchKernelGGL(k_stage<ROLE>, grd, blk, 0, s->st, P, io);
     }
     hs::prof_end(HS_PROF_STAGE, s->st);
-}
-
-// Pick the faster stage-kernel variant (flat / marching) by timing one
-// representative role on the solver's own buffers.
-void tune_stage(hs_solver* s) {
-    const char* env = getenv("HS_STAGE_KERNEL");
-    if (env && !strcmp(env, "flat")) { s->stage_flat = true; return; }
-    if (env && !strcmp(env, "march")) { s->stage_flat = false; return; }
-    if (s->phys.N0 < 2) { s->stage_flat = true; return; }
-    StageIO io{};
-    io.m = s->mA; io.B = s->B0; io.y0 = s->y0; io.acc = s->acc; io.e = s->e;
-    io.k1 = s->k1; io.k2 = s->k2; io.k3 = s->k3; io.mout = s->mB; io.errmax = s->errmax;
-    io.h = 1e-15; io.thK = 0.f;
-    hipMemsetAsync(s->mA, 0, sizeof(float) * 3 * s->N, s->st);
-    hipMemsetAsync(s->y0, 0, sizeof(double) * 3 * s->N, s->st);
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    float t[2];
-    for (int v = 0; v < 2; v++) {
-        s->stage_flat = (v == 0);
-        launch_stage_rk4_2(s, io);
-        hipEventRecord(e0, s->st);
-        for (int r = 0; r < 3; r++) launch_stage_rk4_2(s, io);
-        hipEventRecord(e1, s->st);
-        hipEventSynchronize(e1);
-        hipEventElapsedTime(&t[v], e0, e1);
-    }
-    s->stage_flat = t[0] <= t[1];
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-}
-
-void demag_into(hs_solver* s, const float* m, float* B) {
-    if (s->phys.demag) hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st);
-}
-
-float thermal_K(const hs_solver* s, double dt) {
-    // mod_brown.go:125-145: K = 2 kB TMul / (mu0 V gamma max(cutoff_dt, dt) Ms0T0Mul)
-    const hs_llb_params& p = s->p;
-    double V = p.cell[0] * p.cell[1] * p.cell[2];
-    double KB2TMul = HS_KB * 2.0 * p.TMul;
-    double deltat = fmax(p.cutoff_dt, dt);
-    double den = HS_MU0 * V * p.gammaLL * deltat * p.msat0T0Mul;
-    return (float)(KB2TMul / den);
 }
 
 StageIO base_io(hs_solver* s, double h, uint32_t step, uint32_t attempt) {
@@ -592,21 +700,62 @@ StageIO base_io(hs_solver* s, double h, uint32_t step, uint32_t attempt) {
     io.k3 = s->k3;
     io.errmax = s->errmax;
     io.h = h;
-    io.thK = s->p.thermal ? thermal_K(s, h) : 0.f;
     io.step = step;
     io.attempt = attempt;
+    io.clk = nullptr;
     return io;
 }
 
-double read_err(hs_solver* s) {
-    unsigned long long b = 0;
-    hipMemcpyAsync(&b, s->errmax, sizeof(b), hipMemcpyDeviceToHost, s->st);
-    hipStreamSynchronize(s->st);
-    double v;
-    memcpy(&v, &b, sizeof(v));
-    return v;
+// Pick the faster stage-kernel variant (flat / marching) by timing one
+// representative role on the solver's own buffers.
+void tune_stage(hs_solver* s) {
+    const char* env = getenv("HS_STAGE_KERNEL");
+    if (env && !strcmp(env, "flat")) { s->stage_flat = true; return; }
+    if (env && !strcmp(env, "march")) { s->stage_flat = false; return; }
+    if (s->phys.N0 < 2) { s->stage_flat = true; return; }
+    StageIO io = base_io(s, 1e-15, 0, 0);
+    io.m = s->mA;
+    io.B = s->B0;
+    io.mout = s->mB;
+    hipMemsetAsync(s->mA, 0, sizeof(float) * 3 * s->N * s->nsolve, s->st);
+    hipMemsetAsync(s->y0, 0, sizeof(double) * 3 * s->N * s->nsolve, s->st);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float t[2];
+    for (int v = 0; v < 2; v++) {
+        s->stage_flat = (v == 0);
+        launch_stage<R_RK4_2>(s, io);
+        hipEventRecord(e0, s->st);
+        for (int r = 0; r < 3; r++) launch_stage<R_RK4_2>(s, io);
+        hipEventRecord(e1, s->st);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&t[v], e0, e1);
+    }
+    s->stage_flat = t[0] <= t[1];
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
 }
-void zero_err(hs_solver* s) { hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long), s->st); }
+
+void demag_into(hs_solver* s, const float* m, float* B) {
+    if (s->phys.demag)
+        hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st, s->nsolve);
+}
+
+// max over the batch of the per-solve error maxima (lockstep stepping)
+double read_err(hs_solver* s) {
+    std::vector<unsigned long long> b(s->nsolve);
+    hipMemcpyAsync(b.data(), s->errmax, sizeof(unsigned long long) * s->nsolve, hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    double m = 0.0;
+    for (int q = 0; q < s->nsolve; q++) {
+        double v;
+        memcpy(&v, &b[q], sizeof(v));
+        m = std::max(m, v);
+    }
+    return m;
+}
+void zero_err(hs_solver* s) { hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * s->nsolve, s->st); }
 
 void ensure_b0(hs_solver* s) {
     if (!s->b0_valid) {
@@ -615,65 +764,175 @@ void ensure_b0(hs_solver* s) {
     }
 }
 
-// the reference's step-size controller, solve_rk12.go:164-198
-double controller(double dt, double err, double maxErr, bool bad, double expo_good, double expo_bad,
-                  double mindt, double maxdt) {
-    double factor = bad ? pow(maxErr / err, expo_bad) : pow(maxErr / err, expo_good);
-    factor *= 0.8;
-    if (factor > 1.5) factor = 1.5;
-    if (factor < 0.1) factor = 0.1;
-    double nd = dt * factor;
-    if (nd < mindt) nd = mindt;
-    if (nd > maxdt) nd = maxdt;
-    return nd;
+// One fixed-size RK4 step of every solve (4 demag evaluations, 4 stage kernels).
+void rk4_step(hs_solver* s, StageIO io) {
+    demag_into(s, s->mA, s->B0);
+    io.m = s->mA; io.B = s->B0; io.mout = s->mB;
+    launch_stage<R_RK4_1>(s, io);
+    demag_into(s, s->mB, s->Bs);
+    io.m = s->mB; io.B = s->Bs; io.mout = s->mC;
+    launch_stage<R_RK4_2>(s, io);
+    demag_into(s, s->mC, s->Bs);
+    io.m = s->mC; io.B = s->Bs; io.mout = s->mB;
+    launch_stage<R_RK4_3>(s, io);
+    demag_into(s, s->mB, s->Bs);
+    io.m = s->mB; io.B = s->Bs; io.mout = s->mA;
+    launch_stage<R_RK4_4>(s, io);
+}
+
+// The launches of one RK4 step of the whole batch with step/dt read from the
+// device clocks, followed by the clock advance (graph body).
+void rk4_graph_step(hs_solver* s) {
+    StageIO io = base_io(s, 0.0, 0, 0);
+    io.clk = s->clk;
+    rk4_step(s, io);
+    hipLaunchKernelGGL(k_clock_step, hs::div_up(s->nsolve, 256), 256, 0, s->st, s->clk, s->nsolve);
+}
+
+// Every solve running at step counter step0 with dt (fixed-step replay).
+void set_clocks(hs_solver* s, long step0, double dt) {
+    std::vector<Clock> hc(s->nsolve);
+    for (auto& c : hc) {
+        memset(&c, 0, sizeof(c));
+        c.step = (uint32_t)step0;
+        c.h = dt;
+        c.state = HS_SLOT_RUNNING;
+    }
+    hipMemcpyAsync(s->clk, hc.data(), sizeof(Clock) * s->nsolve, hipMemcpyHostToDevice, s->st);
+    hipStreamSynchronize(s->st);
+}
+
+// Buffer set `par` of slot mode: (FP64 state, FP32 stage input, its field).
+struct Set {
+    double* y;
+    float* m;
+    float* B;
+};
+Set slot_set(hs_solver* s, int par) {
+    return par ? Set{s->acc, s->mB, s->Bs} : Set{s->y0, s->mA, s->B0};
+}
+
+// One RK23 attempt of every running slot with the state in set par: the
+// stages leave the candidate in set par^1, k_control decides per slot and
+// k_fixup restores the rejected slots' state into set par^1.
+void slot_iteration(hs_solver* s, int par) {
+    const Set cur = slot_set(s, par), nxt = slot_set(s, par ^ 1);
+    StageIO io = base_io(s, 0.0, 0, 0);
+    io.clk = s->clk;
+    io.y0 = cur.y;
+    io.acc = nxt.y;
+    io.m = cur.m; io.B = cur.B; io.mout = nxt.m;
+    launch_stage<R_BS_1>(s, io);
+    demag_into(s, nxt.m, nxt.B);
+    io.m = nxt.m; io.B = nxt.B; io.mout = s->mC;
+    launch_stage<R_BS_2>(s, io);
+    demag_into(s, s->mC, nxt.B);
+    io.m = s->mC; io.B = nxt.B; io.mout = nxt.m;
+    launch_stage<R_BS_3>(s, io);
+    demag_into(s, nxt.m, nxt.B);
+    io.m = nxt.m; io.B = nxt.B; io.mout = nullptr;
+    launch_stage<R_BS_4>(s, io);
+    hipLaunchKernelGGL(k_control, hs::div_up(s->nsolve, 256), 256, 0, s->st, s->clk, s->errmax, s->nsolve,
+                       (uint32_t)par);
+    hipLaunchKernelGGL(k_fixup, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->clk, cur.y, cur.m,
+                       cur.B, nxt.y, nxt.m, nxt.B, s->N, s->phys.demag);
+}
+
+bool capture(hs_solver* s, hipGraphExec_t* out, const char* who, void (*body)(hs_solver*, int), int arg) {
+    bool prof = hs::prof_on();
+    if (prof) hs_prof_enable(0);
+    hipGraph_t g;
+    bool ok = hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        body(s, arg);
+        ok = hipStreamEndCapture(s->st, &g) == hipSuccess;
+        if (ok) {
+            ok = hipGraphInstantiate(out, g, nullptr, nullptr, 0) == hipSuccess;
+            hipGraphDestroy(g);
+        }
+    }
+    if (prof) hs_prof_enable(1);
+    if (!ok) {
+        *out = nullptr;
+        hs::set_error(HS_EINVAL, who);
+    }
+    return ok;
 }
 }  // namespace
 
-extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) {
-    if (!p || p->N[0] < 1 || p->N[1] < 1 || p->N[2] < 1) {
-        hs::set_error(HS_EINVAL, "hs_solver_create: bad mesh");
+// A batch of `nsolve` solves on one mesh: per-solve TMul, seed, B_ext.
+extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const double* TMul, const uint64_t* seed,
+                                      const double* Bext, CUstream stream) {
+    if (!p || p->N[0] < 1 || p->N[1] < 1 || p->N[2] < 1 || nsolve < 1) {
+        hs::set_error(HS_EINVAL, "hs_batch_create: bad mesh or batch size");
         return nullptr;
     }
     hs_solver* s = new hs_solver();
     s->p = *p;
+    s->nsolve = nsolve;
     s->st = hs::S(stream);
+    if (!s->st) {  // own a real stream so a step can be captured into a graph
+        hipStreamCreate(&s->st);
+        s->own_stream = true;
+    }
     s->N = (long)p->N[0] * p->N[1] * p->N[2];
-    const long N = s->N;
-    hipMalloc(&s->y0, sizeof(double) * 3 * N);
-    hipMalloc(&s->acc, sizeof(double) * 3 * N);
-    hipMalloc(&s->mA, sizeof(float) * 3 * N);
-    hipMalloc(&s->mB, sizeof(float) * 3 * N);
-    hipMalloc(&s->mC, sizeof(float) * 3 * N);
-    hipMalloc(&s->B0, sizeof(float) * 3 * N);
-    hipMalloc(&s->Bs, sizeof(float) * 3 * N);
-    hipMalloc(&s->e, sizeof(float) * 3 * N);
-    hipMalloc(&s->k1, sizeof(float) * 3 * N);
-    hipMalloc(&s->k2, sizeof(float) * 3 * N);
-    hipMalloc(&s->k3, sizeof(float) * 3 * N);
-    hipMalloc(&s->errmax, sizeof(unsigned long long));
+    const long NN = s->N * nsolve;
+    hipMalloc(&s->y0, sizeof(double) * 3 * NN);
+    hipMalloc(&s->acc, sizeof(double) * 3 * NN);
+    hipMalloc(&s->mA, sizeof(float) * 3 * NN);
+    hipMalloc(&s->mB, sizeof(float) * 3 * NN);
+    hipMalloc(&s->mC, sizeof(float) * 3 * NN);
+    hipMalloc(&s->B0, sizeof(float) * 3 * NN);
+    hipMalloc(&s->Bs, sizeof(float) * 3 * NN);
+    hipMalloc(&s->e, sizeof(float) * 3 * NN);
+    hipMalloc(&s->k1, sizeof(float) * 3 * NN);
+    hipMalloc(&s->k2, sizeof(float) * 3 * NN);
+    hipMalloc(&s->k3, sizeof(float) * 3 * NN);
+    hipMalloc(&s->errmax, sizeof(unsigned long long) * nsolve);
+    hipMalloc(&s->sp, sizeof(SolveP) * nsolve);
+    hipMalloc(&s->clk, sizeof(Clock) * nsolve);
     if (!s->y0 || !s->acc || !s->mA || !s->mB || !s->mC || !s->B0 || !s->Bs || !s->e || !s->k1 || !s->k2 ||
-        !s->k3 || !s->errmax) {
-        hs::set_error(HS_ENOMEM, "hs_solver_create: out of device memory");
+        !s->k3 || !s->errmax || !s->sp || !s->clk) {
+        hs::set_error(HS_ENOMEM, "hs_batch_create: out of device memory");
         hs_solver_destroy(s);
         return nullptr;
     }
-    hipMemsetAsync(s->B0, 0, sizeof(float) * 3 * N, s->st);
-    hipMemsetAsync(s->Bs, 0, sizeof(float) * 3 * N, s->st);
+    hipMemsetAsync(s->B0, 0, sizeof(float) * 3 * NN, s->st);
+    hipMemsetAsync(s->Bs, 0, sizeof(float) * 3 * NN, s->st);
+    hipMemsetAsync(s->clk, 0, sizeof(Clock) * nsolve, s->st);
+    hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * nsolve, s->st);
+    // per-solve parameters; thermal K = thC / max(cutoff_dt, dt), mod_brown.go:125-145
+    std::vector<SolveP> hsp(nsolve);
+    const double V = p->cell[0] * p->cell[1] * p->cell[2];
+    for (int q = 0; q < nsolve; q++) {
+        double T = TMul ? TMul[q] : p->TMul;
+        hsp[q].TMul = (float)T;
+        for (int c = 0; c < 3; c++) hsp[q].bext[c] = (float)(1.0 / HS_MU0 * (Bext ? Bext[3 * q + c] : p->Bext[c]));
+        hsp[q].thC = (HS_KB * 2.0 * T) / (HS_MU0 * V * p->gammaLL * p->msat0T0Mul);
+        hsp[q].seed = seed ? seed[q] : p->seed;
+        hsp[q].stream = p->stream + (uint32_t)(seed ? 0 : q);
+        hsp[q].pad = 0;
+    }
+    hipMemcpy(s->sp, hsp.data(), sizeof(SolveP) * nsolve, hipMemcpyHostToDevice);
+    s->sp_host = hsp;
     Phys& P = s->phys;
     P.N0 = p->N[0];
     P.N1 = p->N[1];
     P.N2 = p->N[2];
     P.w0 = p->pbc[0] != 0;
+    P.w1 = p->pbc[1] != 0;
+    P.w2 = p->pbc[2] != 0;
+    P.nsolve = nsolve;
+    P.sp = s->sp;
+    P.cutoff_dt = (float)p->cutoff_dt;
     {
         // enough blocks to fill 256 CUs several times over: split the planes into chunks
-        long blocks2d = (long)hs::div_up(P.N2, TX) * hs::div_up(P.N1, TY);
+        long blocks2d = (long)hs::div_up(P.N2, TX) * hs::div_up(P.N1, TY) * nsolve;
         long chunks = (4096 + blocks2d - 1) / blocks2d;
         if (chunks < 1) chunks = 1;
         if (chunks > P.N0) chunks = P.N0;
         P.chunk = (int)((P.N0 + chunks - 1) / chunks);
     }
-    P.w1 = p->pbc[1] != 0;
-    P.w2 = p->pbc[2] != 0;
     P.exchange = p->exchange;
     P.demag = p->demag;
     P.zeeman = p->zeeman;
@@ -690,55 +949,73 @@ extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) 
     P.lex = upload_mask(s, p->lex);
     P.msk = upload_mask(s, p->msat0T0);
     P.invMu0 = (float)(1.0 / HS_MU0);
-    for (int c = 0; c < 3; c++) P.bext[c] = (float)(1.0 / HS_MU0 * p->Bext[c]);
     P.JMul = (float)p->JMul;
     P.nMul = (float)p->nMul;
     P.TcMul = (float)p->TcMul;
-    P.TMul = (float)p->TMul;
     P.J = upload_mask(s, p->J);
     P.n = upload_mask(s, p->n);
     P.Tc = upload_mask(s, p->Tc);
     P.T = upload_mask(s, p->T);
-    P.lfPre = (float)(HS_KB / HS_MU0);
-    P.seed = p->seed;
-    P.stream = p->stream;
     P.gamma = (float)p->gammaLL;
     for (int c = 0; c < 3; c++) {
         P.aMul[c] = (float)p->alphaMul[c];
         P.alpha[c] = upload_mask(s, p->alpha[c]);
     }
     if (p->demag) {
-        s->demag = hs_demag_create(p->N, p->cell, p->pbc, p->demag_accuracy > 0 ? p->demag_accuracy : 6, stream);
-        if (!s->demag) {
+        s->demag = hs_demag_create(p->N, p->cell, p->pbc, p->demag_accuracy > 0 ? p->demag_accuracy : 6,
+                                   (CUstream)s->st);
+        if (!s->demag || hs_demag_reserve(s->demag, nsolve) != 0) {
             hs_solver_destroy(s);
             return nullptr;
         }
     }
     tune_stage(s);
     hipStreamSynchronize(s->st);
-    HS_LAUNCH_CHECK("hs_solver_create");
+    HS_LAUNCH_CHECK("hs_batch_create");
+    if (hs_last_error()) {
+        hs_solver_destroy(s);
+        return nullptr;
+    }
     return s;
+}
+
+extern "C" hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream) {
+    return hs_batch_create(p, 1, nullptr, nullptr, nullptr, stream);
 }
 
 extern "C" void hs_solver_destroy(hs_solver* s) {
     if (!s) return;
+    if (s->graph) hipGraphExecDestroy(s->graph);
+    for (int v = 0; v < 2; v++)
+        if (s->slot_graph[v]) hipGraphExecDestroy(s->slot_graph[v]);
+    if (s->slot_ev) hipEventDestroy(s->slot_ev);
+    if (s->slot_host) hipHostFree(s->slot_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
-                  s->errmax};
+                  s->errmax, s->sp, s->clk};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
+    if (s->own_stream) hipStreamDestroy(s->st);
     delete s;
 }
 
 extern "C" CUstream hs_solver_stream(hs_solver* s) { return s ? (CUstream)s->st : nullptr; }
 extern "C" hs_demag* hs_solver_demag(hs_solver* s) { return s ? s->demag : nullptr; }
 extern "C" int hs_solver_stage_variant(hs_solver* s) { return s ? (s->stage_flat ? 1 : 0) : -1; }
+extern "C" int hs_solver_nsolve(hs_solver* s) { return s ? s->nsolve : -1; }
+
+static bool whole_batch_ok(hs_solver* s, const char* who) {
+    if (s->slot_mode) hs::set_error(HS_EINVAL, who);
+    return !s->slot_mode;
+}
 
 extern "C" int hs_solver_set_m(hs_solver* s, const double* m) {
     if (!s || !m) return HS_EINVAL;
-    hipMemcpyAsync(s->y0, m, sizeof(double) * 3 * s->N, hipMemcpyHostToDevice, s->st);
-    hipLaunchKernelGGL(k_set_state, hs::grid1d(s->N), hs::kBlock, 0, s->st, s->y0, s->mA, s->p.normalize, s->N);
+    if (!whole_batch_ok(s, "hs_solver_set_m: handle is in slot mode")) return HS_EINVAL;
+    hipMemcpyAsync(s->y0, m, sizeof(double) * 3 * s->N * s->nsolve, hipMemcpyHostToDevice, s->st);
+    hipLaunchKernelGGL(k_set_state, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->y0, s->mA,
+                       s->p.normalize, s->N);
     s->b0_valid = false;
     hipStreamSynchronize(s->st);
     HS_LAUNCH_CHECK("hs_solver_set_m");
@@ -747,7 +1024,8 @@ extern "C" int hs_solver_set_m(hs_solver* s, const double* m) {
 
 extern "C" int hs_solver_get_m(hs_solver* s, double* m) {
     if (!s || !m) return HS_EINVAL;
-    hipMemcpyAsync(m, s->y0, sizeof(double) * 3 * s->N, hipMemcpyDeviceToHost, s->st);
+    if (!whole_batch_ok(s, "hs_solver_get_m: handle is in slot mode")) return HS_EINVAL;
+    hipMemcpyAsync(m, s->y0, sizeof(double) * 3 * s->N * s->nsolve, hipMemcpyDeviceToHost, s->st);
     hipStreamSynchronize(s->st);
     HS_LAUNCH_CHECK("hs_solver_get_m");
     return hs_last_error();
@@ -756,10 +1034,10 @@ extern "C" int hs_solver_get_m(hs_solver* s, double* m) {
 extern "C" int hs_solver_rhs(hs_solver* s, double* k_host, double* h_host, uint32_t step, uint32_t attempt,
                              double dt) {
     if (!s) return HS_EINVAL;
-    const long N = s->N;
+    const long NN = s->N * s->nsolve;
     if (!s->kout) {
-        hipMalloc(&s->kout, sizeof(float) * 3 * N);
-        hipMalloc(&s->hout, sizeof(float) * 3 * N);
+        hipMalloc(&s->kout, sizeof(float) * 3 * NN);
+        hipMalloc(&s->hout, sizeof(float) * 3 * NN);
     }
     ensure_b0(s);
     StageIO io = base_io(s, dt, step, attempt);
@@ -768,43 +1046,59 @@ extern "C" int hs_solver_rhs(hs_solver* s, double* k_host, double* h_host, uint3
     io.kout = s->kout;
     io.hout = s->hout;
     launch_stage<R_RHS>(s, io);
-    std::vector<float> tmp(3 * N);
+    std::vector<float> tmp(3 * NN);
     if (k_host) {
-        hipMemcpyAsync(tmp.data(), s->kout, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s->st);
+        hipMemcpyAsync(tmp.data(), s->kout, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, s->st);
         hipStreamSynchronize(s->st);
-        for (long i = 0; i < 3 * N; i++) k_host[i] = tmp[i];
+        for (long i = 0; i < 3 * NN; i++) k_host[i] = tmp[i];
     }
     if (h_host) {
-        hipMemcpyAsync(tmp.data(), s->hout, sizeof(float) * 3 * N, hipMemcpyDeviceToHost, s->st);
+        hipMemcpyAsync(tmp.data(), s->hout, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, s->st);
         hipStreamSynchronize(s->st);
-        for (long i = 0; i < 3 * N; i++) h_host[i] = tmp[i];
+        for (long i = 0; i < 3 * NN; i++) h_host[i] = tmp[i];
     }
     HS_LAUNCH_CHECK("hs_solver_rhs");
+    return hs_last_error();
+}
+
+// Fixed-step RK4 replayed from a captured hipGraph: one step (4 demag
+// evaluations, 4 stage kernels, clock advance) is captured once and
+// launched nsteps times; the thermal-noise step counter lives on the device.
+extern "C" int hs_solver_run_graph(hs_solver* s, long nsteps, long step0, double dt) {
+    if (!s || nsteps < 0) return HS_EINVAL;
+    if (s->slot_mode) {
+        hs::set_error(HS_EINVAL, "hs_solver_run_graph: handle is in slot mode");
+        return HS_EINVAL;
+    }
+    if (s->graph && (s->graph_dt != dt)) {
+        hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    set_clocks(s, step0, dt);
+    if (!s->graph) {
+        if (!capture(s, &s->graph, "hs_solver_run_graph: graph capture failed",
+                     [](hs_solver* q, int) { rk4_graph_step(q); }, 0))
+            return HS_EINVAL;
+        s->graph_dt = dt;
+    }
+    for (long n = 0; n < nsteps; n++) hipGraphLaunch(s->graph, s->st);
+    s->b0_valid = false;
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_run_graph");
     return hs_last_error();
 }
 
 extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, double* dtio, double maxErr,
                              double mindt, double maxdt, hs_run_stats* stats) {
     if (!s || !dtio || nsteps < 0) return HS_EINVAL;
+    if (!whole_batch_ok(s, "hs_solver_run: handle is in slot mode")) return HS_EINVAL;
     hs_run_stats st{};
     double dt = *dtio;
     const int maxTry = 10;
     for (long n = 0; n < nsteps; n++) {
         uint32_t step = (uint32_t)(step0 + n);
         if (method == HS_RK4) {
-            ensure_b0(s);
-            StageIO io = base_io(s, dt, step, 0);
-            io.m = s->mA; io.B = s->B0; io.mout = s->mB;
-            launch_stage<R_RK4_1>(s, io);
-            demag_into(s, s->mB, s->Bs);
-            io.m = s->mB; io.B = s->Bs; io.mout = s->mC;
-            launch_stage<R_RK4_2>(s, io);
-            demag_into(s, s->mC, s->Bs);
-            io.m = s->mC; io.B = s->Bs; io.mout = s->mB;
-            launch_stage<R_RK4_3>(s, io);
-            demag_into(s, s->mB, s->Bs);
-            io.m = s->mB; io.B = s->Bs; io.mout = s->mA;
-            launch_stage<R_RK4_4>(s, io);
+            rk4_step(s, base_io(s, dt, step, 0));
             s->b0_valid = false;
             st.rhs_evals += 4;
             st.t += dt;
@@ -837,6 +1131,11 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
                 launch_stage<R_BS_4>(s, io);
                 st.rhs_evals += 3;
                 err = read_err(s);
+                if (!std::isfinite(err)) {
+                    hs::set_error(HS_ECONV, "hs_solver_run: RK23 error estimate is not finite");
+                    if (stats) *stats = st;
+                    return HS_ECONV;
+                }
                 bool bad = err > maxErr;
                 double nd = controller(dt, err, maxErr, bad, 1.0 / 3.0, 1.0 / 3.0, mindt, maxdt);
                 double used = dt;
@@ -869,8 +1168,8 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
             launch_stage<R_RK12_1>(s, io);
             for (;;) {
                 if (tr > 0) {
-                    hipLaunchKernelGGL(k_rk12_predict, hs::grid1d(s->N), hs::kBlock, 0, s->st, s->y0, s->e, s->acc,
-                                       s->mB, dt, s->p.normalize, s->N);
+                    hipLaunchKernelGGL(k_rk12_predict, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->y0,
+                                       s->e, s->acc, s->mB, dt, s->p.normalize, s->N);
                 }
                 demag_into(s, s->mB, s->Bs);
                 zero_err(s);
@@ -883,6 +1182,11 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
                     break;
                 }
                 err = read_err(s) * dt;
+                if (!std::isfinite(err)) {
+                    hs::set_error(HS_ECONV, "hs_solver_run: RK12 error estimate is not finite");
+                    if (stats) *stats = st;
+                    return HS_ECONV;
+                }
                 bool bad = err > maxErr;
                 double nd = controller(dt, err, maxErr, bad, 0.5, 1.0 / 3.0, mindt, maxdt);
                 double used = dt;
@@ -916,4 +1220,231 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
     HS_LAUNCH_CHECK("hs_solver_run");
     if (stats) *stats = st;
     return hs_last_error();
+}
+
+// ---------------------------------------------------------------- slots
+extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMul, uint64_t seed, uint32_t stream,
+                             const double* Bext, long step0, long nsteps, double dt, double maxErr, double mindt,
+                             double maxdt) {
+    if (!s || !m || slot < 0 || slot >= s->nsolve || nsteps < 0 || !(dt > 0.0) || !(maxErr > 0.0)) {
+        hs::set_error(HS_EINVAL, "hs_slot_admit: bad argument");
+        return HS_EINVAL;
+    }
+    if (!s->slot_mode) {
+        // first admission: every slot idle, pinned record buffer, graphs captured lazily
+        hipStreamSynchronize(s->st);
+        std::vector<Clock> hc(s->nsolve);
+        memset(hc.data(), 0, sizeof(Clock) * s->nsolve);
+        hipMemcpy(s->clk, hc.data(), sizeof(Clock) * s->nsolve, hipMemcpyHostToDevice);
+        hipMemset(s->errmax, 0, sizeof(unsigned long long) * s->nsolve);
+        if (hipHostMalloc((void**)&s->slot_host, sizeof(hs_slot) * s->nsolve) != hipSuccess ||
+            hipEventCreateWithFlags(&s->slot_ev, hipEventDisableTiming) != hipSuccess) {
+            hs::set_error(HS_ENOMEM, "hs_slot_admit: pinned host memory");
+            return HS_ENOMEM;
+        }
+        memset(s->slot_host, 0, sizeof(hs_slot) * s->nsolve);
+        s->slot_mode = true;
+        s->slot_iter = 0;
+    }
+    hipStreamSynchronize(s->st);
+    Clock c;
+    hipMemcpy(&c, s->clk + slot, sizeof(Clock), hipMemcpyDeviceToHost);
+    if (c.state == HS_SLOT_RUNNING) {
+        hs::set_error(HS_EINVAL, "hs_slot_admit: slot is running");
+        return HS_EINVAL;
+    }
+    // per-solve parameters (thermal prefactor as hs_batch_create)
+    const hs_llb_params* p = &s->p;
+    SolveP q = s->sp_host[slot];
+    const double V = p->cell[0] * p->cell[1] * p->cell[2];
+    q.TMul = (float)TMul;
+    for (int k = 0; k < 3; k++) q.bext[k] = (float)(1.0 / HS_MU0 * (Bext ? Bext[k] : p->Bext[k]));
+    q.thC = (HS_KB * 2.0 * TMul) / (HS_MU0 * V * p->gammaLL * p->msat0T0Mul);
+    q.seed = seed;
+    q.stream = stream;
+    s->sp_host[slot] = q;
+    hipMemcpy(s->sp + slot, &q, sizeof(SolveP), hipMemcpyHostToDevice);
+    // state into the set that the next iteration reads, its FP32 copy and field
+    const Set cur = slot_set(s, (int)(s->slot_iter & 1));
+    const long off = (long)slot * 3 * s->N;
+    hipMemcpy(cur.y + off, m, sizeof(double) * 3 * s->N, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_set_state, dim3(hs::grid1d(s->N), 1), hs::kBlock, 0, s->st, cur.y + off, cur.m + off,
+                       s->p.normalize, s->N);
+    if (s->phys.demag)
+        hs_demag_field_f32_internal(s->demag, cur.m + off, s->phys.msk, s->p.msat0T0Mul, cur.B + off, s->st, 1);
+    memset(&c, 0, sizeof(c));
+    c.step = (uint32_t)step0;
+    c.h = dt;
+    c.maxErr = maxErr;
+    c.mindt = mindt;
+    c.maxdt = maxdt;
+    c.end = (uint32_t)(step0 + nsteps);
+    c.state = nsteps > 0 ? HS_SLOT_RUNNING : HS_SLOT_DONE;
+    c.home = (uint32_t)(s->slot_iter & 1);
+    hipMemcpyAsync(s->clk + slot, &c, sizeof(Clock), hipMemcpyHostToDevice, s->st);
+    hipStreamSynchronize(s->st);
+    s->slot_host[slot] = c;
+    HS_LAUNCH_CHECK("hs_slot_admit");
+    return hs_last_error();
+}
+
+extern "C" int hs_slot_advance(hs_solver* s, int iters) {
+    if (!s || iters < 0 || !s->slot_mode) {
+        hs::set_error(HS_EINVAL, "hs_slot_advance: no slot admitted");
+        return HS_EINVAL;
+    }
+    for (int v = 0; v < 2; v++)
+        if (!s->slot_graph[v] &&
+            !capture(s, &s->slot_graph[v], "hs_slot_advance: graph capture failed", slot_iteration, v))
+            return HS_EINVAL;
+    for (int n = 0; n < iters; n++) {
+        hipGraphLaunch(s->slot_graph[s->slot_iter & 1], s->st);
+        s->slot_iter++;
+    }
+    hipMemcpyAsync(s->slot_host, s->clk, sizeof(Clock) * s->nsolve, hipMemcpyDeviceToHost, s->st);
+    hipEventRecord(s->slot_ev, s->st);
+    HS_LAUNCH_CHECK("hs_slot_advance");
+    return hs_last_error();
+}
+
+extern "C" int hs_slot_ready(hs_solver* s) {
+    if (!s || !s->slot_ev) return 1;
+    return hipEventQuery(s->slot_ev) == hipSuccess ? 1 : 0;
+}
+
+extern "C" int hs_slot_status(hs_solver* s, hs_slot* out) {
+    if (!s || !out || !s->slot_mode) {
+        hs::set_error(HS_EINVAL, "hs_slot_status: no slot admitted");
+        return HS_EINVAL;
+    }
+    hipEventSynchronize(s->slot_ev);
+    memcpy(out, s->slot_host, sizeof(hs_slot) * s->nsolve);
+    HS_LAUNCH_CHECK("hs_slot_status");
+    return hs_last_error();
+}
+
+extern "C" int hs_slot_get_m(hs_solver* s, int slot, double* m) {
+    if (!s || !m || !s->slot_mode || slot < 0 || slot >= s->nsolve) {
+        hs::set_error(HS_EINVAL, "hs_slot_get_m: bad argument");
+        return HS_EINVAL;
+    }
+    hipStreamSynchronize(s->st);
+    Clock c;
+    hipMemcpy(&c, s->clk + slot, sizeof(Clock), hipMemcpyDeviceToHost);
+    const int set = c.state == HS_SLOT_RUNNING ? (int)(s->slot_iter & 1) : (int)c.home;
+    hipMemcpy(m, slot_set(s, set).y + (long)slot * 3 * s->N, sizeof(double) * 3 * s->N, hipMemcpyDeviceToHost);
+    HS_LAUNCH_CHECK("hs_slot_get_m");
+    return hs_last_error();
+}
+
+extern "C" int hs_slot_release(hs_solver* s, int slot) {
+    if (!s || !s->slot_mode || slot < 0 || slot >= s->nsolve) {
+        hs::set_error(HS_EINVAL, "hs_slot_release: bad argument");
+        return HS_EINVAL;
+    }
+    hipStreamSynchronize(s->st);
+    Clock c;
+    memset(&c, 0, sizeof(c));
+    hipMemcpy(s->clk + slot, &c, sizeof(Clock), hipMemcpyHostToDevice);
+    s->slot_host[slot] = c;
+    HS_LAUNCH_CHECK("hs_slot_release");
+    return hs_last_error();
+}
+
+long hs_demag_footprint(const hs_demag* d);
+
+extern "C" long hs_solver_footprint(hs_solver* s) {
+    if (!s) return 0;
+    const long NN = s->N * s->nsolve;
+    long b = NN * 3 * (2 * sizeof(double) + 9 * sizeof(float));  // y0, acc; mA..mC, B0, Bs, e, k1..k3
+    b += (long)s->nsolve * (sizeof(SolveP) + sizeof(Clock) + sizeof(unsigned long long));
+    if (s->kout) b += NN * 3 * 2 * sizeof(float);
+    b += (long)s->masks.size() * s->N * sizeof(float);
+    if (s->demag) b += hs_demag_footprint(s->demag);
+    return b;
+}
+
+// ---------------------------------------------------------------- ensembles
+struct hs_ensemble {
+    std::vector<hs_solver*> b;
+    hipStream_t origin = nullptr;
+    hipGraphExec_t graph = nullptr;
+    double dt = 0.0;
+    std::vector<hipEvent_t> ev;  // fork + one join per batch
+};
+
+extern "C" hs_ensemble* hs_ensemble_create(hs_solver* const* batches, int nbatch) {
+    if (!batches || nbatch < 1) {
+        hs::set_error(HS_EINVAL, "hs_ensemble_create: no batches");
+        return nullptr;
+    }
+    hs_ensemble* e = new hs_ensemble();
+    for (int i = 0; i < nbatch; i++) {
+        if (!batches[i] || batches[i]->slot_mode) {
+            hs::set_error(HS_EINVAL, "hs_ensemble_create: bad batch");
+            delete e;
+            return nullptr;
+        }
+        for (int j = 0; j < i; j++)
+            if (batches[j]->st == batches[i]->st) {
+                hs::set_error(HS_EINVAL, "hs_ensemble_create: batches must have distinct streams");
+                delete e;
+                return nullptr;
+            }
+        e->b.push_back(batches[i]);
+    }
+    hipStreamCreate(&e->origin);
+    e->ev.resize(nbatch + 1);
+    for (auto& v : e->ev) hipEventCreateWithFlags(&v, hipEventDisableTiming);
+    return e;
+}
+
+extern "C" int hs_ensemble_run_graph(hs_ensemble* e, long nsteps, long step0, double dt) {
+    if (!e || nsteps < 0) return HS_EINVAL;
+    if (e->graph && e->dt != dt) {
+        hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    for (hs_solver* s : e->b) set_clocks(s, step0, dt);
+    if (!e->graph) {
+        bool prof = hs::prof_on();
+        if (prof) hs_prof_enable(0);
+        hipGraph_t g;
+        bool ok = hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            hipEventRecord(e->ev[0], e->origin);
+            for (size_t i = 0; i < e->b.size(); i++) {
+                hs_solver* s = e->b[i];
+                hipStreamWaitEvent(s->st, e->ev[0], 0);
+                rk4_graph_step(s);
+                hipEventRecord(e->ev[i + 1], s->st);
+                hipStreamWaitEvent(e->origin, e->ev[i + 1], 0);
+            }
+            ok = hipStreamEndCapture(e->origin, &g) == hipSuccess;
+            if (ok) {
+                ok = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0) == hipSuccess;
+                hipGraphDestroy(g);
+            }
+        }
+        if (prof) hs_prof_enable(1);
+        if (!ok) {
+            e->graph = nullptr;
+            hs::set_error(HS_EINVAL, "hs_ensemble_run_graph: graph capture failed");
+            return HS_EINVAL;
+        }
+        e->dt = dt;
+    }
+    for (long n = 0; n < nsteps; n++) hipGraphLaunch(e->graph, e->origin);
+    hipStreamSynchronize(e->origin);
+    for (hs_solver* s : e->b) s->b0_valid = false;
+    HS_LAUNCH_CHECK("hs_ensemble_run_graph");
+    return hs_last_error();
+}
+
+extern "C" void hs_ensemble_destroy(hs_ensemble* e) {
+    if (!e) return;
+    if (e->graph) hipGraphExecDestroy(e->graph);
+    for (auto& v : e->ev) hipEventDestroy(v);
+    if (e->origin) hipStreamDestroy(e->origin);
+    delete e;
 }

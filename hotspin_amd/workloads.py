@@ -1,0 +1,129 @@
+"""Synthetic LLB-r workloads on one MI355X: material parameters, initial
+states and request streams.
+
+Material of the LLB-r recipes (examples/p2f.py, examples/laser-demag.py of
+the reference): msat0T0, lex = sqrt(2A/(mu0 Ms0^2)), J (spin), n = Ms0 /
+(g muB J) with g = 2, Tc, T, lambda on all three damping axes, gamma_LL =
+2.211e5.  Meshes are in internal (z, y, x) order: a user mesh X x Y x Z is
+N = (Z, Y, X).
+
+- `single()`: one 512x512x16 solve at 300 K, 2 nm cubes, RK23 tol 1e-5.
+- `ensemble()` (material MATERIAL_WIDE_T): 512 solves, seeds 0..511, meshes drawn per seed from
+  {32x32x32, 64x64x4, 128x128x2, 96x48x8}, 3 nm cells, T ~ U(100, 1000) K,
+  fixed RK4 at dt = 1e-15 s.
+- `requests()` (material MATERIAL_WIDE_T): Poisson arrivals at 200 req/s; cell count log-uniform in
+  [16^3, 256x256x4] mapped to the nearest power-of-two shape class; T ~
+  U(0, 1.2 Tc) (clamped below at 0.1 Tc, where the mean-field longitudinal
+  susceptibility vanishes like exp(-Tc/T) and the explicit LLB-r step has
+  no finite stable dt); B_ext components ~ U(-100, 100) mT; 200..5000 RK23
+  steps at tolerance 1e-5.
+"""
+import math
+
+import numpy as np
+
+MU0 = 4e-7 * math.pi
+MUB = 9.27400968202020202e-24
+GAMMA0 = 2.211e5
+
+MATERIAL = dict(Ms0=1.7e6, Tc=1044.0, A=2.1e-11, lam=0.01, J=0.5)
+# The ensemble and serving workloads reach down to 100 K.  With J = 1/2 the
+# mean-field longitudinal susceptibility is so small there (exp(-2 Tc/T)-like)
+# that explicit RK4 at dt = 1e-15 s diverges below ~170 K (FP64 oracle and
+# GPU alike); J = 7/2 keeps the same Tc and stays stable down to 100 K.
+MATERIAL_WIDE_T = dict(MATERIAL, J=3.5)
+
+
+def equilibrium_m(T, Tc, J):
+    """m_e(T): root of B_J(3J/(J+1) Tc/T m) = m by bisection (host set-up)."""
+    if T <= 0:
+        return 1.0
+    if T >= Tc:
+        return 0.0
+    a, b = (2 * J + 1) / (2 * J), 1 / (2 * J)
+    pre = 3 * J / (J + 1) * Tc / T
+
+    def f(m):
+        x = pre * m
+        return a / math.tanh(a * x) - b / math.tanh(b * x) - m
+
+    lo, hi = 1e-6, 1.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(mid) > 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def llbr_params(T=300.0, seed=1234, thermal=True, material=MATERIAL, demag=True, Bext=None):
+    """Keyword arguments of api.Solver / api.Batch for the LLB-r composition
+    exchange6 + demag + longitudinal field + Brown thermal field + llbar
+    torque + local00nc damping."""
+    Ms0, J = material["Ms0"], material["J"]
+    kw = dict(msat0T0Mul=Ms0, exchange=1, lexMul=math.sqrt(2 * material["A"] / (MU0 * Ms0 * Ms0)),
+              demag=1 if demag else 0, demag_accuracy=6, longfield=1, JMul=J, nMul=Ms0 / (2.0 * MUB * J),
+              TcMul=material["Tc"], TMul=float(T), thermal=1 if thermal else 0, seed=int(seed), stream=0,
+              cutoff_dt=0.0, gammaLL=GAMMA0, torque=1, local00nc=1, alphaMul=(material["lam"],) * 3)
+    if Bext is not None:
+        kw.update(zeeman=1, Bext=tuple(float(b) for b in Bext))
+    return kw
+
+
+def random_domains(N, block, me, rng):
+    """Random unit direction per block^3 cells, scaled to |m| = me; 3 x ncell."""
+    nb = [max(1, n // block) for n in N]
+    dirs = rng.standard_normal((3,) + tuple(nb))
+    dirs /= np.linalg.norm(dirs, axis=0)
+    idx = [np.minimum(np.arange(n) // block, nb[a] - 1) for a, n in enumerate(N)]
+    m = dirs[:, idx[0][:, None, None], idx[1][None, :, None], idx[2][None, None, :]] * me
+    return np.ascontiguousarray(m.reshape(3, -1))
+
+
+def random_state(ncell, me, rng):
+    """Independent random direction per cell, |m| = me; 3 x ncell."""
+    m = rng.standard_normal((3, ncell))
+    m *= me / np.linalg.norm(m, axis=0)
+    return m
+
+
+def single():
+    """512x512x16 LLB-r solve at 300 K (2 nm cubes, padded 1024x1024x32)."""
+    w = dict(name="llb-r T=300K 512x512x16 cells 2nm, demag padded 1024x1024x32, RK23 tol 1e-5",
+             N=(16, 512, 512), cell=(2e-9, 2e-9, 2e-9), T=300.0, seed=1234, tol=1e-5, block=16)
+    me = equilibrium_m(w["T"], MATERIAL["Tc"], MATERIAL["J"])
+    m = random_domains(w["N"], w["block"], me, np.random.default_rng(w["seed"]))
+    return w, llbr_params(T=w["T"], seed=w["seed"]), m, me
+
+
+ENSEMBLE_SHAPES = [(32, 32, 32), (4, 64, 64), (2, 128, 128), (8, 48, 96)]  # user 32^3, 64x64x4, 128x128x2, 96x48x8
+
+
+def ensemble(nsolve=512, seed=0):
+    """Per-solve (shape, T, seed) of the 512-solve stochastic ensemble."""
+    rng = np.random.default_rng(seed)
+    shape_idx = rng.integers(0, len(ENSEMBLE_SHAPES), nsolve)
+    T = rng.uniform(100.0, 1000.0, nsolve)
+    return [dict(N=ENSEMBLE_SHAPES[int(shape_idx[q])], T=float(T[q]), seed=q) for q in range(nsolve)]
+
+
+ENSEMBLE_CELL = (3e-9, 3e-9, 3e-9)
+
+# serving shape classes by cell count (user X x Y x Z -> internal (Z, Y, X))
+SERVE_SHAPES = [(16, 16, 16), (8, 32, 32), (4, 64, 64), (8, 64, 64), (4, 128, 128), (8, 128, 128), (4, 256, 256)]
+SERVE_CELL = (3e-9, 3e-9, 3e-9)
+
+
+def requests(n=2000, rate=200.0, seed=0, Tc=MATERIAL["Tc"]):
+    """The serving request stream: arrival time (s), shape class, T, B_ext,
+    steps, seed."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    logc = rng.uniform(math.log2(4096), math.log2(262144), n)
+    cls = np.clip(np.rint(logc - 12).astype(int), 0, len(SERVE_SHAPES) - 1)
+    T = np.maximum(rng.uniform(0.0, 1.2 * Tc, n), 0.1 * Tc)
+    B = rng.uniform(-0.1, 0.1, (n, 3))
+    steps = rng.integers(200, 5001, n)
+    return [dict(id=i, t=float(t[i]), cls=int(cls[i]), N=SERVE_SHAPES[int(cls[i])], T=float(T[i]),
+                 Bext=tuple(B[i]), steps=int(steps[i]), seed=i) for i in range(n)]

@@ -67,6 +67,9 @@ HS_API int hs_demag_field_f64(hs_demag* d, const double* m, const double* msat0T
                               double* B, CUstream stream);
 HS_API int hs_demag_field_f32(hs_demag* d, const float* m, const float* msat0T0, double msat0T0Mul,
                               float* B, CUstream stream);
+/* Grow the plan's FP32 spectrum buffers to hold nsolve solves of the mesh
+ * (batched solves share one plan and one kernel). */
+HS_API int hs_demag_reserve(hs_demag* d, int nsolve);
 
 /* ---------------------------------------------------------------- thermal noise */
 /* Philox4x32-10 (Salmon et al., SC'11) keyed by seed, counter
@@ -129,6 +132,19 @@ typedef struct {
 
 typedef struct hs_solver hs_solver;
 HS_API hs_solver* hs_solver_create(const hs_llb_params* p, CUstream stream);
+/* A batch of nsolve independent solves of one mesh and material, stepped by
+ * the same launches: per-solve temperature multiplier TMul[q], thermal seed
+ * seed[q] and applied field Bext[3q..3q+2] (tesla, internal order); NULL
+ * arrays take the value of p for every solve.  State arrays of the batch
+ * (set_m / get_m / rhs) are nsolve x 3 x N, solve-major.  stream NULL: the
+ * solver creates its own stream. */
+HS_API hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const double* TMul, const uint64_t* seed,
+                                  const double* Bext, CUstream stream);
+HS_API int hs_solver_nsolve(hs_solver* s);
+/* Fixed-step RK4 of all solves replayed from a captured hipGraph (one step
+ * captured once, then launched nsteps times; the thermal-noise step counter
+ * advances on the device). */
+HS_API int hs_solver_run_graph(hs_solver* s, long nsteps, long step0, double dt);
 HS_API void hs_solver_destroy(hs_solver* s);
 /* state m: 3 planes of N doubles on the host (internal order) */
 HS_API int hs_solver_set_m(hs_solver* s, const double* m_host);
@@ -148,6 +164,68 @@ HS_API hs_demag* hs_solver_demag(hs_solver* s);
 /* stage-kernel variant picked by timing at creation: 1 = one plane per
  * block, 0 = marching planes (env HS_STAGE_KERNEL=flat|march overrides) */
 HS_API int hs_solver_stage_variant(hs_solver* s);
+
+/* ------------------------------------------------ slots (continuous batching)
+ * No counterpart in the reference (it steps one solve per process).  The
+ * nsolve solves of a batch become slots that are filled, stepped and emptied
+ * independently: each running slot takes adaptive RK23 steps (Bogacki-
+ * Shampine, the reference's controller of solve_rk12.go:164-198 with
+ * exponent 1/3, at most 10 retries) with its own dt, accept/reject decision
+ * and step count, decided on the device so that many iterations are
+ * enqueued (as a captured hipGraph) without host round trips.  Idle, done
+ * and failed slots are skipped by the stage kernels.  The whole-batch calls
+ * above (hs_solver_run, _run_graph, _set_m, _get_m) must not be used on a
+ * handle once a slot has been admitted. */
+#define HS_SLOT_IDLE 0
+#define HS_SLOT_RUNNING 1
+#define HS_SLOT_DONE 2
+#define HS_SLOT_FAILED 3 /* more than 10 retries of one step */
+
+typedef struct {
+    uint32_t step;     /* thermal-noise step counter of the step in progress */
+    uint32_t attempt;  /* retry number of that step */
+    double h;          /* dt of the attempt in progress (next dt when done) */
+    double t;          /* simulated time advanced since admission */
+    double err;        /* error estimate of the last attempt */
+    double maxErr, mindt, maxdt;
+    uint32_t end;      /* step counter at which the slot is done */
+    int32_t state;     /* HS_SLOT_* */
+    uint32_t rejected; /* rejected attempts since admission */
+    uint32_t accepted; /* accepted steps since admission */
+    uint32_t home;     /* internal: buffer set that holds a finished state */
+    uint32_t retry;    /* internal: last attempt was rejected */
+} hs_slot;
+
+/* Fill slot `slot` (idle, done or failed) with a new solve: m_host is 3 x N
+ * doubles (internal order); per-solve TMul, thermal seed and noise stream,
+ * Bext (tesla, 3 values, NULL = the batch's); nsteps accepted steps from
+ * step counter step0 starting at dt.  Synchronous on the solver's stream. */
+HS_API int hs_slot_admit(hs_solver* s, int slot, const double* m_host, double TMul, uint64_t seed, uint32_t stream,
+                         const double* Bext, long step0, long nsteps, double dt, double maxErr, double mindt,
+                         double maxdt);
+/* Enqueue `iters` RK23 iterations (one attempt of every running slot each)
+ * and a copy of the slot records to host memory; returns without waiting. */
+HS_API int hs_slot_advance(hs_solver* s, int iters);
+/* 1 when the work enqueued by hs_slot_advance has finished, 0 if not. */
+HS_API int hs_slot_ready(hs_solver* s);
+/* Wait for hs_slot_advance's work and copy the nsolve slot records out. */
+HS_API int hs_slot_status(hs_solver* s, hs_slot* out);
+/* State of one slot (3 x N doubles); waits for the solver's stream. */
+HS_API int hs_slot_get_m(hs_solver* s, int slot, double* m_host);
+/* Mark a slot idle (its result has been read). */
+HS_API int hs_slot_release(hs_solver* s, int slot);
+/* Device bytes held by a solver handle (state, stage buffers, demag plan). */
+HS_API long hs_solver_footprint(hs_solver* s);
+
+/* ------------------------------------------------ ensembles of uneven meshes
+ * No counterpart in the reference.  Several batches (one per mesh shape)
+ * stepped together: one fixed-step RK4 step of every solve of every batch is
+ * captured as one hipGraph whose branches run on the batches' own streams,
+ * so a step of the whole ensemble is one graph launch. */
+typedef struct hs_ensemble hs_ensemble;
+HS_API hs_ensemble* hs_ensemble_create(hs_solver* const* batches, int nbatch);
+HS_API int hs_ensemble_run_graph(hs_ensemble* e, long nsteps, long step0, double dt);
+HS_API void hs_ensemble_destroy(hs_ensemble* e);
 
 /* ---------------------------------------------------------------- timing */
 /* Per-kernel durations from HIP events recorded on the launching stream
