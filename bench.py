@@ -6,10 +6,19 @@ mesh (user x, y, z) of 2 nm cubes, demag padded to 1024x1024x32, Ms0 =
 thermal field with a fixed seed, initial m = seeded random directions per
 16^3-cell block with |m| = m_e(300 K), adaptive RK23 (tolerance 1e-5).
 value = cells x accepted steps / wall time of the timed steps, with the state
-resident in HBM.  Prints one JSON line (see the keys below); progress goes to
-stderr.
+resident in HBM.
+
+Two more legs report batched vs plain per-solve throughput:
+- "ensemble": 512 stochastic solves on four mesh shapes stepped together
+  (fixed RK4, one graph launch per step) vs the same solves run one at a
+  time through the plain solver;
+- "serving": Poisson request stream served by the continuous-batching loop
+  (p50/p99 latency, cell-steps/s) vs the plain solver per request.
+
+Prints one JSON line (see the keys below); progress goes to stderr.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       [--ens-steps K] [--serve-requests R] [--no-ensemble] [--no-serving]
 """
 import argparse
 import json
@@ -23,62 +32,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-MU0 = 4e-7 * math.pi
-KB = 1.38064881313131313e-23
-MUB = 9.27400968202020202e-24
-HBM_PEAK_GBS = 8000.0
+from hotspin_amd import workloads as W  # noqa: E402
 
-WORKLOAD = dict(
-    name="llb-r T=300K 512x512x16 cells 2nm, demag padded 1024x1024x32, RK23 tol 1e-5",
-    N=(16, 512, 512), cell=(2e-9, 2e-9, 2e-9), Ms0=1.7e6, Tc=1044.0, A=2.1e-11, lam=0.01, J=0.5,
-    T=300.0, seed=1234, tol=1e-5, block=16,
-)
+HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def equilibrium_m(T, Tc, J):
-    """m_e(T): root of B_J(3J/(J+1) Tc/T m) = m by bisection (host set-up only)."""
-    if T <= 0:
-        return 1.0
-    if T >= Tc:
-        return 0.0
-    a, b = (2 * J + 1) / (2 * J), 1 / (2 * J)
-    pre = 3 * J / (J + 1) * Tc / T
-
-    def f(m):
-        x = pre * m
-        return a / math.tanh(a * x) - b / math.tanh(b * x) - m
-
-    lo, hi = 1e-6, 1.0
-    for _ in range(200):
-        mid = 0.5 * (lo + hi)
-        if f(mid) > 0:
-            lo = mid
-        else:
-            hi = mid
-    return 0.5 * (lo + hi)
-
-
-def problem(w):
-    N = w["N"]
-    Ms0, J = w["Ms0"], w["J"]
-    n_spin = Ms0 / (2.0 * MUB * J)
-    lex = math.sqrt(2 * w["A"] / (MU0 * Ms0 * Ms0))
-    kw = dict(msat0T0Mul=Ms0, exchange=1, lexMul=lex, demag=1, demag_accuracy=6, longfield=1, JMul=J,
-              nMul=n_spin, TcMul=w["Tc"], TMul=w["T"], thermal=1, seed=w["seed"], stream=0, cutoff_dt=0.0,
-              gammaLL=2.211e5, torque=1, local00nc=1, alphaMul=(w["lam"],) * 3)
-    rng = np.random.default_rng(w["seed"])
-    bs = w["block"]
-    nb = [max(1, n // bs) for n in N]
-    dirs = rng.standard_normal((3,) + tuple(nb))
-    dirs /= np.linalg.norm(dirs, axis=0)
-    me = equilibrium_m(w["T"], w["Tc"], J)
-    idx = [np.minimum(np.arange(n) // bs, nb[a] - 1) for a, n in enumerate(N)]
-    m = dirs[:, idx[0][:, None, None], idx[1][None, :, None], idx[2][None, None, :]] * me
-    return kw, np.ascontiguousarray(m.reshape(3, -1)), me
 
 
 def demag_pass_bytes(N, P):
@@ -103,6 +63,104 @@ def stage_bytes_rk23(ncell, normalize=False):
          12 + 12 + 24 + 12 + 12 + 24 + 12 + 12,                   # m, B, y0, k1, k2 in; y1, e, m' out
          12 + 12 + 12 + (48 if normalize else 0)]                 # m, B, e in (+y1 rw)
     return [x * ncell for x in b]
+
+
+def ensemble_leg(steps, warmup, plain_steps=10):
+    """512 solves on four shapes: one ensemble graph launch per RK4 step vs
+    the plain solver stepping each solve alone (eager, host loop)."""
+    import torch
+    from hotspin_amd.api import Batch, Ensemble, Solver
+    mat = W.MATERIAL_WIDE_T
+    cfg = W.ensemble(512, seed=0)
+    dt = 1e-15
+    batches, counts, cells = [], [], 0
+    t0 = time.time()
+    for shape in W.ENSEMBLE_SHAPES:
+        qs = [c for c in cfg if c["N"] == shape]
+        if not qs:
+            continue
+        kw = W.llbr_params(T=300.0, seed=0, material=mat)
+        b = Batch(shape, W.ENSEMBLE_CELL, len(qs), TMuls=[c["T"] for c in qs], seeds=[c["seed"] for c in qs], **kw)
+        b.set_m(np.stack([W.random_domains(shape, 8, W.equilibrium_m(c["T"], mat["Tc"], mat["J"]),
+                                           np.random.default_rng(c["seed"])) for c in qs]))
+        batches.append((shape, qs, b))
+        counts.append(len(qs))
+        cells += len(qs) * int(np.prod(shape))
+    e = Ensemble([b for _, _, b in batches])
+    log("ensemble: %d solves, %d cells, set-up %.1f s" % (sum(counts), cells, time.time() - t0))
+    e.run_graph(warmup, dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.run_graph(steps, dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    batched = cells * steps / el
+    bad = sum(int(not np.isfinite(b.get_m()).all(axis=(1, 2)).all()) for _, _, b in batches)
+    log("ensemble: %d RK4 steps in %.3f s: %.3e cell-steps/s" % (steps, el, batched))
+    # plain path: every solve alone, eager launches, host loop per step
+    plain_time = 0.0
+    for shape, qs, _ in batches:
+        c = qs[0]
+        kw = W.llbr_params(T=c["T"], seed=c["seed"], material=mat)
+        s = Solver(shape, W.ENSEMBLE_CELL, **kw)
+        s.set_m(W.random_domains(shape, 8, W.equilibrium_m(c["T"], mat["Tc"], mat["J"]),
+                                 np.random.default_rng(c["seed"])))
+        s.run("rk4", 2, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run("rk4", plain_steps, dt)
+        torch.cuda.synchronize()
+        per_step = (time.perf_counter() - t0) / plain_steps
+        plain_time += len(qs) * per_step * steps
+        log("ensemble: plain %s %.3f ms/step" % (shape, per_step * 1e3))
+        s.close()
+    plain = cells * steps / plain_time
+    e.close()
+    for _, _, b in batches:
+        b.close()
+    return {"workload": "512 LLB-r solves, seeds 0..511, meshes {32x32x32, 64x64x4, 128x128x2, 96x48x8} "
+                        "3 nm, T~U(100,1000) K, J=7/2, RK4 dt=1e-15", "solves": sum(counts),
+            "solves_per_shape": counts, "cells": cells, "steps": steps, "batched_cell_steps_per_s": batched,
+            "plain_cell_steps_per_s": plain, "ratio": batched / plain, "nonfinite_solves": bad,
+            "plain_sample": "%d eager RK4 steps of one solve per shape, scaled by the shape counts" % plain_steps}
+
+
+def serving_leg(nreq, plain_steps=20):
+    """Continuous batching of a Poisson request stream vs the plain solver."""
+    import torch
+    from hotspin_amd.api import Solver
+    from hotspin_amd.serve import Server
+    reqs = W.requests(n=nreq, rate=200.0, seed=0)
+    srv = Server(keep_results=False)
+    srv.run(reqs, log=log)
+    summ = srv.summary()
+    log("serving: %d requests in %.2f s: %.3e cell-steps/s, p50 %.2f s p99 %.2f s" % (
+        summ["requests"], summ["wall_s"], summ["cell_steps_per_s"], summ["p50_latency_s"], summ["p99_latency_s"]))
+    # plain: each request alone through hs_solver_run RK23 (host-side control); one sample request per class
+    rate = {}
+    for cls, shape in enumerate(W.SERVE_SHAPES):
+        rs = [r for r in reqs if r["cls"] == cls]
+        if not rs:
+            continue
+        r = rs[0]
+        kw = W.llbr_params(T=r["T"], seed=r["seed"], material=W.MATERIAL_WIDE_T, Bext=r["Bext"])
+        s = Solver(shape, W.SERVE_CELL, **kw)
+        s.set_m(srv._initial_state(r))
+        dt, _ = s.run("rk23", 3, 1e-15, maxErr=srv.tol, mindt=srv.mindt, maxdt=srv.maxdt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, st = s.run("rk23", plain_steps, dt, maxErr=srv.tol, mindt=srv.mindt, maxdt=srv.maxdt)
+        torch.cuda.synchronize()
+        rate[cls] = (time.perf_counter() - t0) / max(st.steps, 1)
+        s.close()
+    srv.close()
+    plain_time = sum(r["steps"] * rate[r["cls"]] for r in reqs)
+    plain = summ["cell_steps"] / plain_time
+    summ.update({"workload": "%d of 2000 Poisson requests at 200 req/s, cells log-uniform 16^3..256x256x4, "
+                             "T~U(0,1.2Tc) (>=0.1Tc), |B_ext|<=100 mT, 200-5000 RK23 steps tol 1e-5, J=7/2" % nreq,
+                 "plain_cell_steps_per_s": plain, "ratio": summ["cell_steps_per_s"] / plain,
+                 "plain_sample": "%d RK23 steps of one request per shape class through hs_solver_run" % plain_steps})
+    return summ
 
 
 def cpu_baseline(w, kw, m, threads, nsteps=1):
@@ -148,6 +206,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=150)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (copyUnPad3DAsync with equal extents), "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -166,8 +228,7 @@ def main():
     from hotspin_amd import _lib
     from hotspin_amd.api import Solver
 
-    w = WORKLOAD
-    kw, m, me = problem(w)
+    w, kw, m, me = W.single()
     ncell = int(np.prod(w["N"]))
     log("bench: building solver (kernel set-up on the GPU)...")
     t0 = time.time()
@@ -240,6 +301,11 @@ def main():
                      "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom_bytes},
         "kernels": per,
     }
+    s.close()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args.ens_steps, 5)
+    if not args.no_serving:
+        out["serving"] = serving_leg(args.serve_requests)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log("bench: cpu baseline (%d threads)..." % threads)

@@ -200,7 +200,7 @@ void launch_r2c(const void* x, const void* msk, long mrows, cpx<T>* out, long nr
     constexpr int W = row_w<M>();
     unsigned g = (unsigned)((nrows + W - 1) / W);
     hipLaunchKernelGGL((k_r2c_rows<T, TI, M>), dim3(g), dim3(row_nt<M>()), 0, st, (const TI*)x, (const TI*)msk,
-                       mrows, out, nrows, n2, H2p, twM, twP);
+                       mrows, out, nrows, n2, H2p, twM, twP, Live{nullptr, 1});
 }
 template <typename T, typename TO, int M>
 void launch_c2r(const cpx<T>* in, void* x, long nrows, int n2, int H2p, const cpx<T>* twM, const cpx<T>* twP,
@@ -208,20 +208,22 @@ void launch_c2r(const cpx<T>* in, void* x, long nrows, int n2, int H2p, const cp
     constexpr int W = row_w<M>();
     unsigned g = (unsigned)((nrows + W - 1) / W);
     hipLaunchKernelGGL((k_c2r_rows<T, TO, M>), dim3(g), dim3(row_nt<M>()), 0, st, in, (TO*)x, nrows, n2, H2p, twM,
-                       twP);
+                       twP, Live{nullptr, 1});
 }
 template <typename T, int L, int S>
 void launch_cols(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int npencil, int nin, int nkeep, int ncol,
                  const cpx<T>* tw, hipStream_t st) {
     constexpr int W = col_w<T, L>();
     dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
-    hipLaunchKernelGGL((k_cols<T, L, S>), g, dim3(col_nt<T, L>()), 0, st, in, out, gi, go, nin, nkeep, ncol, tw);
+    hipLaunchKernelGGL((k_cols<T, L, S>), g, dim3(col_nt<T, L>()), 0, st, in, out, gi, go, nin, nkeep, ncol, tw,
+                       Live{nullptr, 1});
 }
 template <typename T, typename TK, int L>
 void launch_zconv(cpx<T>* X, const TK* kern, int N0, int P1, int H2, int H2p, int K1, T srcMul, const cpx<T>* tw,
                   hipStream_t st) {
     dim3 g((unsigned)((H2 + ZW - 1) / ZW), (unsigned)P1);
-    hipLaunchKernelGGL((k_zconv<T, TK, L>), g, dim3(z_nt<L>()), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul, tw);
+    hipLaunchKernelGGL((k_zconv<T, TK, L>), g, dim3(z_nt<L>()), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul, tw,
+                       Live{nullptr, 1});
 }
 
 template <typename T, typename TI>
@@ -327,8 +329,10 @@ bool sizes_ok(const hs_demag* d) {
            !hsdm::variants(d->P[0]).z.empty();
 }
 
+// live: per-solve flags of a batch (null = all solves); blocks of solves
+// with flag 0 exit without touching memory.
 int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, float* B, hipStream_t st,
-                int nsolve = 1) {
+                int nsolve = 1, const int* live = nullptr) {
     const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1];
     const int H2 = d->H2, H2p = d->H2p;
     cpx<float>*X1 = d->X1f, *X2 = d->X2f;
@@ -340,19 +344,20 @@ int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, fl
     Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
     Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
     hs::prof_begin(HS_PROF_F1, st);
-    d->row32.r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, d->twMf, d->twP2f, st);
+    const Live lrows{live, 3L * N0 * N1}, lpen{live, 3L * N0}, lsol{live, 1};
+    d->row32.r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, d->twMf, d->twP2f, st, lrows);
     hs::prof_end(HS_PROF_F1, st);
     hs::prof_begin(HS_PROF_F2, st);
-    d->col32.fwd(X1, X2, g1, g2, 3 * N0 * nsolve, N1, P1, H2, d->tw1f, st);
+    d->col32.fwd(X1, X2, g1, g2, 3 * N0 * nsolve, N1, P1, H2, d->tw1f, st, lpen);
     hs::prof_end(HS_PROF_F2, st);
     hs::prof_begin(HS_PROF_F3K, st);
-    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], (float)srcMul, d->tw0f, st, nsolve);
+    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], (float)srcMul, d->tw0f, st, nsolve, lsol);
     hs::prof_end(HS_PROF_F3K, st);
     hs::prof_begin(HS_PROF_I2, st);
-    d->col32.inv(X2, X1, g2, g1, 3 * N0 * nsolve, P1, N1, H2, d->tw1f, st);
+    d->col32.inv(X2, X1, g2, g1, 3 * N0 * nsolve, P1, N1, H2, d->tw1f, st, lpen);
     hs::prof_end(HS_PROF_I2, st);
     hs::prof_begin(HS_PROF_I1, st);
-    d->row32.c2r(X1, B, nrows, N2, H2p, d->twMf, d->twP2f, st);
+    d->row32.c2r(X1, B, nrows, N2, H2p, d->twMf, d->twP2f, st, lrows);
     hs::prof_end(HS_PROF_I1, st);
     HS_LAUNCH_CHECK("hs_demag_field");
     return hs_last_error();
@@ -397,19 +402,21 @@ void tune32(hs_demag* d, hipStream_t st) {
     Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
     float best = 1e30f;
     for (auto& v : vr.rows) {
-        float t = timeit([&] { v.r2c(m, nullptr, (long)N0 * N1, d->X1f, nrows, N2, H2p, d->twMf, d->twP2f, st); }) +
-                  timeit([&] { v.c2r(d->X1f, B, nrows, N2, H2p, d->twMf, d->twP2f, st); });
+        const Live all{nullptr, 1};
+        float t = timeit([&] { v.r2c(m, nullptr, (long)N0 * N1, d->X1f, nrows, N2, H2p, d->twMf, d->twP2f, st, all); }) +
+                  timeit([&] { v.c2r(d->X1f, B, nrows, N2, H2p, d->twMf, d->twP2f, st, all); });
         if (t < best) { best = t; d->row32 = v; }
     }
     best = 1e30f;
     for (auto& v : vc.cols) {
-        float t = timeit([&] { v.fwd(d->X1f, d->X2f, g1, g2, 3 * N0, N1, P1, H2, d->tw1f, st); }) +
-                  timeit([&] { v.inv(d->X2f, d->X1f, g2, g1, 3 * N0, P1, N1, H2, d->tw1f, st); });
+        const Live all{nullptr, 1};
+        float t = timeit([&] { v.fwd(d->X1f, d->X2f, g1, g2, 3 * N0, N1, P1, H2, d->tw1f, st, all); }) +
+                  timeit([&] { v.inv(d->X2f, d->X1f, g2, g1, 3 * N0, P1, N1, H2, d->tw1f, st, all); });
         if (t < best) { best = t; d->col32 = v; }
     }
     best = 1e30f;
     for (auto& v : vz.z) {
-        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], 1.0f, d->tw0f, st, 1); });
+        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], 1.0f, d->tw0f, st, 1, Live{nullptr, 1}); });
         if (t < best) { best = t; d->z32 = v; }
     }
     hipEventDestroy(e0);
@@ -648,11 +655,11 @@ extern "C" int hs_demag_field_f64(hs_demag* d, const double* m, const double* ms
                              d->twP2d, d->tw1d, d->tw0d);
 }
 
-// Internal entry for the solver: FP32 field of the first `ncomp`-component
-// planes with an FP32 mask.
+// Internal entry for the solver: FP32 field of a batch of nsolve solves
+// (layout [solve][3][cells]) with an FP32 mask; live = per-solve flags or null.
 int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
-                                hipStream_t st, int nsolve) {
-    return run_field32(d, m, msk, HS_MU0 * msat0T0Mul, B, st, nsolve);
+                                hipStream_t st, int nsolve, const int* live) {
+    return run_field32(d, m, msk, HS_MU0 * msat0T0Mul, B, st, nsolve, live);
 }
 
 // Grow the FP32 spectrum buffers to hold `nsolve` solves of this mesh.

@@ -31,6 +31,20 @@ constexpr int z_nt() {
     return clampi(L * 3 * ZW / 16, 64, 1024);
 }
 
+// Per-solve liveness of a batched pass: work units [q*per, (q+1)*per) (rows,
+// pencils or solves) belong to solve q; flag null = every solve live.  A
+// block whose units all belong to one solve with flag 0 exits at once (idle
+// slots of a continuous batch cost no bandwidth).
+struct Live {
+    const int* flag;
+    long per;
+};
+__device__ __forceinline__ bool all_dead(const Live& lv, long u0, long u1) {
+    if (!lv.flag) return false;
+    const long q0 = u0 / lv.per, q1 = u1 / lv.per;
+    return q0 == q1 && !lv.flag[q0];
+}
+
 struct Geo {  // pencil a starts at (a / A2) * S1 + (a % A2) * S2; element t at + t * ts
     long S1, S2, ts;
     int A2;
@@ -46,10 +60,11 @@ template <typename T, typename TI, int M, int W = row_w<M>(), int NT = row_nt<M>
 __global__ void __launch_bounds__(NT) k_r2c_rows(const TI* __restrict__ x, const TI* __restrict__ msk,
                                                          long mrows, cpx<T>* __restrict__ out, long nrows, int n2,
                                                          int H2p, const cpx<T>* __restrict__ twM,
-                                                         const cpx<T>* __restrict__ twP) {
+                                                         const cpx<T>* __restrict__ twP, Live lv) {
     __shared__ cpx<T> lds[M * W];
     const int tid = threadIdx.x;
     long row0 = (long)blockIdx.x * W;
+    if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
     for (int e = tid; e < M * W; e += NT) {
         int p = e / M, n = e % M;
         long r = row0 + p;
@@ -85,10 +100,11 @@ __global__ void __launch_bounds__(NT) k_r2c_rows(const TI* __restrict__ x, const
 template <typename T, typename TO, int M, int W = row_w<M>(), int NT = row_nt<M>()>
 __global__ void __launch_bounds__(NT) k_c2r_rows(const cpx<T>* __restrict__ in, TO* __restrict__ x,
                                                          long nrows, int n2, int H2p, const cpx<T>* __restrict__ twM,
-                                                         const cpx<T>* __restrict__ twP) {
+                                                         const cpx<T>* __restrict__ twP, Live lv) {
     __shared__ cpx<T> lds[(M + 1) * W];
     const int tid = threadIdx.x;
     long row0 = (long)blockIdx.x * W;
+    if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
     for (int e = tid; e < (M + 1) * W; e += NT) {
         int p = e / (M + 1), k = e % (M + 1);
         long r = row0 + p;
@@ -138,10 +154,11 @@ __global__ void __launch_bounds__(NT) k_c2r_rows(const cpx<T>* __restrict__ in, 
 // (zero beyond), output rows t < nkeep.  in and out may be the same buffer.
 template <typename T, int L, int S, int W = col_w<T, L>(), int NT = col_nt<T, L>()>
 __global__ void __launch_bounds__(NT) k_cols(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
-                                             int ncol, const cpx<T>* __restrict__ tw) {
+                                             int ncol, const cpx<T>* __restrict__ tw, Live lv) {
     __shared__ cpx<T> lds[L * W];
     const int tid = threadIdx.x;
     const int a = blockIdx.y;
+    if (all_dead(lv, a, a)) return;
     const int col0 = blockIdx.x * W;
     const long bi = geo_base(gi, a) + col0;
     const long bo = geo_base(go, a) + col0;
@@ -164,8 +181,9 @@ __global__ void __launch_bounds__(NT) k_cols(const cpx<T>* in, cpx<T>* out, Geo 
 // Kernel octants: 6 planes (XX,YY,ZZ,YZ,XZ,XY) of (K0, K1, H2).
 template <typename T, typename TK, int L, int W = ZW, int NT = z_nt<L>()>
 __global__ void __launch_bounds__(NT) k_zconv(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
-                                              int H2p, int K1, T srcMul, const cpx<T>* __restrict__ tw) {
+                                              int H2p, int K1, T srcMul, const cpx<T>* __restrict__ tw, Live lv) {
     constexpr int WP = 3 * W;
+    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
     X += (long)blockIdx.z * 3 * N0 * P1 * H2p;  // solve of a batch
     __shared__ cpx<T> lds[L * WP];
     const int tid = threadIdx.x;

@@ -10,13 +10,13 @@
 
 namespace hsdm {
 using r2c32_t = void (*)(const float*, const float*, long, cpx<float>*, long, int, int, const cpx<float>*,
-                         const cpx<float>*, hipStream_t);
+                         const cpx<float>*, hipStream_t, Live);
 using c2r32_t = void (*)(const cpx<float>*, float*, long, int, int, const cpx<float>*, const cpx<float>*,
-                         hipStream_t);
+                         hipStream_t, Live);
 using cols32_t = void (*)(const cpx<float>*, cpx<float>*, Geo, Geo, int, int, int, int, const cpx<float>*,
-                          hipStream_t);
+                          hipStream_t, Live);
 using z32_t = void (*)(cpx<float>*, const float*, int, int, int, int, int, float, const cpx<float>*, hipStream_t,
-                       int);
+                       int, Live);
 struct RowV {
     int W, NT;
     r2c32_t r2c;
@@ -43,28 +43,28 @@ constexpr int clamp_nt(long v) { return round64((int)(v < 64 ? 64 : (v > 1024 ? 
 
 template <int M, int W, int NT>
 void launch_r2c32(const float* x, const float* msk, long mrows, cpx<float>* out, long nrows, int n2, int H2p,
-                  const cpx<float>* twM, const cpx<float>* twP, hipStream_t st) {
+                  const cpx<float>* twM, const cpx<float>* twP, hipStream_t st, Live lv) {
     hipLaunchKernelGGL((k_r2c_rows<float, float, M, W, NT>), dim3((unsigned)((nrows + W - 1) / W)), dim3(NT), 0,
-                       st, x, msk, mrows, out, nrows, n2, H2p, twM, twP);
+                       st, x, msk, mrows, out, nrows, n2, H2p, twM, twP, lv);
 }
 template <int M, int W, int NT>
 void launch_c2r32(const cpx<float>* in, float* x, long nrows, int n2, int H2p, const cpx<float>* twM,
-                  const cpx<float>* twP, hipStream_t st) {
+                  const cpx<float>* twP, hipStream_t st, Live lv) {
     hipLaunchKernelGGL((k_c2r_rows<float, float, M, W, NT>), dim3((unsigned)((nrows + W - 1) / W)), dim3(NT), 0,
-                       st, in, x, nrows, n2, H2p, twM, twP);
+                       st, in, x, nrows, n2, H2p, twM, twP, lv);
 }
 template <int L, int S, int W, int NT>
 void launch_cols32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int npencil, int nin, int nkeep,
-                   int ncol, const cpx<float>* tw, hipStream_t st) {
+                   int ncol, const cpx<float>* tw, hipStream_t st, Live lv) {
     dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
-    hipLaunchKernelGGL((k_cols<float, L, S, W, NT>), g, dim3(NT), 0, st, in, out, gi, go, nin, nkeep, ncol, tw);
+    hipLaunchKernelGGL((k_cols<float, L, S, W, NT>), g, dim3(NT), 0, st, in, out, gi, go, nin, nkeep, ncol, tw, lv);
 }
 template <int L, int W, int NT>
 void launch_z32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, float srcMul,
-                const cpx<float>* tw, hipStream_t st, int nsolve) {
+                const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
     dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)P1, (unsigned)nsolve);
     hipLaunchKernelGGL((k_zconv<float, float, L, W, NT>), g, dim3(NT), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul,
-                       tw);
+                       tw, lv);
 }
 
 template <int M, int W, int E>

@@ -22,7 +22,7 @@
 #include "philox.hpp"
 
 int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
-                                hipStream_t st, int nsolve);
+                                hipStream_t st, int nsolve, const int* live);
 extern "C" int hs_demag_reserve(hs_demag* d, int nsolve);
 
 namespace {
@@ -220,16 +220,56 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
     for (int c = 0; c < 3; c++) H[c] = P.ms0T0Mul * (e0[c] + e2[c] + e1[c]);
 }
 
+// Operands of one cell's stage update, loaded at the top of the kernel so a
+// wave has all its memory requests in flight at once.
+struct Pre {
+    float ms, B[3], J, n, Tc, T, al[3];
+    double y[3];    // y0 (or acc for RK12_2 / BS_4 with the normaliser)
+    float ka[3];    // k1 / e
+    float kb[3];    // k2
+    float kc[3];    // k3
+};
+
+template <int ROLE>
+__device__ __forceinline__ void preload(const Phys& P, const StageIO& io, long I, long N, Pre& q) {
+    q.ms = P.msk ? P.msk[I] : 1.f;
+    if (P.demag)
+        for (int c = 0; c < 3; c++) q.B[c] = io.B[c * N + I];
+    if (P.longfield) {
+        q.J = P.J ? P.J[I] : 1.f;
+        q.n = P.n ? P.n[I] : 1.f;
+        q.Tc = P.Tc ? P.Tc[I] : 1.f;
+    }
+    q.T = P.T ? P.T[I] : 1.f;
+    for (int c = 0; c < 3; c++) q.al[c] = P.alpha[c] ? P.alpha[c][I] : 1.f;
+    if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER || ROLE == R_RK4_2 ||
+                  ROLE == R_RK4_3 || ROLE == R_BS_2 || ROLE == R_RK4_4 || ROLE == R_BS_3)
+        for (int c = 0; c < 3; c++) q.y[c] = io.y0[c * N + I];
+    if constexpr (ROLE == R_RK12_2 || ROLE == R_BS_4) {
+        if (ROLE == R_RK12_2 || P.normalize)
+            for (int c = 0; c < 3; c++) q.y[c] = io.acc[c * N + I];
+    }
+    if constexpr (ROLE == R_RK4_4 || ROLE == R_BS_3)
+        for (int c = 0; c < 3; c++) {
+            q.ka[c] = io.k1[c * N + I];
+            q.kb[c] = io.k2[c * N + I];
+        }
+    if constexpr (ROLE == R_RK4_4)
+        for (int c = 0; c < 3; c++) q.kc[c] = io.k3[c * N + I];
+    if constexpr (ROLE == R_BS_4 || ROLE == R_RK12_2)
+        for (int c = 0; c < 3; c++) q.ka[c] = io.e[c * N + I];
+}
+
 // Everything after the exchange term for one cell: B/mu0, B_ext/mu0, H_lf,
 // H_therm, k = gamma (m x H + alpha H) and the RK stage update (FP64 state).
 template <int ROLE>
 __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, const StageIO& io, long I, long N,
-                                            const float m0[3], float H[3], double& errloc) {
-    const float ms = P.msk ? P.msk[I] : 1.f;
+                                            const float m0[3], float H[3], const Pre& q, double& errloc) {
+    const float ms = q.ms;
     if (P.demag) {
-        H[0] += P.invMu0 * io.B[I];
-        H[1] += P.invMu0 * io.B[N + I];
-        H[2] += P.invMu0 * io.B[2 * N + I];
+        H[0] += P.invMu0 * q.B[0];
+        H[1] += P.invMu0 * q.B[1];
+        H[2] += P.invMu0 * q.B[2];
     }
     if (P.zeeman) {
         H[0] += sp.bext[0];
@@ -239,10 +279,10 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
     if (P.longfield) {
         // FP64: dB_S/dx is a difference of O(1) terms (common_func.h:95-98)
         double Ms0T0 = (double)P.ms0T0Mul * ms;
-        double S = (double)P.JMul * (P.J ? P.J[I] : 1.f);
-        double n = (double)P.nMul * (P.n ? P.n[I] : 1.f);
-        double Tc = (double)P.TcMul * (P.Tc ? P.Tc[I] : 1.f);
-        double Ts = (double)sp.TMul * (P.T ? P.T[I] : 1.f);
+        double S = (double)P.JMul * q.J;
+        double n = (double)P.nMul * q.n;
+        double Tc = (double)P.TcMul * q.Tc;
+        double Ts = (double)sp.TMul * q.T;
         double mx = m0[0], my = m0[1], mz = m0[2];
         double am = sqrt(mx * mx + my * my + mz * mz);
         if (!(Ms0T0 == 0.0 || n == 0.0 || am <= HS_ZERO)) {
@@ -259,12 +299,11 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
     if (P.thermal && ms != 0.f) {
         float x0, x1, x2;
         thermal_normals_f((uint32_t)I, io.step, io.attempt, sp.stream, sp.seed, x0, x1, x2);
-        float Ti = P.T ? P.T[I] : 1.f;
         float thK = (float)(sp.thC / fmax((double)P.cutoff_dt, io.h));
-        float pre = sqrtf((Ti * thK) / ms);
-        float s0 = sqrtf(P.aMul[0] * (P.alpha[0] ? P.alpha[0][I] : 1.f));
-        float s1 = sqrtf(P.aMul[1] * (P.alpha[1] ? P.alpha[1][I] : 1.f));
-        float s2 = sqrtf(P.aMul[2] * (P.alpha[2] ? P.alpha[2][I] : 1.f));
+        float pre = sqrtf((q.T * thK) / ms);
+        float s0 = sqrtf(P.aMul[0] * q.al[0]);
+        float s1 = sqrtf(P.aMul[1] * q.al[1]);
+        float s2 = sqrtf(P.aMul[2] * q.al[2]);
         H[0] += pre * (s0 * x0);
         H[1] += pre * (s1 * x1);
         H[2] += pre * (s2 * x2);
@@ -278,7 +317,7 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
         }
         if (P.local) {
             for (int c = 0; c < 3; c++) {
-                float al = P.aMul[c] * (P.alpha[c] ? P.alpha[c][I] : 1.f);
+                float al = P.aMul[c] * q.al[c];
                 kk[c] += P.gamma * (al * H[c]);
             }
         }
@@ -307,7 +346,7 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
             for (int c = 0; c < 3; c++) io.mout[c * N + I] = (float)w[c];
         };
         if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER) {
-            for (int c = 0; c < 3; c++) y[c] = io.y0[c * N + I];
+            for (int c = 0; c < 3; c++) y[c] = q.y[c];
             if (P.normalize) {
                 nrm(y);
                 for (int c = 0; c < 3; c++) io.y0[c * N + I] = y[c];
@@ -324,32 +363,32 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
             float* kst = (ROLE == R_RK4_3) ? io.k3 : io.k2;
             for (int c = 0; c < 3; c++) {
                 kst[c * N + I] = kk[c];
-                acc[c] = io.y0[c * N + I] + cm * (double)kk[c];
+                acc[c] = q.y[c] + cm * (double)kk[c];
             }
             store_m(acc);
         } else if constexpr (ROLE == R_RK4_4) {
             const double h6 = h / 6.0;
             for (int c = 0; c < 3; c++) {
-                double k1 = io.k1[c * N + I], k2 = io.k2[c * N + I], k3 = io.k3[c * N + I];
-                acc[c] = io.y0[c * N + I] + h6 * (k1 + 2.0 * k2 + 2.0 * k3 + (double)kk[c]);
+                double k1 = q.ka[c], k2 = q.kb[c], k3 = q.kc[c];
+                acc[c] = q.y[c] + h6 * (k1 + 2.0 * k2 + 2.0 * k3 + (double)kk[c]);
                 io.y0[c * N + I] = acc[c];
             }
             store_m(acc);
         } else if constexpr (ROLE == R_BS_3) {
             for (int c = 0; c < 3; c++) {
-                double k1 = io.k1[c * N + I], k2 = io.k2[c * N + I], k3 = kk[c];
-                acc[c] = io.y0[c * N + I] + h * ((2.0 / 9.0) * k1 + (1.0 / 3.0) * k2 + (4.0 / 9.0) * k3);
+                double k1 = q.ka[c], k2 = q.kb[c], k3 = kk[c];
+                acc[c] = q.y[c] + h * ((2.0 / 9.0) * k1 + (1.0 / 3.0) * k2 + (4.0 / 9.0) * k3);
                 io.acc[c * N + I] = acc[c];
                 io.e[c * N + I] = (float)(h * ((-5.0 / 72.0) * k1 + (1.0 / 12.0) * k2 + (1.0 / 9.0) * k3));
             }
             store_m(acc);
         } else if constexpr (ROLE == R_BS_4) {
             for (int c = 0; c < 3; c++) {
-                double ev = (double)io.e[c * N + I] - (h / 8.0) * (double)kk[c];
+                double ev = (double)q.ka[c] - (h / 8.0) * (double)kk[c];
                 errloc = fmax(errloc, fabs(ev));
             }
             if (P.normalize) {
-                for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
+                for (int c = 0; c < 3; c++) acc[c] = q.y[c];
                 nrm(acc);
                 for (int c = 0; c < 3; c++) io.acc[c * N + I] = acc[c];
             }
@@ -361,10 +400,10 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
             }
             store_m(acc);
         } else if constexpr (ROLE == R_RK12_2) {
-            for (int c = 0; c < 3; c++) acc[c] = io.acc[c * N + I];
+            for (int c = 0; c < 3; c++) acc[c] = q.y[c];
             nrm(acc);
             for (int c = 0; c < 3; c++) {
-                float k1 = io.e[c * N + I];
+                float k1 = q.ka[c];
                 acc[c] += (0.5 * h) * (double)kk[c] + (-0.5 * h) * (double)k1;
                 io.acc[c * N + I] = acc[c];
                 errloc = fmax(errloc, fabs((double)kk[c] - (double)k1));
@@ -438,6 +477,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
             for (int c = 0; c < 3; c++) nxt[c] = m[c * N + a2];
             if (P.lex) lnx = P.lex[a2];
         }
+        Pre q;
+        preload<ROLE>(P, io, I, N, q);
         load_tile(P, m, N, i, jc, kc, tx, ty, a[1], la[1], tile, ltile);
         __syncthreads();
         float m0[3] = {a[1][0], a[1][1], a[1][2]};
@@ -445,7 +486,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
         if (P.exchange && in)
             exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, a[0], a[2], la[1], la[0], la[2], tile, ltile, H);
         __syncthreads();
-        if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, errloc);
+        if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, q, errloc);
         for (int c = 0; c < 3; c++) {
             a[0][c] = a[1][c];
             a[1][c] = a[2][c];
@@ -481,6 +522,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
     float m0[3], lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
     float l0 = 1.f, llo = 1.f, lhi = 1.f;
     for (int c = 0; c < 3; c++) m0[c] = m[c * N + I];
+    Pre q;
+    preload<ROLE>(P, io, I, N, q);
     if (P.lex) l0 = P.lex[I];
     if (P.exchange) {
         long Ilo = wrapc(i - 1, N0, P.w0) * N12 + (long)jc * N2 + kc;
@@ -498,7 +541,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
     __syncthreads();
     float H[3] = {0.f, 0.f, 0.f};
     if (P.exchange && in) exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, lo, hi, l0, llo, lhi, tile, ltile, H);
-    if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, errloc);
+    if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, q, errloc);
     flush_err<ROLE>(errloc, io);
 }
 
@@ -567,11 +610,14 @@ __host__ __device__ inline double controller(double dt, double err, double maxEr
 // Per-slot accept/reject after an RK23 attempt (one thread per slot).  The
 // state of iteration `par` is in buffer set par; an accepted attempt leaves
 // the new state in set par^1, a rejected one is copied there by k_fixup.
-__global__ void k_control(Clock* clk, unsigned long long* errmax, int nsolve, uint32_t par) {
+__global__ void k_control(Clock* clk, unsigned long long* errmax, int* live, int nsolve, uint32_t par) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nsolve) return;
     Clock c = clk[q];
-    if (c.state != HS_SLOT_RUNNING) return;
+    if (c.state != HS_SLOT_RUNNING) {
+        live[q] = 0;
+        return;
+    }
     double err = __longlong_as_double((long long)errmax[q]);
     errmax[q] = 0ull;
     c.err = err;
@@ -580,6 +626,7 @@ __global__ void k_control(Clock* clk, unsigned long long* errmax, int nsolve, ui
         c.home = par;
         c.retry = 0;
         clk[q] = c;
+        live[q] = 0;
         return;
     }
     bool bad = err > c.maxErr;
@@ -608,6 +655,7 @@ __global__ void k_control(Clock* clk, unsigned long long* errmax, int nsolve, ui
         }
     }
     clk[q] = c;
+    live[q] = c.state == HS_SLOT_RUNNING ? 1 : 0;
 }
 
 // A rejected slot restarts its step from the state of set `cur`: copy it
@@ -642,6 +690,7 @@ struct hs_solver {
     long slot_iter = 0;                 // iterations enqueued (parity = slot_iter & 1)
     hipGraphExec_t slot_graph[2] = {nullptr, nullptr};
     hs_slot* slot_host = nullptr;       // pinned copy of the slot records
+    int* live = nullptr;                // device, 1 per running slot (demag passes skip the others)
     hipEvent_t slot_ev = nullptr;
     std::vector<SolveP> sp_host;
     int graph_method = -1;
@@ -737,9 +786,12 @@ void tune_stage(hs_solver* s) {
     hipEventDestroy(e1);
 }
 
+// demag field of the whole batch; in slot mode the blocks of idle, done and
+// failed slots skip their rows
 void demag_into(hs_solver* s, const float* m, float* B) {
     if (s->phys.demag)
-        hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st, s->nsolve);
+        hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st, s->nsolve,
+                                    s->slot_mode ? s->live : nullptr);
 }
 
 // max over the batch of the per-solve error maxima (lockstep stepping)
@@ -832,8 +884,8 @@ void slot_iteration(hs_solver* s, int par) {
     demag_into(s, nxt.m, nxt.B);
     io.m = nxt.m; io.B = nxt.B; io.mout = nullptr;
     launch_stage<R_BS_4>(s, io);
-    hipLaunchKernelGGL(k_control, hs::div_up(s->nsolve, 256), 256, 0, s->st, s->clk, s->errmax, s->nsolve,
-                       (uint32_t)par);
+    hipLaunchKernelGGL(k_control, hs::div_up(s->nsolve, 256), 256, 0, s->st, s->clk, s->errmax, s->live,
+                       s->nsolve, (uint32_t)par);
     hipLaunchKernelGGL(k_fixup, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->clk, cur.y, cur.m,
                        cur.B, nxt.y, nxt.m, nxt.B, s->N, s->phys.demag);
 }
@@ -992,7 +1044,7 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
     if (s->slot_host) hipHostFree(s->slot_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
-                  s->errmax, s->sp, s->clk};
+                  s->errmax, s->sp, s->clk, s->live};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
@@ -1238,6 +1290,8 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
         hipMemcpy(s->clk, hc.data(), sizeof(Clock) * s->nsolve, hipMemcpyHostToDevice);
         hipMemset(s->errmax, 0, sizeof(unsigned long long) * s->nsolve);
         if (hipHostMalloc((void**)&s->slot_host, sizeof(hs_slot) * s->nsolve) != hipSuccess ||
+            hipMalloc(&s->live, sizeof(int) * s->nsolve) != hipSuccess ||
+            hipMemset(s->live, 0, sizeof(int) * s->nsolve) != hipSuccess ||
             hipEventCreateWithFlags(&s->slot_ev, hipEventDisableTiming) != hipSuccess) {
             hs::set_error(HS_ENOMEM, "hs_slot_admit: pinned host memory");
             return HS_ENOMEM;
@@ -1271,7 +1325,8 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
     hipLaunchKernelGGL(k_set_state, dim3(hs::grid1d(s->N), 1), hs::kBlock, 0, s->st, cur.y + off, cur.m + off,
                        s->p.normalize, s->N);
     if (s->phys.demag)
-        hs_demag_field_f32_internal(s->demag, cur.m + off, s->phys.msk, s->p.msat0T0Mul, cur.B + off, s->st, 1);
+        hs_demag_field_f32_internal(s->demag, cur.m + off, s->phys.msk, s->p.msat0T0Mul, cur.B + off, s->st, 1,
+                                    nullptr);
     memset(&c, 0, sizeof(c));
     c.step = (uint32_t)step0;
     c.h = dt;
@@ -1282,6 +1337,8 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
     c.state = nsteps > 0 ? HS_SLOT_RUNNING : HS_SLOT_DONE;
     c.home = (uint32_t)(s->slot_iter & 1);
     hipMemcpyAsync(s->clk + slot, &c, sizeof(Clock), hipMemcpyHostToDevice, s->st);
+    const int lv = c.state == HS_SLOT_RUNNING ? 1 : 0;
+    hipMemcpyAsync(s->live + slot, &lv, sizeof(int), hipMemcpyHostToDevice, s->st);
     hipStreamSynchronize(s->st);
     s->slot_host[slot] = c;
     HS_LAUNCH_CHECK("hs_slot_admit");
@@ -1346,6 +1403,8 @@ extern "C" int hs_slot_release(hs_solver* s, int slot) {
     Clock c;
     memset(&c, 0, sizeof(c));
     hipMemcpy(s->clk + slot, &c, sizeof(Clock), hipMemcpyHostToDevice);
+    const int lv = 0;
+    hipMemcpy(s->live + slot, &lv, sizeof(int), hipMemcpyHostToDevice);
     s->slot_host[slot] = c;
     HS_LAUNCH_CHECK("hs_slot_release");
     return hs_last_error();

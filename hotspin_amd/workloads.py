@@ -7,6 +7,10 @@ the reference): msat0T0, lex = sqrt(2A/(mu0 Ms0^2)), J (spin), n = Ms0 /
 2.211e5.  Meshes are in internal (z, y, x) order: a user mesh X x Y x Z is
 N = (Z, Y, X).
 
+- `thin_film()`: the LLG limit of LLB-r on a 128x32x1 film of 3.90625 x
+  3.90625 x 3 nm cells (Ms = 8e5 A/m, A = 1.3e-11 J/m, alpha = 0.02), m
+  along +x tilted per cell by up to 10 degrees, an in-plane field of 20-30 mT
+  at 150-190 degrees, fixed RK4 at dt = 1e-13 s, lazy normalisation.
 - `single()`: one 512x512x16 solve at 300 K, 2 nm cubes, RK23 tol 1e-5.
 - `ensemble()` (material MATERIAL_WIDE_T): 512 solves, seeds 0..511, meshes drawn per seed from
   {32x32x32, 64x64x4, 128x128x2, 96x48x8}, 3 nm cells, T ~ U(100, 1000) K,
@@ -86,6 +90,26 @@ def random_state(ncell, me, rng):
     m = rng.standard_normal((3, ncell))
     m *= me / np.linalg.norm(m, axis=0)
     return m
+
+
+def thin_film(seed=0):
+    """128x32x1 LLG-limit film (seed -> numpy PCG64 stream: |B|, angle,
+    then per-cell tilt and azimuth).  Returns (N, cell, kw, m0)."""
+    rng = np.random.default_rng(seed)
+    Ms, A, alpha = 8.0e5, 1.3e-11, 0.02
+    bmag = rng.uniform(0.020, 0.030)
+    phi = math.radians(rng.uniform(150.0, 190.0))
+    N = (1, 32, 128)
+    cell = (3e-9, 125e-9 / 32, 500e-9 / 128)
+    ncell = N[0] * N[1] * N[2]
+    theta = np.radians(rng.uniform(0.0, 10.0, ncell))
+    psi = rng.uniform(0.0, 2 * math.pi, ncell)
+    # user (x, y, z) -> internal (z, y, x)
+    m = np.stack([np.sin(theta) * np.sin(psi), np.sin(theta) * np.cos(psi), np.cos(theta)])
+    kw = dict(msat0T0Mul=Ms, exchange=1, lexMul=math.sqrt(2 * A / (MU0 * Ms * Ms)), demag=1, demag_accuracy=6,
+              zeeman=1, Bext=(0.0, bmag * math.sin(phi), bmag * math.cos(phi)), gammaLL=GAMMA0 / (1 + alpha ** 2),
+              torque=1, local00nc=1, alphaMul=(alpha,) * 3, normalize=1)
+    return N, cell, kw, np.ascontiguousarray(m)
 
 
 def single():
