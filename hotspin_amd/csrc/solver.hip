@@ -43,6 +43,11 @@ struct SolveP {
 // the step, attempt and dt live on the device (graph replay, slots).
 using Clock = hs_slot;
 constexpr int kMaxTry = 10;  // solve_rk12.go:44 (maxTry), retries of one step
+// The error max of a solve is gathered in kErrLanes partial maxima, one per
+// 128-byte line (stride kErrStride), so the per-block atomics of a launch
+// spread over many L2 lines instead of serialising on one address.
+constexpr int kErrLanes = 32, kErrStride = 16;
+constexpr int kErrPerSolve = kErrLanes * kErrStride;
 
 struct Phys {
     int N0, N1, N2, w0, w1, w2;
@@ -82,7 +87,7 @@ struct StageIO {
     float* mout;      // next stage input
     float* kout;      // R_RHS: k
     float* hout;      // R_RHS: H_eff
-    unsigned long long* errmax;  // max |e| as double bits, one per solve
+    unsigned long long* errmax;  // max |e| as double bits: kErrLanes lanes per solve
     double h;
     uint32_t step, attempt;
     const Clock* clk;  // when set, step/attempt/h of each solve are read from here
@@ -103,7 +108,7 @@ __device__ __forceinline__ StageIO solve_io(const StageIO& io, int sol, long N) 
     if (r.mout) r.mout += b;
     if (r.kout) r.kout += b;
     if (r.hout) r.hout += b;
-    r.errmax += sol;
+    r.errmax += (long)sol * kErrPerSolve;
     if (io.clk) {
         r.step = io.clk[sol].step;
         r.attempt = io.clk[sol].attempt;
@@ -419,12 +424,22 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
     }
 }
 
+// Error max of the block: wave shuffles, then the block's waves through LDS,
+// then one atomic max (ordered double bits) into lane blockIdx % kErrLanes.
 template <int ROLE>
 __device__ __forceinline__ void flush_err(double errloc, const StageIO& io) {
     if constexpr (ROLE == R_BS_4 || ROLE == R_RK12_2) {
+        __shared__ double wmax[TX * TY / 64];
         for (int o = 32; o > 0; o >>= 1) errloc = fmax(errloc, __shfl_xor(errloc, o, 64));
-        if (((threadIdx.y * TX + threadIdx.x) & 63) == 0)
-            atomicMax(io.errmax, (unsigned long long)__double_as_longlong(errloc));
+        const int t = threadIdx.y * TX + threadIdx.x;
+        if ((t & 63) == 0) wmax[t >> 6] = errloc;
+        __syncthreads();
+        if (t == 0) {
+            double v = wmax[0];
+            for (int w = 1; w < TX * TY / 64; w++) v = fmax(v, wmax[w]);
+            const unsigned lane = (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % kErrLanes;
+            atomicMax(io.errmax + lane * kErrStride, (unsigned long long)__double_as_longlong(v));
+        }
     }
 }
 
@@ -618,8 +633,12 @@ __global__ void k_control(Clock* clk, unsigned long long* errmax, int* live, int
         live[q] = 0;
         return;
     }
-    double err = __longlong_as_double((long long)errmax[q]);
-    errmax[q] = 0ull;
+    unsigned long long* em = errmax + (long)q * kErrPerSolve;
+    double err = 0.0;
+    for (int l = 0; l < kErrLanes; l++) {
+        err = fmax(err, __longlong_as_double((long long)em[l * kErrStride]));
+        em[l * kErrStride] = 0ull;
+    }
     c.err = err;
     if (!isfinite(err)) {  // the state has diverged: stop this slot
         c.state = HS_SLOT_FAILED;
@@ -786,28 +805,30 @@ void tune_stage(hs_solver* s) {
     hipEventDestroy(e1);
 }
 
-// demag field of the whole batch; in slot mode the blocks of idle, done and
-// failed slots skip their rows
-void demag_into(hs_solver* s, const float* m, float* B) {
+// demag field of the whole batch; with `live` (device-controlled stepping)
+// the blocks of solves that are not running skip their rows
+void demag_into(hs_solver* s, const float* m, float* B, const int* live = nullptr) {
     if (s->phys.demag)
-        hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st, s->nsolve,
-                                    s->slot_mode ? s->live : nullptr);
+        hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st, s->nsolve, live);
 }
 
 // max over the batch of the per-solve error maxima (lockstep stepping)
 double read_err(hs_solver* s) {
-    std::vector<unsigned long long> b(s->nsolve);
-    hipMemcpyAsync(b.data(), s->errmax, sizeof(unsigned long long) * s->nsolve, hipMemcpyDeviceToHost, s->st);
+    const long n = (long)s->nsolve * kErrPerSolve;
+    std::vector<unsigned long long> b(n);
+    hipMemcpyAsync(b.data(), s->errmax, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s->st);
     hipStreamSynchronize(s->st);
     double m = 0.0;
-    for (int q = 0; q < s->nsolve; q++) {
+    for (long q = 0; q < n; q += kErrStride) {
         double v;
         memcpy(&v, &b[q], sizeof(v));
         m = std::max(m, v);
     }
     return m;
 }
-void zero_err(hs_solver* s) { hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * s->nsolve, s->st); }
+void zero_err(hs_solver* s) {
+    hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * s->nsolve * kErrPerSolve, s->st);
+}
 
 void ensure_b0(hs_solver* s) {
     if (!s->b0_valid) {
@@ -875,13 +896,13 @@ void slot_iteration(hs_solver* s, int par) {
     io.acc = nxt.y;
     io.m = cur.m; io.B = cur.B; io.mout = nxt.m;
     launch_stage<R_BS_1>(s, io);
-    demag_into(s, nxt.m, nxt.B);
+    demag_into(s, nxt.m, nxt.B, s->live);
     io.m = nxt.m; io.B = nxt.B; io.mout = s->mC;
     launch_stage<R_BS_2>(s, io);
-    demag_into(s, s->mC, nxt.B);
+    demag_into(s, s->mC, nxt.B, s->live);
     io.m = s->mC; io.B = nxt.B; io.mout = nxt.m;
     launch_stage<R_BS_3>(s, io);
-    demag_into(s, nxt.m, nxt.B);
+    demag_into(s, nxt.m, nxt.B, s->live);
     io.m = nxt.m; io.B = nxt.B; io.mout = nullptr;
     launch_stage<R_BS_4>(s, io);
     hipLaunchKernelGGL(k_control, hs::div_up(s->nsolve, 256), 256, 0, s->st, s->clk, s->errmax, s->live,
@@ -909,6 +930,138 @@ bool capture(hs_solver* s, hipGraphExec_t* out, const char* who, void (*body)(hs
         hs::set_error(HS_EINVAL, who);
     }
     return ok;
+}
+
+// Captured graphs bake in buffer addresses: drop them when the host swaps
+// the roles of the state buffers (Euler, RK12 and Heun steps).
+void drop_graphs(hs_solver* s) {
+    if (s->graph) hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    for (int v = 0; v < 2; v++) {
+        if (s->slot_graph[v]) hipGraphExecDestroy(s->slot_graph[v]);
+        s->slot_graph[v] = nullptr;
+    }
+}
+
+// Buffers of device-controlled stepping (slots, and RK23 of hs_solver_run):
+// pinned slot records, live flags, completion event.
+bool ensure_devctl(hs_solver* s) {
+    if (s->slot_host) return true;
+    if (hipHostMalloc((void**)&s->slot_host, sizeof(hs_slot) * s->nsolve) != hipSuccess ||
+        hipMalloc(&s->live, sizeof(int) * s->nsolve) != hipSuccess ||
+        hipEventCreateWithFlags(&s->slot_ev, hipEventDisableTiming) != hipSuccess) {
+        hs::set_error(HS_ENOMEM, "device-controlled stepping: out of memory");
+        return false;
+    }
+    memset(s->slot_host, 0, sizeof(hs_slot) * s->nsolve);
+    hipMemsetAsync(s->live, 0, sizeof(int) * s->nsolve, s->st);
+    return true;
+}
+
+bool ensure_slot_graphs(hs_solver* s) {
+    for (int v = 0; v < 2; v++)
+        if (!s->slot_graph[v] &&
+            !capture(s, &s->slot_graph[v], "device-controlled stepping: graph capture failed", slot_iteration, v))
+            return false;
+    return true;
+}
+
+// After device-controlled stepping: solves whose state ended in buffer set 1
+// copy it (state, stage input, its field) to set 0.
+__global__ void k_gather(const Clock* clk, const double* y1, const float* m1, const float* B1, double* y0,
+                         float* m0, float* B0, long N, int demag) {
+    const int q = blockIdx.y;
+    if (clk[q].home != 1u) return;
+    long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const long b = (long)q * 3 * N;
+    for (int c = 0; c < 3; c++) {
+        y0[b + c * N + I] = y1[b + c * N + I];
+        m0[b + c * N + I] = m1[b + c * N + I];
+        if (demag) B0[b + c * N + I] = B1[b + c * N + I];
+    }
+}
+
+// hs_solver_run's RK23: every solve of the batch takes nsteps accepted steps
+// with its own dt, accept/reject decided on the device (k_control), the
+// iterations launched as captured graphs; the host only checks completion.
+int run_rk23_device(hs_solver* s, long nsteps, long step0, double* dtio, double maxErr, double mindt,
+                    double maxdt, hs_run_stats* stats) {
+    hs_run_stats st{};
+    if (nsteps == 0) {
+        if (stats) *stats = st;
+        return hs_last_error();
+    }
+    if (!ensure_devctl(s)) return HS_ENOMEM;
+    ensure_b0(s);
+    std::vector<Clock> hc(s->nsolve);
+    for (auto& c : hc) {
+        memset(&c, 0, sizeof(c));
+        c.step = (uint32_t)step0;
+        c.h = *dtio;
+        c.maxErr = maxErr;
+        c.mindt = mindt;
+        c.maxdt = maxdt;
+        c.end = (uint32_t)(step0 + nsteps);
+        c.state = HS_SLOT_RUNNING;
+    }
+    std::vector<int> lv(s->nsolve, 1);
+    hipMemcpyAsync(s->clk, hc.data(), sizeof(Clock) * s->nsolve, hipMemcpyHostToDevice, s->st);
+    hipMemcpyAsync(s->live, lv.data(), sizeof(int) * s->nsolve, hipMemcpyHostToDevice, s->st);
+    zero_err(s);
+    hipStreamSynchronize(s->st);
+    if (!ensure_slot_graphs(s)) return HS_EINVAL;
+    s->slot_iter = 0;
+    long todo = nsteps, iters = 0;
+    const long cap = nsteps * (kMaxTry + 2) + 2;  // every solve has ended by then
+    for (;;) {
+        // with the event profiler on, launch the same kernels eagerly so each
+        // launch is bracketed by events (graph replay carries none)
+        const bool eager = hs::prof_on();
+        for (long n = 0; n < todo; n++) {
+            if (eager)
+                slot_iteration(s, (int)(s->slot_iter & 1));
+            else
+                hipGraphLaunch(s->slot_graph[s->slot_iter & 1], s->st);
+            s->slot_iter++;
+        }
+        iters += todo;
+        hipMemcpyAsync(s->slot_host, s->clk, sizeof(Clock) * s->nsolve, hipMemcpyDeviceToHost, s->st);
+        hipStreamSynchronize(s->st);
+        long left = 0;
+        for (int q = 0; q < s->nsolve; q++)
+            if (s->slot_host[q].state == HS_SLOT_RUNNING)
+                left = std::max(left, (long)(s->slot_host[q].end - s->slot_host[q].step));
+        if (left == 0 || iters >= cap) break;
+        todo = std::max(left, 2L);
+    }
+    hipLaunchKernelGGL(k_gather, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->clk, s->acc, s->mB,
+                       s->Bs, s->y0, s->mA, s->B0, s->N, s->phys.demag);
+    hipStreamSynchronize(s->st);
+    s->slot_iter = 0;
+    s->b0_valid = s->phys.demag != 0;
+    bool failed = false;
+    double dtmin = 0.0;
+    for (int q = 0; q < s->nsolve; q++) {
+        const Clock& c = s->slot_host[q];
+        failed |= c.state != HS_SLOT_DONE;
+        st.steps = std::max(st.steps, (long)c.accepted);
+        st.rejected += c.rejected;
+        st.rhs_evals += 3L * (c.accepted + c.rejected);
+        if (q == 0 || c.h < dtmin) dtmin = c.h;
+    }
+    const Clock& c0 = s->slot_host[0];
+    st.t = c0.t;
+    st.last_err = c0.err;
+    st.dt = dtmin;
+    *dtio = dtmin;
+    if (stats) *stats = st;
+    HS_LAUNCH_CHECK("hs_solver_run");
+    if (failed) {
+        hs::set_error(HS_ECONV, "hs_solver_run: RK23 did not converge (or diverged)");
+        return HS_ECONV;
+    }
+    return hs_last_error();
 }
 }  // namespace
 
@@ -940,7 +1093,7 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     hipMalloc(&s->k1, sizeof(float) * 3 * NN);
     hipMalloc(&s->k2, sizeof(float) * 3 * NN);
     hipMalloc(&s->k3, sizeof(float) * 3 * NN);
-    hipMalloc(&s->errmax, sizeof(unsigned long long) * nsolve);
+    hipMalloc(&s->errmax, sizeof(unsigned long long) * nsolve * kErrPerSolve);
     hipMalloc(&s->sp, sizeof(SolveP) * nsolve);
     hipMalloc(&s->clk, sizeof(Clock) * nsolve);
     if (!s->y0 || !s->acc || !s->mA || !s->mB || !s->mC || !s->B0 || !s->Bs || !s->e || !s->k1 || !s->k2 ||
@@ -952,7 +1105,7 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     hipMemsetAsync(s->B0, 0, sizeof(float) * 3 * NN, s->st);
     hipMemsetAsync(s->Bs, 0, sizeof(float) * 3 * NN, s->st);
     hipMemsetAsync(s->clk, 0, sizeof(Clock) * nsolve, s->st);
-    hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * nsolve, s->st);
+    hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * nsolve * kErrPerSolve, s->st);
     // per-solve parameters; thermal K = thC / max(cutoff_dt, dt), mod_brown.go:125-145
     std::vector<SolveP> hsp(nsolve);
     const double V = p->cell[0] * p->cell[1] * p->cell[2];
@@ -1144,6 +1297,7 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
                              double mindt, double maxdt, hs_run_stats* stats) {
     if (!s || !dtio || nsteps < 0) return HS_EINVAL;
     if (!whole_batch_ok(s, "hs_solver_run: handle is in slot mode")) return HS_EINVAL;
+    if (method == HS_RK23) return run_rk23_device(s, nsteps, step0, dtio, maxErr, mindt, maxdt, stats);
     hs_run_stats st{};
     double dt = *dtio;
     const int maxTry = 10;
@@ -1160,57 +1314,10 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
             io.m = s->mA; io.B = s->B0; io.mout = s->mB;
             launch_stage<R_EULER>(s, io);
             std::swap(s->mA, s->mB);
+            drop_graphs(s);
             s->b0_valid = false;
             st.rhs_evals += 1;
             st.t += dt;
-        } else if (method == HS_RK23) {
-            int tr = 0;
-            double err = 0;
-            for (;;) {
-                ensure_b0(s);
-                StageIO io = base_io(s, dt, step, (uint32_t)tr);
-                io.m = s->mA; io.B = s->B0; io.mout = s->mB;
-                launch_stage<R_BS_1>(s, io);
-                demag_into(s, s->mB, s->Bs);
-                io.m = s->mB; io.B = s->Bs; io.mout = s->mC;
-                launch_stage<R_BS_2>(s, io);
-                demag_into(s, s->mC, s->Bs);
-                io.m = s->mC; io.B = s->Bs; io.mout = s->mB;
-                launch_stage<R_BS_3>(s, io);
-                demag_into(s, s->mB, s->Bs);
-                zero_err(s);
-                io.m = s->mB; io.B = s->Bs; io.mout = nullptr;
-                launch_stage<R_BS_4>(s, io);
-                st.rhs_evals += 3;
-                err = read_err(s);
-                if (!std::isfinite(err)) {
-                    hs::set_error(HS_ECONV, "hs_solver_run: RK23 error estimate is not finite");
-                    if (stats) *stats = st;
-                    return HS_ECONV;
-                }
-                bool bad = err > maxErr;
-                double nd = controller(dt, err, maxErr, bad, 1.0 / 3.0, 1.0 / 3.0, mindt, maxdt);
-                double used = dt;
-                dt = nd;
-                if (!bad || nd == mindt) {
-                    // accept: y1 (acc) becomes the state; its stage input and field are mB/Bs
-                    std::swap(s->y0, s->acc);
-                    std::swap(s->mA, s->mB);
-                    std::swap(s->B0, s->Bs);
-                    s->b0_valid = s->phys.demag != 0;
-                    st.t += used;
-                    break;
-                }
-                if (tr > maxTry) {
-                    hs::set_error(HS_ECONV, "hs_solver_run: RK23 did not converge");
-                    *dtio = dt;
-                    if (stats) *stats = st;
-                    return HS_ECONV;
-                }
-                tr++;
-                st.rejected++;
-            }
-            st.last_err = err;
         } else if (method == HS_RK12 || method == HS_HEUN) {
             int tr = 0;
             double err = 0;
@@ -1258,6 +1365,7 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
             }
             std::swap(s->y0, s->acc);
             std::swap(s->mA, s->mC);
+            drop_graphs(s);
             s->b0_valid = false;
             st.last_err = err;
         } else {
@@ -1288,14 +1396,9 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
         std::vector<Clock> hc(s->nsolve);
         memset(hc.data(), 0, sizeof(Clock) * s->nsolve);
         hipMemcpy(s->clk, hc.data(), sizeof(Clock) * s->nsolve, hipMemcpyHostToDevice);
-        hipMemset(s->errmax, 0, sizeof(unsigned long long) * s->nsolve);
-        if (hipHostMalloc((void**)&s->slot_host, sizeof(hs_slot) * s->nsolve) != hipSuccess ||
-            hipMalloc(&s->live, sizeof(int) * s->nsolve) != hipSuccess ||
-            hipMemset(s->live, 0, sizeof(int) * s->nsolve) != hipSuccess ||
-            hipEventCreateWithFlags(&s->slot_ev, hipEventDisableTiming) != hipSuccess) {
-            hs::set_error(HS_ENOMEM, "hs_slot_admit: pinned host memory");
-            return HS_ENOMEM;
-        }
+        hipMemset(s->errmax, 0, sizeof(unsigned long long) * s->nsolve * kErrPerSolve);
+        if (!ensure_devctl(s)) return HS_ENOMEM;
+        hipMemset(s->live, 0, sizeof(int) * s->nsolve);
         memset(s->slot_host, 0, sizeof(hs_slot) * s->nsolve);
         s->slot_mode = true;
         s->slot_iter = 0;
@@ -1350,10 +1453,7 @@ extern "C" int hs_slot_advance(hs_solver* s, int iters) {
         hs::set_error(HS_EINVAL, "hs_slot_advance: no slot admitted");
         return HS_EINVAL;
     }
-    for (int v = 0; v < 2; v++)
-        if (!s->slot_graph[v] &&
-            !capture(s, &s->slot_graph[v], "hs_slot_advance: graph capture failed", slot_iteration, v))
-            return HS_EINVAL;
+    if (!ensure_slot_graphs(s)) return HS_EINVAL;
     for (int n = 0; n < iters; n++) {
         hipGraphLaunch(s->slot_graph[s->slot_iter & 1], s->st);
         s->slot_iter++;
@@ -1416,7 +1516,7 @@ extern "C" long hs_solver_footprint(hs_solver* s) {
     if (!s) return 0;
     const long NN = s->N * s->nsolve;
     long b = NN * 3 * (2 * sizeof(double) + 9 * sizeof(float));  // y0, acc; mA..mC, B0, Bs, e, k1..k3
-    b += (long)s->nsolve * (sizeof(SolveP) + sizeof(Clock) + sizeof(unsigned long long));
+    b += (long)s->nsolve * (sizeof(SolveP) + sizeof(Clock) + sizeof(unsigned long long) * kErrPerSolve);
     if (s->kout) b += NN * 3 * 2 * sizeof(float);
     b += (long)s->masks.size() * s->N * sizeof(float);
     if (s->demag) b += hs_demag_footprint(s->demag);
