@@ -159,9 +159,13 @@ __global__ void k_kernel_fill(double* __restrict__ full, const double* __restric
 
 // Keep the real part of the first octant of the transformed kernel, scaled by
 // 1/(P0 P1 P2); record max|imag| and max|real| (extract(), maxwell_convolution.go:350-388).
-__global__ void k_kernel_extract(const cpx<double>* __restrict__ X, double* __restrict__ kern64,
-                                 float* __restrict__ kern32, int P0, int P1, int H2, int H2p, int K0, int K1,
+// kern64 rows at pitch H2, kern32 rows at pitch KP; the maxima go through a
+// block reduction into kExtractLanes (imag, real) pairs of partial maxima.
+constexpr int kExtractLanes = 32;
+__global__ void __launch_bounds__(256) k_kernel_extract(const cpx<double>* __restrict__ X, double* __restrict__ kern64,
+                                 float* __restrict__ kern32, int P0, int P1, int H2, int H2p, int K0, int K1, int KP,
                                  double norm, unsigned long long* __restrict__ maxbits) {
+    __shared__ double wm[2][4];
     long nk = (long)K0 * K1 * H2;
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int c = blockIdx.y;
@@ -173,7 +177,7 @@ __global__ void k_kernel_extract(const cpx<double>* __restrict__ X, double* __re
         cpx<double> v = X[(((long)c * P0 + i) * P1 + j) * H2p + k];
         double r = v.x * norm;
         kern64[c * nk + g] = r;
-        kern32[c * nk + g] = (float)r;
+        kern32[(long)c * K0 * K1 * KP + ((long)i * K1 + j) * KP + k] = (float)r;
         mi = fabs(v.y);
         mr = fabs(v.x);
     }
@@ -182,8 +186,18 @@ __global__ void k_kernel_extract(const cpx<double>* __restrict__ X, double* __re
         mr = fmax(mr, __shfl_xor(mr, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&maxbits[0], (unsigned long long)__double_as_longlong(mi));
-        atomicMax(&maxbits[1], (unsigned long long)__double_as_longlong(mr));
+        wm[0][threadIdx.x >> 6] = mi;
+        wm[1][threadIdx.x >> 6] = mr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            mi = fmax(mi, wm[0][w]);
+            mr = fmax(mr, wm[1][w]);
+        }
+        const int l = (int)((blockIdx.x + blockIdx.y) % kExtractLanes);
+        atomicMax(&maxbits[2 * l], (unsigned long long)__double_as_longlong(mi));
+        atomicMax(&maxbits[2 * l + 1], (unsigned long long)__double_as_longlong(mr));
     }
 }
 
@@ -221,9 +235,10 @@ void launch_cols(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int npencil, int
 template <typename T, typename TK, int L>
 void launch_zconv(cpx<T>* X, const TK* kern, int N0, int P1, int H2, int H2p, int K1, T srcMul, const cpx<T>* tw,
                   hipStream_t st) {
-    dim3 g((unsigned)((H2 + ZW - 1) / ZW), (unsigned)P1);
-    hipLaunchKernelGGL((k_zconv<T, TK, L>), g, dim3(z_nt<L>()), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul, tw,
-                       Live{nullptr, 1});
+    constexpr int W = z_w<T, L>();
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1));
+    hipLaunchKernelGGL((k_zconv<T, TK, L>), g, dim3(z_nt<T, L>()), 0, st, X, kern, N0, P1, H2, H2p, K1, H2, srcMul,
+                       tw, Live{nullptr, 1});
 }
 
 template <typename T, typename TI>
@@ -242,7 +257,7 @@ constexpr bool rows_fit() { return (long)(M + 1) * row_w<M>() * (long)sizeof(cpx
 template <typename T, int L>
 constexpr bool cols_fit() { return (long)L * col_w<T, L>() * (long)sizeof(cpx<T>) <= kMaxLds; }
 template <typename T, int L>
-constexpr bool z_fit() { return L <= 256 && (long)L * 3 * ZW * (long)sizeof(cpx<T>) <= kMaxLds; }
+constexpr bool z_fit() { return L <= 256 && (long)L * 6 * z_w<T, L>() * (long)sizeof(cpx<T>) <= kMaxLds; }
 
 template <typename T, typename TI>
 r2c_t<T, TI> pick_r2c(int M) {
@@ -296,7 +311,8 @@ cpx<T>* make_twiddles(int L) {
 struct hs_demag {
     int N[3], P[3], K[3], pbc[3];
     double cell[3];
-    int M, H2, H2p;
+    int M, H2, H2p;  // H2p: spectrum row pitch (complex), a multiple of 16 = 128 bytes in FP32
+    int KP;          // FP32 kernel row pitch (floats), a multiple of 32 = 128 bytes
     long ncell;
     double* kern64 = nullptr;
     float* kern32 = nullptr;
@@ -351,7 +367,7 @@ int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, fl
     d->col32.fwd(X1, X2, g1, g2, 3 * N0 * nsolve, N1, P1, H2, d->tw1f, st, lpen);
     hs::prof_end(HS_PROF_F2, st);
     hs::prof_begin(HS_PROF_F3K, st);
-    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], (float)srcMul, d->tw0f, st, nsolve, lsol);
+    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], d->KP, (float)srcMul, d->tw0f, st, nsolve, lsol);
     hs::prof_end(HS_PROF_F3K, st);
     hs::prof_begin(HS_PROF_I2, st);
     d->col32.inv(X2, X1, g2, g1, 3 * N0 * nsolve, P1, N1, H2, d->tw1f, st, lpen);
@@ -416,7 +432,7 @@ void tune32(hs_demag* d, hipStream_t st) {
     }
     best = 1e30f;
     for (auto& v : vz.z) {
-        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], 1.0f, d->tw0f, st, 1, Live{nullptr, 1}); });
+        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], d->KP, 1.0f, d->tw0f, st, 1, Live{nullptr, 1}); });
         if (t < best) { best = t; d->z32 = v; }
     }
     hipEventDestroy(e0);
@@ -486,7 +502,8 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     }
     d->M = d->P[2] / 2;
     d->H2 = d->M + 1;
-    d->H2p = (d->H2 + 7) / 8 * 8;
+    d->H2p = (d->H2 + 15) / 16 * 16;
+    d->KP = (d->H2 + 31) / 32 * 32;
     d->K[0] = d->P[0] / 2 + 1;
     d->K[1] = d->P[1] / 2 + 1;
     d->K[2] = d->H2;
@@ -508,7 +525,9 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     d->tw1d = make_twiddles<double>(P1);
     d->tw0d = make_twiddles<double>(P0);
     hipMalloc(&d->kern64, sizeof(double) * 6 * nk);
-    hipMalloc(&d->kern32, sizeof(float) * 6 * nk);
+    const long nk32 = (long)d->K[0] * d->K[1] * d->KP;
+    hipMalloc(&d->kern32, sizeof(float) * 6 * nk32);
+    if (d->kern32) hipMemsetAsync(d->kern32, 0, sizeof(float) * 6 * nk32, st);
     hipMalloc(&d->X1f, sizeof(cpx<float>) * 3L * N[0] * N[1] * H2p);
     hipMalloc(&d->X2f, sizeof(cpx<float>) * 3L * N[0] * P1 * H2p);
     // ---- Kernel_Arne on the device (FP64)
@@ -532,7 +551,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     hipMalloc(&oct, sizeof(double) * 6 * noct);
     hipMalloc(&full, sizeof(double) * 6 * P);
     hipMalloc(&X, sizeof(cpx<double>) * 6L * P0 * P1 * H2p);
-    hipMalloc(&maxbits, 2 * sizeof(unsigned long long));
+    hipMalloc(&maxbits, 2 * kExtractLanes * sizeof(unsigned long long));
     if (!oct || !full || !X || !maxbits || !d->kern64 || !d->kern32 || !d->X1f || !d->X2f) {
         hs::set_error(HS_ENOMEM, "hs_demag_create: out of device memory");
         hipFree(oct); hipFree(full); hipFree(X); hipFree(maxbits);
@@ -540,7 +559,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
         return nullptr;
     }
     hipMemsetAsync(oct, 0, sizeof(double) * 6 * noct, st);
-    hipMemsetAsync(maxbits, 0, 2 * sizeof(unsigned long long), st);
+    hipMemsetAsync(maxbits, 0, 2 * kExtractLanes * sizeof(unsigned long long), st);
     hipLaunchKernelGGL(k_kernel_arne, dim3(hs::div_up(3 * noct, 64)), dim3(64), 0, st, oct, P0, P1, P2, cell[0],
                        cell[1], cell[2], r1[0], r1[1], r1[2], r2[0], r2[1], r2[2], (double)(accuracy > 0 ? accuracy : 6));
     hipLaunchKernelGGL(k_kernel_fill, dim3(hs::grid1d(P), 6), dim3(hs::kBlock), 0, st, full, oct, P0, P1, P2);
@@ -563,19 +582,24 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     Geo ga0 = {(long)P0 * P1 * H2p, (long)H2p, (long)P1 * H2p, P1};
     f0(X, X, ga0, ga0, 6 * P1, P0, P0, H2, d->tw0d, st);
     hipLaunchKernelGGL(k_kernel_extract, dim3(hs::grid1d(nk), 6), dim3(hs::kBlock), 0, st, X, d->kern64, d->kern32,
-                       P0, P1, H2, H2p, d->K[0], d->K[1], 1.0 / (double)P, maxbits);
+                       P0, P1, H2, H2p, d->K[0], d->K[1], d->KP, 1.0 / (double)P, maxbits);
     for (int c = 0; c < 6; c++) {
         if (gmax > 0 && cmax[c] / gmax < 1e-12) {
             hipMemsetAsync(d->kern64 + c * nk, 0, sizeof(double) * nk, st);
-            hipMemsetAsync(d->kern32 + c * nk, 0, sizeof(float) * nk, st);
+            hipMemsetAsync(d->kern32 + c * nk32, 0, sizeof(float) * nk32, st);
         }
     }
-    unsigned long long mb[2];
+    unsigned long long mb[2 * kExtractLanes];
     hipMemcpyAsync(mb, maxbits, sizeof(mb), hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);
-    double mi, mr;
-    memcpy(&mi, &mb[0], 8);
-    memcpy(&mr, &mb[1], 8);
+    double mi = 0, mr = 0;
+    for (int l = 0; l < kExtractLanes; l++) {
+        double a, b;
+        memcpy(&a, &mb[2 * l], 8);
+        memcpy(&b, &mb[2 * l + 1], 8);
+        mi = fmax(mi, a);
+        mr = fmax(mr, b);
+    }
     d->max_imag_ratio = mr > 0 ? mi / mr : 0;
     hipFree(oct);
     hipFree(full);
@@ -689,6 +713,7 @@ long hs_demag_footprint(const hs_demag* d) {
     const long nk = (long)d->K[0] * d->K[1] * d->K[2];
     long b = 6 * nk * (sizeof(double) + sizeof(float));
     b += (long)sizeof(cpx<float>) * 3L * d->N[0] * (d->N[1] + d->P[1]) * d->H2p * d->cap;
+    b += 6L * d->K[0] * d->K[1] * (d->KP - d->H2) * (long)sizeof(float);
     if (d->X1d) b += (long)sizeof(cpx<double>) * 3L * d->N[0] * (d->N[1] + d->P[1]) * d->H2p;
     b += (long)(sizeof(cpx<float>) + sizeof(cpx<double>)) * (d->M + d->P[2] + d->P[1] + d->P[0]);
     return b;

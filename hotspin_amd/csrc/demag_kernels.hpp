@@ -25,10 +25,17 @@ template <int M>
 constexpr int row_nt() {
     return clampi(M * row_w<M>() / 8, 64, 512);
 }
-constexpr int ZW = 16;  // columns per block of the axis-0 convolution pass
-template <int L>
+// columns per block of the axis-0 convolution pass (pencil pair: 6W pencils
+// of length L in LDS, at most 128 KB)
+template <typename T, int L>
+constexpr int z_w() {
+    int w = 16;
+    while (w > 1 && (long)L * 6 * w * (long)sizeof(cpx<T>) > 131072) w /= 2;
+    return w;
+}
+template <typename T, int L>
 constexpr int z_nt() {
-    return clampi(L * 3 * ZW / 16, 64, 1024);
+    return clampi(L * 6 * z_w<T, L>() / 16, 64, 1024);
 }
 
 // Per-solve liveness of a batched pass: work units [q*per, (q+1)*per) (rows,
@@ -43,6 +50,17 @@ __device__ __forceinline__ bool all_dead(const Live& lv, long u0, long u1) {
     if (!lv.flag) return false;
     const long q0 = u0 / lv.per, q1 = u1 / lv.per;
     return q0 == q1 && !lv.flag[q0];
+}
+
+// XCD-aware block order.  Workgroups are handed to the 8 XCDs round-robin
+// (block b runs on XCD b % 8), and each XCD has its own L2.  Renumbering the
+// blocks of a 2-D grid so that consecutive logical tiles run on the same XCD
+// lets two tiles that share a 128-byte line hit one L2.  Bijective on [0, nb).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_swizzle(int pid, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = pid % kXcds, i = pid / kXcds;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
 struct Geo {  // pencil a starts at (a / A2) * S1 + (a % A2) * S2; element t at + t * ts
@@ -157,9 +175,10 @@ __global__ void __launch_bounds__(NT) k_cols(const cpx<T>* in, cpx<T>* out, Geo 
                                              int ncol, const cpx<T>* __restrict__ tw, Live lv) {
     __shared__ cpx<T> lds[L * W];
     const int tid = threadIdx.x;
-    const int a = blockIdx.y;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int a = lb / gridDim.x;
     if (all_dead(lv, a, a)) return;
-    const int col0 = blockIdx.x * W;
+    const int col0 = (lb % gridDim.x) * W;
     const long bi = geo_base(gi, a) + col0;
     const long bo = geo_base(go, a) + col0;
     for (int e = tid; e < L * W; e += NT) {
@@ -177,57 +196,71 @@ __global__ void __launch_bounds__(NT) k_cols(const cpx<T>* in, cpx<T>* out, Geo 
 }
 
 // ------------------------------------------------------------------ F3K: axis 0 + kernel product
-// X2 layout (3, N0, P1, H2p); block = (column tile, j'), all 3 components.
-// Kernel octants: 6 planes (XX,YY,ZZ,YZ,XZ,XY) of (K0, K1, H2).
-template <typename T, typename TK, int L, int W = ZW, int NT = z_nt<L>()>
+// X2 layout (3, N0, P1, H2p).  A block takes a tile of W columns of pencil
+// j' = blockIdx.y and of its mirror P1 - j' (both use kernel row |j'|, so the
+// kernel octant is read once for the pair), all 3 components: 6W pencils of
+// length L in LDS as lds[t * 6W + s * 3W + c * W + p], s = 0 (j') / 1 (mirror).
+// Kernel octants: 6 planes (XX,YY,ZZ,YZ,XZ,XY) of (K0, K1, H2), rows at
+// pitch kp (kp >= H2; padded so FP32 rows start on 128-byte lines).  Grid y =
+// P1/2 + 1; j' = 0 and j' = P1/2 (even P1) have no distinct mirror.
+template <typename T, typename TK, int L, int W = z_w<T, L>(), int NT = z_nt<T, L>()>
 __global__ void __launch_bounds__(NT) k_zconv(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
-                                              int H2p, int K1, T srcMul, const cpx<T>* __restrict__ tw, Live lv) {
-    constexpr int WP = 3 * W;
+                                              int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
+                                              Live lv) {
+    constexpr int WP = 3 * W, W2 = 2 * WP;
     if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
     X += (long)blockIdx.z * 3 * N0 * P1 * H2p;  // solve of a batch
-    __shared__ cpx<T> lds[L * WP];
+    __shared__ cpx<T> lds[L * W2];
     const int tid = threadIdx.x;
-    const int jp = blockIdx.y;
-    const int col0 = blockIdx.x * W;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int jp = lb / gridDim.x;                              // 0 .. P1/2
+    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;    // mirror pencil, if distinct
+    const int col0 = (lb % gridDim.x) * W;
     const long cstride = (long)N0 * P1 * H2p;
     const long tstride = (long)P1 * H2p;
-    const long base = (long)jp * H2p + col0;
-    for (int e = tid; e < L * WP; e += NT) {
-        int t = e / WP, q = e % WP;
-        int c = q / W, p = q % W;
+    for (int e = tid; e < L * W2; e += NT) {
+        int t = e / W2, q = e % W2;
+        int sm = q / WP, r = q % WP;
+        int c = r / W, p = r % W;
+        int j = sm ? jm : jp;
         cpx<T> v = {T(0), T(0)};
-        if (t < N0 && col0 + p < H2) v = X[c * cstride + base + t * tstride + p];
+        if (t < N0 && col0 + p < H2 && j >= 0) v = X[c * cstride + (long)j * H2p + col0 + t * tstride + p];
         lds[e] = v;
     }
     __syncthreads();
-    fft_lds<T, L, WP, NT, -1>(lds, tw, tid);
-    const int jj = (jp > P1 / 2) ? P1 - jp : jp;
-    const T sj = (jp > P1 / 2) ? T(-1) : T(1);
-    const long kplane = (long)(L / 2 + 1) * K1 * H2;
+    fft_lds<T, L, W2, NT, -1>(lds, tw, tid);
+    const long kplane = (long)(L / 2 + 1) * K1 * kp;
     for (int e = tid; e < L * W; e += NT) {
         int t = e / W, p = e % W;
         int col = col0 + p;
         if (col >= H2) continue;
         int ii = (t > L / 2) ? L - t : t;
         T si = (t > L / 2) ? T(-1) : T(1);
-        long K = ((long)ii * K1 + jj) * H2 + col;
+        long K = ((long)ii * K1 + jp) * kp + col;
         T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
-        T kyz = sj * T(kern[3 * kplane + K]);
+        T kyz0 = T(kern[3 * kplane + K]);
         T kxz = si * T(kern[4 * kplane + K]);
-        T kxy = si * sj * T(kern[5 * kplane + K]);
-        cpx<T> mx = cscale(lds[t * WP + p], srcMul);
-        cpx<T> my = cscale(lds[t * WP + W + p], srcMul);
-        cpx<T> mz = cscale(lds[t * WP + 2 * W + p], srcMul);
-        lds[t * WP + p] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
-        lds[t * WP + W + p] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
-        lds[t * WP + 2 * W + p] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
+        T kxy0 = si * T(kern[5 * kplane + K]);
+#pragma unroll
+        for (int sm = 0; sm < 2; sm++) {  // mirror pencil: YZ and XY change sign
+            const T sj = sm ? T(-1) : T(1);
+            const T kyz = sj * kyz0, kxy = sj * kxy0;
+            cpx<T>* a = lds + t * W2 + sm * WP + p;
+            cpx<T> mx = cscale(a[0], srcMul);
+            cpx<T> my = cscale(a[W], srcMul);
+            cpx<T> mz = cscale(a[2 * W], srcMul);
+            a[0] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
+            a[W] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
+            a[2 * W] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
+        }
     }
     __syncthreads();
-    fft_lds<T, L, WP, NT, +1>(lds, tw, tid);
-    for (int e = tid; e < N0 * WP; e += NT) {
-        int t = e / WP, q = e % WP;
-        int c = q / W, p = q % W;
-        if (col0 + p < H2) X[c * cstride + base + t * tstride + p] = lds[e];
+    fft_lds<T, L, W2, NT, +1>(lds, tw, tid);
+    for (int e = tid; e < N0 * W2; e += NT) {
+        int t = e / W2, q = e % W2;
+        int sm = q / WP, r = q % WP;
+        int c = r / W, p = r % W;
+        int j = sm ? jm : jp;
+        if (col0 + p < H2 && j >= 0) X[c * cstride + (long)j * H2p + col0 + t * tstride + p] = lds[e];
     }
 }
-

@@ -15,8 +15,8 @@ using c2r32_t = void (*)(const cpx<float>*, float*, long, int, int, const cpx<fl
                          hipStream_t, Live);
 using cols32_t = void (*)(const cpx<float>*, cpx<float>*, Geo, Geo, int, int, int, int, const cpx<float>*,
                           hipStream_t, Live);
-using z32_t = void (*)(cpx<float>*, const float*, int, int, int, int, int, float, const cpx<float>*, hipStream_t,
-                       int, Live);
+using z32_t = void (*)(cpx<float>*, const float*, int, int, int, int, int, int, float, const cpx<float>*,
+                       hipStream_t, int, Live);
 struct RowV {
     int W, NT;
     r2c32_t r2c;
@@ -60,11 +60,11 @@ void launch_cols32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int np
     hipLaunchKernelGGL((k_cols<float, L, S, W, NT>), g, dim3(NT), 0, st, in, out, gi, go, nin, nkeep, ncol, tw, lv);
 }
 template <int L, int W, int NT>
-void launch_z32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, float srcMul,
+void launch_z32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
                 const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
-    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)P1, (unsigned)nsolve);
-    hipLaunchKernelGGL((k_zconv<float, float, L, W, NT>), g, dim3(NT), 0, st, X, kern, N0, P1, H2, H2p, K1, srcMul,
-                       tw, lv);
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
+    hipLaunchKernelGGL((k_zconv<float, float, L, W, NT>), g, dim3(NT), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
+                       srcMul, tw, lv);
 }
 
 template <int M, int W, int E>
@@ -78,9 +78,9 @@ void add_col(Variants& v) {
     if constexpr ((long)L * W * 8 <= kLdsMax) v.cols.push_back({W, NT, &launch_cols32<L, -1, W, NT>, &launch_cols32<L, 1, W, NT>});
 }
 template <int L, int W, int E>
-void add_z(Variants& v) {
-    constexpr int NT = clamp_nt((long)L * 3 * W / E);
-    if constexpr (L <= 256 && (long)L * 3 * W * 8 <= kLdsMax) v.z.push_back({W, NT, &launch_z32<L, W, NT>});
+void add_z(Variants& v) {  // a block holds a pencil pair: 6W pencils
+    constexpr int NT = clamp_nt((long)L * 6 * W / E);
+    if constexpr (L <= 256 && (long)L * 6 * W * 8 <= kLdsMax) v.z.push_back({W, NT, &launch_z32<L, W, NT>});
 }
 
 template <int L>
@@ -97,10 +97,11 @@ void register_size() {
     add_col<L, 16, 8>(v);
     add_col<L, 4, 8>(v);
     add_col<L, 4, 4>(v);
+    add_z<L, 8, 12>(v);
     add_z<L, 16, 12>(v);
     add_z<L, 32, 12>(v);
-    add_z<L, 64, 12>(v);
-    add_z<L, 32, 6>(v);
-    add_z<L, 16, 6>(v);
+    add_z<L, 16, 24>(v);
+    add_z<L, 32, 24>(v);
+    add_z<L, 8, 6>(v);
 }
 }  // namespace hsdm
