@@ -34,9 +34,23 @@ struct SolveP {
     float TMul;     // temperature multiplier
     float bext[3];  // B_ext / mu0
     double thC;     // 2 kB TMul / (mu0 V gamma Ms0T0Mul); thermal K = thC / max(cutoff_dt, dt)
+    double lf_b;    // uniform longitudinal field: x = |m| * lf_b = 3 preS Tc / T |m|
+    double lf_c1;   // and H_lf = lf_c1 (B(x) - |m|) / (msk dB/dx |m|) m, lf_c1 = n kB T / (mu0 Ms0T0)
     uint64_t seed;
     uint32_t stream;
     uint32_t pad;
+};
+
+// Material constants of the longitudinal field when J, n, Tc and T are
+// uniform (no masks): B_S(x) and dB_S/dx of common_func.h:68-136 with the
+// per-spin constants folded in, coth by one exp, and dB/dx written as
+// l^2/sinh^2(l x) - g^2/sinh^2(g x) (no O(1) terms cancelling).
+struct LfConst {
+    int uniform;   // fast path usable
+    int langevin;  // S > 1e5
+    double l, g, lim;
+    double a1, a3, a5, a7;  // series of B: x (a1 - x^2 (a3 - x^2 (a5 - x^2 a7)))
+    double d0, d2, d4, d6;  // series of dB/dx
 };
 
 // Per-solve clock and slot record (hs_slot): read by the stage kernels when
@@ -66,6 +80,7 @@ struct Phys {
     float gamma;
     float aMul[3];
     const float* alpha[3];
+    LfConst lf;
 };
 
 // Roles of the fused stage kernel.
@@ -158,11 +173,15 @@ __device__ __forceinline__ void load_tile(const Phys& P, const float* m, long N,
 
 // 6-neighbour exchange field of cell (i,j,k), exchange6.cu:11-127 in FP32
 // with the differences formed first: pre_d (pre1 (m1-m0) + pre2 (m2-m0)).
-// lo/hi: planes i-1, i+1 (registers); the j,k neighbours come from the tile.
+// lo/hi: planes i-1, i+1 (registers); the j,k neighbours come from the tile
+// (row ty+1, column cc holds the cell itself; halo rows/columns hold the
+// neighbours of the tile's edge cells).  At a periodic domain edge the
+// wrapped neighbour is read from memory; at an open edge it is the cell itself.
+template <int TW>
 __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, long N, int i, int j, int k, int jc,
-                                              int kc, int tx, int ty, const float m0[3], const float lo[3],
+                                              int cc, int ty, const float m0[3], const float lo[3],
                                               const float hi[3], float l0, float llo, float lhi,
-                                              float (*tile)[TY + 2][TX + 2], float (*ltile)[TX + 2], float H[3]) {
+                                              float (*tile)[TY + 2][TW + 2], float (*ltile)[TW + 2], float H[3]) {
     const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
     const long N12 = (long)N1 * N2;
     const float l02 = l0 * l0;
@@ -183,7 +202,7 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
             for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
             l1 = P.lex ? P.lex[aa] : 1.f;
         } else {
-            int col = (k - 1 < 0) ? tx + 1 : tx;
+            int col = (k - 1 < 0) ? cc : cc - 1;
             for (int c = 0; c < 3; c++) v1[c] = tile[c][ty + 1][col];
             l1 = P.lex ? ltile[ty + 1][col] : 1.f;
         }
@@ -192,7 +211,7 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
             for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
             l2 = P.lex ? P.lex[aa] : 1.f;
         } else {
-            int col = (k + 1 >= N2) ? tx + 1 : tx + 2;
+            int col = (k + 1 >= N2) ? cc : cc + 1;
             for (int c = 0; c < 3; c++) v2[c] = tile[c][ty + 1][col];
             l2 = P.lex ? ltile[ty + 1][col] : 1.f;
         }
@@ -202,22 +221,22 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
     {  // axis 1 (Y)
         float v1[3], v2[3], l1, l2;
         if (j - 1 < 0 && P.w1) {
-            long aa = i * N12 + (long)(N1 - 1) * N2 + kc;
+            long aa = i * N12 + (long)(N1 - 1) * N2 + k;
             for (int c = 0; c < 3; c++) v1[c] = m[c * N + aa];
             l1 = P.lex ? P.lex[aa] : 1.f;
         } else {
             int row = (j - 1 < 0) ? ty + 1 : ty;
-            for (int c = 0; c < 3; c++) v1[c] = tile[c][row][tx + 1];
-            l1 = P.lex ? ltile[row][tx + 1] : 1.f;
+            for (int c = 0; c < 3; c++) v1[c] = tile[c][row][cc];
+            l1 = P.lex ? ltile[row][cc] : 1.f;
         }
         if (j + 1 >= N1 && P.w1) {
-            long aa = i * N12 + kc;
+            long aa = i * N12 + k;
             for (int c = 0; c < 3; c++) v2[c] = m[c * N + aa];
             l2 = P.lex ? P.lex[aa] : 1.f;
         } else {
             int row = (j + 1 >= N1) ? ty + 1 : ty + 2;
-            for (int c = 0; c < 3; c++) v2[c] = tile[c][row][tx + 1];
-            l2 = P.lex ? ltile[row][tx + 1] : 1.f;
+            for (int c = 0; c < 3; c++) v2[c] = tile[c][row][cc];
+            l2 = P.lex ? ltile[row][cc] : 1.f;
         }
         float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
         for (int c = 0; c < 3; c++) e1[c] = P.ex1 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
@@ -266,10 +285,9 @@ __device__ __forceinline__ void preload(const Phys& P, const StageIO& io, long I
 }
 
 // Everything after the exchange term for one cell: B/mu0, B_ext/mu0, H_lf,
-// H_therm, k = gamma (m x H + alpha H) and the RK stage update (FP64 state).
-template <int ROLE>
-__device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, const StageIO& io, long I, long N,
-                                            const float m0[3], float H[3], const Pre& q, double& errloc) {
+// H_therm and k = gamma (m x H + alpha H).
+__device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, const StageIO& io, long I,
+                                             const float m0[3], float H[3], const Pre& q, float kk[3]) {
     const float ms = q.ms;
     if (P.demag) {
         H[0] += P.invMu0 * q.B[0];
@@ -281,7 +299,33 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
         H[1] += sp.bext[1];
         H[2] += sp.bext[2];
     }
-    if (P.longfield) {
+    if (P.longfield && P.lf.uniform) {
+        const LfConst& C = P.lf;
+        double mx = m0[0], my = m0[1], mz = m0[2];
+        double am = sqrt(mx * mx + my * my + mz * mz);
+        if (!(ms == 0.f || am <= HS_ZERO)) {
+            double x = am * sp.lf_b;
+            double W, dW;
+            if (x < C.lim) {
+                double x2 = x * x;
+                W = x * (C.a1 - x2 * (C.a3 - x2 * (C.a5 - x2 * C.a7)));
+                dW = C.d0 - x2 * (C.d2 - x2 * (C.d4 - x2 * C.d6));
+            } else if (!C.langevin) {
+                double eg = exp(-2.0 * C.g * x), el = exp(-2.0 * C.l * x);
+                double ig = 1.0 / (1.0 - eg), il = 1.0 / (1.0 - el);
+                W = C.g * (1.0 + eg) * ig - C.l * (1.0 + el) * il;
+                dW = 4.0 * (C.l * C.l) * el * (il * il) - 4.0 * (C.g * C.g) * eg * (ig * ig);
+            } else {
+                double e = exp(-2.0 * x), i1 = 1.0 / (1.0 - e);
+                W = (1.0 + e) * i1 - 1.0 / x;
+                dW = 1.0 / (x * x) - 4.0 * e * (i1 * i1);
+            }
+            double mult = sp.lf_c1 * (W - am) / ((double)ms * dW * am);
+            H[0] += (float)(mult * mx);
+            H[1] += (float)(mult * my);
+            H[2] += (float)(mult * mz);
+        }
+    } else if (P.longfield) {
         // FP64: dB_S/dx is a difference of O(1) terms (common_func.h:95-98)
         double Ms0T0 = (double)P.ms0T0Mul * ms;
         double S = (double)P.JMul * q.J;
@@ -313,7 +357,7 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
         H[1] += pre * (s1 * x1);
         H[2] += pre * (s2 * x2);
     }
-    float kk[3] = {0.f, 0.f, 0.f};
+    kk[0] = kk[1] = kk[2] = 0.f;
     if (ms != 0.f) {
         if (P.torque) {  // cross(H, m) = m x H
             kk[0] = P.gamma * (-H[1] * m0[2] + H[2] * m0[1]);
@@ -327,101 +371,195 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, con
             }
         }
     }
+}
+
+// What a stage writes for one cell (which fields are used depends on ROLE):
+// k -> kout / k1 / k2 / k3 / e (RK12 k1); m -> next stage input; h -> hout
+// (RHS) / e (BS_3); y -> y0 or acc.
+struct Out {
+    float k[3], m[3], h[3];
+    double y[3];
+};
+
+// The RK stage update of one cell (FP64 state) into `o`.
+template <int ROLE>
+__device__ __forceinline__ void stage_update(const Phys& P, const StageIO& io, const Pre& q, const float kk[3],
+                                             const float H[3], Out& o, double& errloc) {
     const double h = io.h;
+    auto nrm = [&](double* v) {
+        if (P.normalize) {
+            double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            double sc = (l == 0.0) ? 0.0 : 1.0 / l;
+            v[0] *= sc;
+            v[1] *= sc;
+            v[2] *= sc;
+        }
+    };
+    auto set_m = [&](const double* v) {
+        double w[3] = {v[0], v[1], v[2]};
+        nrm(w);
+        for (int c = 0; c < 3; c++) o.m[c] = (float)w[c];
+    };
+    double y[3], acc[3];
     if constexpr (ROLE == R_RHS) {
         for (int c = 0; c < 3; c++) {
-            io.kout[c * N + I] = kk[c];
-            io.hout[c * N + I] = H[c];
+            o.k[c] = kk[c];
+            o.h[c] = H[c];
         }
         return;
-    } else {
-        double y[3], acc[3];
-        auto nrm = [&](double* v) {
-            if (P.normalize) {
-                double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                double s = (l == 0.0) ? 0.0 : 1.0 / l;
-                v[0] *= s;
-                v[1] *= s;
-                v[2] *= s;
-            }
-        };
-        auto store_m = [&](const double* v) {
-            double w[3] = {v[0], v[1], v[2]};
-            nrm(w);
-            for (int c = 0; c < 3; c++) io.mout[c * N + I] = (float)w[c];
-        };
-        if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER) {
-            for (int c = 0; c < 3; c++) y[c] = q.y[c];
-            if (P.normalize) {
-                nrm(y);
-                for (int c = 0; c < 3; c++) io.y0[c * N + I] = y[c];
-            }
+    }
+    if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER) {
+        for (int c = 0; c < 3; c++) y[c] = q.y[c];
+        nrm(y);
+        for (int c = 0; c < 3; c++) o.y[c] = y[c];  // y0 written back only with the normaliser
+    }
+    if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1) {
+        for (int c = 0; c < 3; c++) {
+            o.k[c] = kk[c];
+            acc[c] = y[c] + (0.5 * h) * (double)kk[c];
         }
-        if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1) {
-            for (int c = 0; c < 3; c++) {
-                io.k1[c * N + I] = kk[c];
-                acc[c] = y[c] + (0.5 * h) * (double)kk[c];
-            }
-            store_m(acc);
-        } else if constexpr (ROLE == R_RK4_2 || ROLE == R_RK4_3 || ROLE == R_BS_2) {
-            const double cm = (ROLE == R_RK4_3) ? h : ((ROLE == R_BS_2) ? 0.75 * h : 0.5 * h);
-            float* kst = (ROLE == R_RK4_3) ? io.k3 : io.k2;
-            for (int c = 0; c < 3; c++) {
-                kst[c * N + I] = kk[c];
-                acc[c] = q.y[c] + cm * (double)kk[c];
-            }
-            store_m(acc);
-        } else if constexpr (ROLE == R_RK4_4) {
-            const double h6 = h / 6.0;
-            for (int c = 0; c < 3; c++) {
-                double k1 = q.ka[c], k2 = q.kb[c], k3 = q.kc[c];
-                acc[c] = q.y[c] + h6 * (k1 + 2.0 * k2 + 2.0 * k3 + (double)kk[c]);
-                io.y0[c * N + I] = acc[c];
-            }
-            store_m(acc);
-        } else if constexpr (ROLE == R_BS_3) {
-            for (int c = 0; c < 3; c++) {
-                double k1 = q.ka[c], k2 = q.kb[c], k3 = kk[c];
-                acc[c] = q.y[c] + h * ((2.0 / 9.0) * k1 + (1.0 / 3.0) * k2 + (4.0 / 9.0) * k3);
-                io.acc[c * N + I] = acc[c];
-                io.e[c * N + I] = (float)(h * ((-5.0 / 72.0) * k1 + (1.0 / 12.0) * k2 + (1.0 / 9.0) * k3));
-            }
-            store_m(acc);
-        } else if constexpr (ROLE == R_BS_4) {
-            for (int c = 0; c < 3; c++) {
-                double ev = (double)q.ka[c] - (h / 8.0) * (double)kk[c];
-                errloc = fmax(errloc, fabs(ev));
-            }
-            if (P.normalize) {
-                for (int c = 0; c < 3; c++) acc[c] = q.y[c];
-                nrm(acc);
-                for (int c = 0; c < 3; c++) io.acc[c * N + I] = acc[c];
-            }
-        } else if constexpr (ROLE == R_RK12_1) {
-            for (int c = 0; c < 3; c++) {
-                io.e[c * N + I] = kk[c];
-                acc[c] = y[c] + h * (double)kk[c];
-                io.acc[c * N + I] = acc[c];
-            }
-            store_m(acc);
-        } else if constexpr (ROLE == R_RK12_2) {
-            for (int c = 0; c < 3; c++) acc[c] = q.y[c];
-            nrm(acc);
-            for (int c = 0; c < 3; c++) {
-                float k1 = q.ka[c];
-                acc[c] += (0.5 * h) * (double)kk[c] + (-0.5 * h) * (double)k1;
-                io.acc[c * N + I] = acc[c];
-                errloc = fmax(errloc, fabs((double)kk[c] - (double)k1));
-            }
-            store_m(acc);
-        } else if constexpr (ROLE == R_EULER) {
-            for (int c = 0; c < 3; c++) {
-                acc[c] = y[c] + h * (double)kk[c];
-                io.y0[c * N + I] = acc[c];
-            }
-            store_m(acc);
+        set_m(acc);
+    } else if constexpr (ROLE == R_RK4_2 || ROLE == R_RK4_3 || ROLE == R_BS_2) {
+        const double cm = (ROLE == R_RK4_3) ? h : ((ROLE == R_BS_2) ? 0.75 * h : 0.5 * h);
+        for (int c = 0; c < 3; c++) {
+            o.k[c] = kk[c];
+            acc[c] = q.y[c] + cm * (double)kk[c];
+        }
+        set_m(acc);
+    } else if constexpr (ROLE == R_RK4_4) {
+        const double h6 = h / 6.0;
+        for (int c = 0; c < 3; c++) {
+            double k1 = q.ka[c], k2 = q.kb[c], k3 = q.kc[c];
+            acc[c] = q.y[c] + h6 * (k1 + 2.0 * k2 + 2.0 * k3 + (double)kk[c]);
+            o.y[c] = acc[c];
+        }
+        set_m(acc);
+    } else if constexpr (ROLE == R_BS_3) {
+        for (int c = 0; c < 3; c++) {
+            double k1 = q.ka[c], k2 = q.kb[c], k3 = kk[c];
+            acc[c] = q.y[c] + h * ((2.0 / 9.0) * k1 + (1.0 / 3.0) * k2 + (4.0 / 9.0) * k3);
+            o.y[c] = acc[c];
+            o.h[c] = (float)(h * ((-5.0 / 72.0) * k1 + (1.0 / 12.0) * k2 + (1.0 / 9.0) * k3));
+        }
+        set_m(acc);
+    } else if constexpr (ROLE == R_BS_4) {
+        for (int c = 0; c < 3; c++) {
+            double ev = (double)q.ka[c] - (h / 8.0) * (double)kk[c];
+            errloc = fmax(errloc, fabs(ev));
+        }
+        for (int c = 0; c < 3; c++) acc[c] = q.y[c];
+        nrm(acc);
+        for (int c = 0; c < 3; c++) o.y[c] = acc[c];  // written only with the normaliser
+    } else if constexpr (ROLE == R_RK12_1) {
+        for (int c = 0; c < 3; c++) {
+            o.k[c] = kk[c];
+            acc[c] = y[c] + h * (double)kk[c];
+            o.y[c] = acc[c];
+        }
+        set_m(acc);
+    } else if constexpr (ROLE == R_RK12_2) {
+        for (int c = 0; c < 3; c++) acc[c] = q.y[c];
+        nrm(acc);
+        for (int c = 0; c < 3; c++) {
+            float k1 = q.ka[c];
+            acc[c] += (0.5 * h) * (double)kk[c] + (-0.5 * h) * (double)k1;
+            o.y[c] = acc[c];
+            errloc = fmax(errloc, fabs((double)kk[c] - (double)k1));
+        }
+        set_m(acc);
+    } else if constexpr (ROLE == R_EULER) {
+        for (int c = 0; c < 3; c++) {
+            acc[c] = y[c] + h * (double)kk[c];
+            o.y[c] = acc[c];
+        }
+        set_m(acc);
+    }
+}
+
+// Destinations of a role's outputs.
+template <int ROLE>
+__device__ __forceinline__ float* k_dst(const StageIO& io) {
+    if constexpr (ROLE == R_RHS) return io.kout;
+    else if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1) return io.k1;
+    else if constexpr (ROLE == R_RK4_2 || ROLE == R_BS_2) return io.k2;
+    else if constexpr (ROLE == R_RK4_3) return io.k3;
+    else if constexpr (ROLE == R_RK12_1) return io.e;
+    else return nullptr;
+}
+template <int ROLE>
+__device__ __forceinline__ float* h_dst(const StageIO& io) {
+    if constexpr (ROLE == R_RHS) return io.hout;
+    else if constexpr (ROLE == R_BS_3) return io.e;
+    else return nullptr;
+}
+template <int ROLE>
+__device__ __forceinline__ double* y_dst(const Phys& P, const StageIO& io) {
+    if constexpr (ROLE == R_RK4_4 || ROLE == R_EULER) return io.y0;
+    else if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1) return P.normalize ? io.y0 : nullptr;
+    else if constexpr (ROLE == R_RK12_1) return io.acc;  // the normalised y0 is written by store_out
+    else if constexpr (ROLE == R_BS_3 || ROLE == R_RK12_2) return io.acc;
+    else if constexpr (ROLE == R_BS_4) return P.normalize ? io.acc : nullptr;
+    else return nullptr;
+}
+template <int ROLE>
+__device__ __forceinline__ bool has_m(const StageIO& io) {
+    return !(ROLE == R_RHS || ROLE == R_BS_4) && io.mout;
+}
+
+// Scalar stores of one cell's outputs.
+template <int ROLE>
+__device__ __forceinline__ void store_out(const Phys& P, const StageIO& io, long I, long N, const Out& o,
+                                          const Pre& q) {
+    if (float* d = k_dst<ROLE>(io))
+        for (int c = 0; c < 3; c++) d[c * N + I] = o.k[c];
+    if (float* d = h_dst<ROLE>(io))
+        for (int c = 0; c < 3; c++) d[c * N + I] = o.h[c];
+    if (has_m<ROLE>(io))
+        for (int c = 0; c < 3; c++) io.mout[c * N + I] = o.m[c];
+    if (double* d = y_dst<ROLE>(P, io))
+        for (int c = 0; c < 3; c++) d[c * N + I] = o.y[c];
+    if constexpr (ROLE == R_RK12_1 || ROLE == R_EULER) {  // lazily normalised y0 written back
+        if (P.normalize && ROLE == R_RK12_1) {
+            double y[3] = {q.y[0], q.y[1], q.y[2]};
+            double l = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+            double sc = (l == 0.0) ? 0.0 : 1.0 / l;
+            for (int c = 0; c < 3; c++) io.y0[c * N + I] = y[c] * sc;
         }
     }
+}
+
+// Both cells (I, I+1) of a thread at once: 8-byte (FP32) and 16-byte (FP64)
+// stores.  I is even and the planes start 8-byte aligned.
+template <int ROLE>
+__device__ __forceinline__ void store_out2(const Phys& P, const StageIO& io, long I, long N, const Out& a,
+                                           const Out& b, const Pre& qa, const Pre& qb) {
+    if (float* d = k_dst<ROLE>(io))
+        for (int c = 0; c < 3; c++) *(float2*)(d + c * N + I) = make_float2(a.k[c], b.k[c]);
+    if (float* d = h_dst<ROLE>(io))
+        for (int c = 0; c < 3; c++) *(float2*)(d + c * N + I) = make_float2(a.h[c], b.h[c]);
+    if (has_m<ROLE>(io))
+        for (int c = 0; c < 3; c++) *(float2*)(io.mout + c * N + I) = make_float2(a.m[c], b.m[c]);
+    if (double* d = y_dst<ROLE>(P, io))
+        for (int c = 0; c < 3; c++) *(double2*)(d + c * N + I) = make_double2(a.y[c], b.y[c]);
+    if constexpr (ROLE == R_RK12_1) {
+        if (P.normalize) {
+            double ya[3] = {qa.y[0], qa.y[1], qa.y[2]}, yb[3] = {qb.y[0], qb.y[1], qb.y[2]};
+            double la = sqrt(ya[0] * ya[0] + ya[1] * ya[1] + ya[2] * ya[2]);
+            double lb = sqrt(yb[0] * yb[0] + yb[1] * yb[1] + yb[2] * yb[2]);
+            double sa = (la == 0.0) ? 0.0 : 1.0 / la, sb = (lb == 0.0) ? 0.0 : 1.0 / lb;
+            for (int c = 0; c < 3; c++) *(double2*)(io.y0 + c * N + I) = make_double2(ya[c] * sa, yb[c] * sb);
+        }
+    }
+}
+
+template <int ROLE>
+__device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, const StageIO& io, long I, long N,
+                                            const float m0[3], float H[3], const Pre& q, double& errloc) {
+    float kk[3];
+    field_torque(P, sp, io, I, m0, H, q, kk);
+    Out o;
+    stage_update<ROLE>(P, io, q, kk, H, o, errloc);
+    store_out<ROLE>(P, io, I, N, o, q);
 }
 
 // Error max of the block: wave shuffles, then the block's waves through LDS,
@@ -499,7 +637,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
         float m0[3] = {a[1][0], a[1][1], a[1][2]};
         float H[3] = {0.f, 0.f, 0.f};
         if (P.exchange && in)
-            exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, a[0], a[2], la[1], la[0], la[2], tile, ltile, H);
+            exchange_cell<TX>(P, m, N, i, j, k, jc, tx + 1, ty, m0, a[0], a[2], la[1], la[0], la[2], tile, ltile, H);
         __syncthreads();
         if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, q, errloc);
         for (int c = 0; c < 3; c++) {
@@ -555,8 +693,162 @@ __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
     load_tile(P, m, N, i, jc, kc, tx, ty, m0, l0, tile, ltile);
     __syncthreads();
     float H[3] = {0.f, 0.f, 0.f};
-    if (P.exchange && in) exchange_cell(P, m, N, i, j, k, jc, kc, tx, ty, m0, lo, hi, l0, llo, lhi, tile, ltile, H);
+    if (P.exchange && in) exchange_cell<TX>(P, m, N, i, j, k, jc, tx + 1, ty, m0, lo, hi, l0, llo, lhi, tile, ltile, H);
     if (in) finish_cell<ROLE>(P, sp, io, I, N, m0, H, q, errloc);
+    flush_err<ROLE>(errloc, io);
+}
+
+// ---- two cells per thread ------------------------------------------------
+// A 64 x 4 thread block covers 128 (k) x 4 (j) cells of one plane; each
+// thread owns cells (k, k+1), so every FP32 plane moves as 8-byte and the
+// FP64 state as 16-byte accesses (4-byte lanes cap a streaming kernel well
+// below the HBM rate).  Needs N2 even.
+constexpr int TX2 = 2 * TX;
+
+__device__ __forceinline__ float2 ld2(const float* p, long i) { return *(const float2*)(p + i); }
+__device__ __forceinline__ float2 ld2m(const float* p, long i) { return p ? ld2(p, i) : make_float2(1.f, 1.f); }
+
+template <int ROLE>
+__device__ __forceinline__ void preload2(const Phys& P, const StageIO& io, long I, long N, Pre q[2]) {
+    float2 v;
+    v = ld2m(P.msk, I);
+    q[0].ms = v.x, q[1].ms = v.y;
+    if (P.demag)
+        for (int c = 0; c < 3; c++) {
+            v = ld2(io.B, c * N + I);
+            q[0].B[c] = v.x, q[1].B[c] = v.y;
+        }
+    if (P.longfield) {
+        v = ld2m(P.J, I);
+        q[0].J = v.x, q[1].J = v.y;
+        v = ld2m(P.n, I);
+        q[0].n = v.x, q[1].n = v.y;
+        v = ld2m(P.Tc, I);
+        q[0].Tc = v.x, q[1].Tc = v.y;
+    }
+    v = ld2m(P.T, I);
+    q[0].T = v.x, q[1].T = v.y;
+    for (int c = 0; c < 3; c++) {
+        v = ld2m(P.alpha[c], I);
+        q[0].al[c] = v.x, q[1].al[c] = v.y;
+    }
+    const double* ys = nullptr;
+    if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER || ROLE == R_RK4_2 ||
+                  ROLE == R_RK4_3 || ROLE == R_BS_2 || ROLE == R_RK4_4 || ROLE == R_BS_3)
+        ys = io.y0;
+    if constexpr (ROLE == R_RK12_2) ys = io.acc;
+    if constexpr (ROLE == R_BS_4) ys = P.normalize ? io.acc : nullptr;
+    if (ys)
+        for (int c = 0; c < 3; c++) {
+            double2 w = *(const double2*)(ys + c * N + I);
+            q[0].y[c] = w.x, q[1].y[c] = w.y;
+        }
+    auto ldk = [&](const float* src, int which) {
+        for (int c = 0; c < 3; c++) {
+            float2 u = ld2(src, c * N + I);
+            float* d0 = which == 0 ? q[0].ka : (which == 1 ? q[0].kb : q[0].kc);
+            float* d1 = which == 0 ? q[1].ka : (which == 1 ? q[1].kb : q[1].kc);
+            d0[c] = u.x;
+            d1[c] = u.y;
+        }
+    };
+    if constexpr (ROLE == R_RK4_4 || ROLE == R_BS_3) {
+        ldk(io.k1, 0);
+        ldk(io.k2, 1);
+    }
+    if constexpr (ROLE == R_RK4_4) ldk(io.k3, 2);
+    if constexpr (ROLE == R_BS_4 || ROLE == R_RK12_2) ldk(io.e, 0);
+}
+
+template <int ROLE>
+__global__ void __launch_bounds__(TX* TY) k_stage_pair(Phys P, StageIO io0) {
+    __shared__ float tile[3][TY + 2][TX2 + 2];
+    __shared__ float ltile[TY + 2][TX2 + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
+    const int kb = blockIdx.x * TX2;  // first cell of the tile row
+    const int k = kb + 2 * tx, j = blockIdx.y * TY + ty, i = blockIdx.z % N0;
+    const int sol = blockIdx.z / N0;
+    if (solve_idle(io0, sol)) return;
+    const long N12 = (long)N1 * N2;
+    const long N = (long)N0 * N12;
+    const StageIO io = solve_io(io0, sol, N);
+    const SolveP sp = P.sp[sol];
+    const bool in = (k < N2) && (j < N1);  // N2 even: k < N2 implies k + 1 < N2
+    const float* m = io.m;
+    const int kc = min(k, N2 - 2), jc = min(j, N1 - 1);
+    const long I = i * N12 + (long)jc * N2 + kc;
+    float2 m0[3], lo[3], hi[3];
+    float2 l0 = make_float2(1.f, 1.f), llo = l0, lhi = l0;
+    for (int c = 0; c < 3; c++) m0[c] = ld2(m, c * N + I);
+    if (P.lex) l0 = ld2(P.lex, I);
+    Pre q[2];
+    preload2<ROLE>(P, io, I, N, q);
+    if (P.exchange) {
+        const long Ilo = wrapc(i - 1, N0, P.w0) * N12 + (long)jc * N2 + kc;
+        const long Ihi = wrapc(i + 1, N0, P.w0) * N12 + (long)jc * N2 + kc;
+        for (int c = 0; c < 3; c++) {
+            lo[c] = ld2(m, c * N + Ilo);
+            hi[c] = ld2(m, c * N + Ihi);
+        }
+        if (P.lex) {
+            llo = ld2(P.lex, Ilo);
+            lhi = ld2(P.lex, Ihi);
+        }
+    }
+    const int cc = 1 + 2 * tx;  // tile column of cell k
+    for (int c = 0; c < 3; c++) {
+        tile[c][ty + 1][cc] = m0[c].x;
+        tile[c][ty + 1][cc + 1] = m0[c].y;
+    }
+    if (P.lex) {
+        ltile[ty + 1][cc] = l0.x;
+        ltile[ty + 1][cc + 1] = l0.y;
+    }
+    if (P.exchange) {
+        if (ty == 0 || ty == TY - 1) {  // halo rows, two cells per thread
+            const int jn = (ty == 0) ? wrapc(blockIdx.y * TY - 1, N1, P.w1)
+                                     : wrapc(min(blockIdx.y * TY + TY, N1), N1, P.w1);
+            const int row = (ty == 0) ? 0 : TY + 1;
+            const long a2 = i * N12 + (long)jn * N2 + kc;
+            for (int c = 0; c < 3; c++) {
+                float2 v = ld2(m, c * N + a2);
+                tile[c][row][cc] = v.x;
+                tile[c][row][cc + 1] = v.y;
+            }
+            if (P.lex) {
+                float2 v = ld2(P.lex, a2);
+                ltile[row][cc] = v.x;
+                ltile[row][cc + 1] = v.y;
+            }
+        }
+        if (tx == 0 || tx == TX - 1) {  // halo columns kb-1 and kb+128
+            const int kn = (tx == 0) ? wrapc(kb - 1, N2, P.w2) : wrapc(min(kb + TX2, N2), N2, P.w2);
+            const int col = (tx == 0) ? 0 : TX2 + 1;
+            const long a2 = i * N12 + (long)jc * N2 + kn;
+            for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = m[c * N + a2];
+            if (P.lex) ltile[ty + 1][col] = P.lex[a2];
+        }
+    }
+    __syncthreads();
+    double errloc = 0.0;
+    Out o[2];
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const float mm[3] = {e ? m0[0].y : m0[0].x, e ? m0[1].y : m0[1].x, e ? m0[2].y : m0[2].x};
+        const float ml[3] = {e ? lo[0].y : lo[0].x, e ? lo[1].y : lo[1].x, e ? lo[2].y : lo[2].x};
+        const float mh[3] = {e ? hi[0].y : hi[0].x, e ? hi[1].y : hi[1].x, e ? hi[2].y : hi[2].x};
+        float H[3] = {0.f, 0.f, 0.f};
+        if (P.exchange && in)
+            exchange_cell<TX2>(P, m, N, i, j, k + e, jc, cc + e, ty, mm, ml, mh, e ? l0.y : l0.x,
+                               e ? llo.y : llo.x, e ? lhi.y : lhi.x, tile, ltile, H);
+        if (in) {
+            float kk[3];
+            field_torque(P, sp, io, I + e, mm, H, q[e], kk);
+            stage_update<ROLE>(P, io, q[e], kk, H, o[e], errloc);
+        }
+    }
+    if (in) store_out2<ROLE>(P, io, I, N, o[0], o[1], q[0], q[1]);
     flush_err<ROLE>(errloc, io);
 }
 
@@ -726,7 +1018,7 @@ struct hs_solver {
     unsigned long long* errmax = nullptr;
     std::vector<float*> masks;
     bool b0_valid = false;  // B0 = demag(mA) is current
-    bool stage_flat = true;  // flat (one plane per block) or marching stage kernel
+    int stage_kind = 1;      // 0 marching planes, 1 one plane per block, 2 two cells per thread
 };
 
 namespace {
@@ -748,7 +1040,10 @@ template <int ROLE>
 void launch_stage(hs_solver* s, const StageIO& io) {
     const Phys& P = s->phys;
     hs::prof_begin(HS_PROF_STAGE, s->st);
-    if (s->stage_flat) {
+    if (s->stage_kind == 2) {
+        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX2), hs::div_up(P.N1, TY), P.N0 * s->nsolve);
+        hipLaunchKernelGGL(k_stage_pair<ROLE>, grd, blk, 0, s->st, P, io);
+    } else if (s->stage_kind == 1) {
         dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), P.N0 * s->nsolve);
         hipLaunchKernelGGL(k_stage_flat<ROLE>, grd, blk, 0, s->st, P, io);
     } else {
@@ -774,13 +1069,15 @@ StageIO base_io(hs_solver* s, double h, uint32_t step, uint32_t attempt) {
     return io;
 }
 
-// Pick the faster stage-kernel variant (flat / marching) by timing one
-// representative role on the solver's own buffers.
+// Pick the fastest stage-kernel variant (marching / one plane per block /
+// two cells per thread) by timing one representative role on the solver's
+// own buffers.  HS_STAGE_KERNEL=march|flat|pair forces one.
 void tune_stage(hs_solver* s) {
+    const bool pair_ok = s->phys.N2 % 2 == 0;
     const char* env = getenv("HS_STAGE_KERNEL");
-    if (env && !strcmp(env, "flat")) { s->stage_flat = true; return; }
-    if (env && !strcmp(env, "march")) { s->stage_flat = false; return; }
-    if (s->phys.N0 < 2) { s->stage_flat = true; return; }
+    if (env && !strcmp(env, "flat")) { s->stage_kind = 1; return; }
+    if (env && !strcmp(env, "march")) { s->stage_kind = 0; return; }
+    if (env && !strcmp(env, "pair") && pair_ok) { s->stage_kind = 2; return; }
     StageIO io = base_io(s, 1e-15, 0, 0);
     io.m = s->mA;
     io.B = s->B0;
@@ -790,17 +1087,24 @@ void tune_stage(hs_solver* s) {
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    float t[2];
-    for (int v = 0; v < 2; v++) {
-        s->stage_flat = (v == 0);
+    float best = -1.f;
+    int pick = 1;
+    for (int v = 0; v < 3; v++) {
+        if ((v == 0 && s->phys.N0 < 2) || (v == 2 && !pair_ok)) continue;
+        s->stage_kind = v;
         launch_stage<R_RK4_2>(s, io);
         hipEventRecord(e0, s->st);
         for (int r = 0; r < 3; r++) launch_stage<R_RK4_2>(s, io);
         hipEventRecord(e1, s->st);
         hipEventSynchronize(e1);
-        hipEventElapsedTime(&t[v], e0, e1);
+        float t;
+        hipEventElapsedTime(&t, e0, e1);
+        if (best < 0.f || t < best) {
+            best = t;
+            pick = v;
+        }
     }
-    s->stage_flat = t[0] <= t[1];
+    s->stage_kind = pick;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
 }
@@ -1063,6 +1367,39 @@ int run_rk23_device(hs_solver* s, long nsteps, long step0, double* dtio, double 
     }
     return hs_last_error();
 }
+
+// Material constants of the uniform longitudinal field (see LfConst).
+LfConst lf_constants(const hs_llb_params* p) {
+    LfConst C{};
+    C.uniform = p->longfield && !p->J && !p->n && !p->Tc && !p->T;
+    const double S = p->JMul;
+    if (S > HS_INFSPIN) {
+        C.langevin = 1;
+        C.lim = HS_LINRANGE;
+        C.a1 = 1.0 / 3.0, C.a3 = 1.0 / 45.0, C.a5 = 2.0 / 945.0, C.a7 = 1.0 / 4725.0;
+        C.d0 = 1.0 / 3.0, C.d2 = 1.0 / 15.0, C.d4 = 2.0 / 189.0, C.d6 = 1.0 / 675.0;
+        return C;
+    }
+    const double l = 1.0 / (2.0 * S), g = (2.0 * S + 1.0) * l;
+    const double l2 = l * l, g2 = g * g, l4 = l2 * l2, g4 = g2 * g2, l6 = l4 * l2, g6 = g4 * g2;
+    const double l8 = l4 * l4, g8 = g4 * g4;
+    C.l = l, C.g = g;
+    C.lim = fmin(HS_LINRANGE / g, HS_LINRANGE / l);
+    C.a1 = (g2 - l2) / 3.0, C.a3 = (g4 - l4) / 45.0, C.a5 = (g6 - l6) * 2.0 / 945.0, C.a7 = (g8 - l8) / 4725.0;
+    C.d0 = (g2 - l2) / 3.0, C.d2 = (g4 - l4) / 15.0, C.d4 = (g6 - l6) * 2.0 / 189.0, C.d6 = (g8 - l8) / 675.0;
+    return C;
+}
+
+// Per-solve constants of a solve at temperature multiplier T.
+void solve_constants(const hs_llb_params* p, double T, SolveP& q) {
+    const double V = p->cell[0] * p->cell[1] * p->cell[2];
+    const double S = p->JMul;
+    const double preS = (S > HS_INFSPIN) ? 1.0 : S / (S + 1.0);
+    q.TMul = (float)T;
+    q.thC = (HS_KB * 2.0 * T) / (HS_MU0 * V * p->gammaLL * p->msat0T0Mul);
+    q.lf_b = 3.0 * preS * p->TcMul / T;
+    q.lf_c1 = p->nMul * (HS_KB / HS_MU0) * T / p->msat0T0Mul;
+}
 }  // namespace
 
 // A batch of `nsolve` solves on one mesh: per-solve TMul, seed, B_ext.
@@ -1108,12 +1445,10 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * nsolve * kErrPerSolve, s->st);
     // per-solve parameters; thermal K = thC / max(cutoff_dt, dt), mod_brown.go:125-145
     std::vector<SolveP> hsp(nsolve);
-    const double V = p->cell[0] * p->cell[1] * p->cell[2];
     for (int q = 0; q < nsolve; q++) {
         double T = TMul ? TMul[q] : p->TMul;
-        hsp[q].TMul = (float)T;
+        solve_constants(p, T, hsp[q]);
         for (int c = 0; c < 3; c++) hsp[q].bext[c] = (float)(1.0 / HS_MU0 * (Bext ? Bext[3 * q + c] : p->Bext[c]));
-        hsp[q].thC = (HS_KB * 2.0 * T) / (HS_MU0 * V * p->gammaLL * p->msat0T0Mul);
         hsp[q].seed = seed ? seed[q] : p->seed;
         hsp[q].stream = p->stream + (uint32_t)(seed ? 0 : q);
         hsp[q].pad = 0;
@@ -1162,6 +1497,7 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     P.Tc = upload_mask(s, p->Tc);
     P.T = upload_mask(s, p->T);
     P.gamma = (float)p->gammaLL;
+    P.lf = lf_constants(p);
     for (int c = 0; c < 3; c++) {
         P.aMul[c] = (float)p->alphaMul[c];
         P.alpha[c] = upload_mask(s, p->alpha[c]);
@@ -1207,7 +1543,7 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
 
 extern "C" CUstream hs_solver_stream(hs_solver* s) { return s ? (CUstream)s->st : nullptr; }
 extern "C" hs_demag* hs_solver_demag(hs_solver* s) { return s ? s->demag : nullptr; }
-extern "C" int hs_solver_stage_variant(hs_solver* s) { return s ? (s->stage_flat ? 1 : 0) : -1; }
+extern "C" int hs_solver_stage_variant(hs_solver* s) { return s ? s->stage_kind : -1; }
 extern "C" int hs_solver_nsolve(hs_solver* s) { return s ? s->nsolve : -1; }
 
 static bool whole_batch_ok(hs_solver* s, const char* who) {
@@ -1413,10 +1749,8 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
     // per-solve parameters (thermal prefactor as hs_batch_create)
     const hs_llb_params* p = &s->p;
     SolveP q = s->sp_host[slot];
-    const double V = p->cell[0] * p->cell[1] * p->cell[2];
-    q.TMul = (float)TMul;
+    solve_constants(p, TMul, q);
     for (int k = 0; k < 3; k++) q.bext[k] = (float)(1.0 / HS_MU0 * (Bext ? Bext[k] : p->Bext[k]));
-    q.thC = (HS_KB * 2.0 * TMul) / (HS_MU0 * V * p->gammaLL * p->msat0T0Mul);
     q.seed = seed;
     q.stream = stream;
     s->sp_host[slot] = q;

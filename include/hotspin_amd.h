@@ -161,8 +161,9 @@ HS_API int hs_solver_rhs(hs_solver* s, double* k_host, double* heff_host, uint32
 HS_API CUstream hs_solver_stream(hs_solver* s);
 /* the solver's demag plan (NULL when demag is off) */
 HS_API hs_demag* hs_solver_demag(hs_solver* s);
-/* stage-kernel variant picked by timing at creation: 1 = one plane per
- * block, 0 = marching planes (env HS_STAGE_KERNEL=flat|march overrides) */
+/* stage-kernel variant picked by timing at creation: 0 = marching planes,
+ * 1 = one plane per block, 2 = two cells per thread (even N2 only); env
+ * HS_STAGE_KERNEL=march|flat|pair overrides */
 HS_API int hs_solver_stage_variant(hs_solver* s);
 
 /* ------------------------------------------------ slots (continuous batching)

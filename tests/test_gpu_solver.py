@@ -143,3 +143,29 @@ def test_llb_rk23_adaptive_matches_oracle(masks):
         assert stg.rejected == tries - 1, (i, stg.rejected, tries)
         assert abs(dtg / dt - 1) <= 2e-2, (i, dtg, dt)
         assert np.abs(g.get_m() - mo).max() <= 1e-5, i
+
+
+@pytest.mark.parametrize("variant", ["march", "flat", "pair"])
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (1, 1, 1)])
+def test_stage_kernel_variants_match_oracle(variant, pbc, monkeypatch):
+    """Each stage-kernel variant (marching planes, one plane per block, two
+    cells per thread) against the oracle: RHS and 20 thermal RK4 steps, on a
+    mesh whose rows are not a multiple of the tile widths."""
+    monkeypatch.setenv("HS_STAGE_KERNEL", variant)
+    N, cell = (5, 12, 160), (2e-9, 2e-9, 2e-9)
+    kw, m = llb_problem(N, cell, masks=True, pbc=pbc)
+    o = po.LLB(N, cell, pbc, **kw)
+    mo, ko = o.rhs(m, step=3, attempt=0, dt=1e-15)
+    Ho = o.heff()
+    g = Solver(N, cell, pbc, **kw)
+    from hotspin_amd._lib import lib
+    assert lib().hs_solver_stage_variant(g._h) == {"march": 0, "flat": 1, "pair": 2}[variant]
+    g.set_m(m)
+    kg, Hg = g.rhs(step=3, attempt=0, dt=1e-15)
+    assert relerr(Hg, Ho) <= 2e-5
+    assert relerr(kg, ko) <= 2e-5
+    mo = m.copy()
+    for i in range(20):
+        o.step("rk4", mo, 2e-15, i)
+    g.run("rk4", 20, 2e-15)
+    assert np.abs(g.get_m() - mo).max() <= 1e-5
