@@ -20,6 +20,10 @@
 // 1/(P0 P1 P2) (maxwell_convolution.go:135-223,350-388).
 #include <math.h>
 #include <stdio.h>
+
+#include <array>
+#include <map>
+#include <mutex>
 #include <string.h>
 
 #include <map>
@@ -381,6 +385,39 @@ int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, fl
 
 // Time each registered FP32 variant of every pass on the plan's own buffers
 // and keep the fastest (the tile autotuner).
+// The choice is cached per mesh shape for the life of the process, so every
+// plan of one shape runs the same variants: the variants differ in operation
+// order, and solves of one shape must give identical results whichever plan
+// (batch, slot batch, single solve) steps them.
+struct TuneKey {
+    int N[3], P[3];
+    bool operator<(const TuneKey& o) const {
+        for (int i = 0; i < 3; i++) {
+            if (N[i] != o.N[i]) return N[i] < o.N[i];
+            if (P[i] != o.P[i]) return P[i] < o.P[i];
+        }
+        return false;
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::array<int, 3>>& tune_cache() {
+    static std::map<TuneKey, std::array<int, 3>> c;
+    return c;
+}
+
+void apply_force(hs_demag* d, hsdm::Variants& vr, hsdm::Variants& vc, hsdm::Variants& vz) {
+    // HS_DEMAG_FORCE="r,c,z": variant indices to use instead of the timed
+    // choice (-1 keeps it), for tests that cover every registered variant
+    if (const char* f = getenv("HS_DEMAG_FORCE")) {
+        int ir = -1, ic = -1, iz = -1;
+        if (sscanf(f, "%d,%d,%d", &ir, &ic, &iz) == 3) {
+            if (ir >= 0 && ir < (int)vr.rows.size()) d->row32 = vr.rows[ir];
+            if (ic >= 0 && ic < (int)vc.cols.size()) d->col32 = vc.cols[ic];
+            if (iz >= 0 && iz < (int)vz.z.size()) d->z32 = vz.z[iz];
+        }
+    }
+}
+
 void tune32(hs_demag* d, hipStream_t st) {
     hsdm::Variants& vr = hsdm::variants(d->M);
     hsdm::Variants& vc = hsdm::variants(d->P[1]);
@@ -389,6 +426,19 @@ void tune32(hs_demag* d, hipStream_t st) {
     d->col32 = vc.cols.empty() ? hsdm::ColV{} : vc.cols[0];
     d->z32 = vz.z.empty() ? hsdm::ZV{} : vz.z[0];
     if (!d->row32.r2c || !d->col32.fwd || !d->z32.fn) return;
+    const TuneKey key{{d->N[0], d->N[1], d->N[2]}, {d->P[0], d->P[1], d->P[2]}};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end()) {
+            d->row32 = vr.rows[it->second[0]];
+            d->col32 = vc.cols[it->second[1]];
+            d->z32 = vz.z[it->second[2]];
+            apply_force(d, vr, vc, vz);
+            d->tuned = true;
+            return;
+        }
+    }
     const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1];
     const int H2 = d->H2, H2p = d->H2p;
     const long nrows = 3L * N0 * N1;
@@ -416,25 +466,34 @@ void tune32(hs_demag* d, hipStream_t st) {
     };
     Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
     Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
+    std::array<int, 3> pick{0, 0, 0};
     float best = 1e30f;
-    for (auto& v : vr.rows) {
+    for (size_t i = 0; i < vr.rows.size(); i++) {
+        auto& v = vr.rows[i];
         const Live all{nullptr, 1};
         float t = timeit([&] { v.r2c(m, nullptr, (long)N0 * N1, d->X1f, nrows, N2, H2p, d->twMf, d->twP2f, st, all); }) +
                   timeit([&] { v.c2r(d->X1f, B, nrows, N2, H2p, d->twMf, d->twP2f, st, all); });
-        if (t < best) { best = t; d->row32 = v; }
+        if (t < best) { best = t; d->row32 = v; pick[0] = (int)i; }
     }
     best = 1e30f;
-    for (auto& v : vc.cols) {
+    for (size_t i = 0; i < vc.cols.size(); i++) {
+        auto& v = vc.cols[i];
         const Live all{nullptr, 1};
         float t = timeit([&] { v.fwd(d->X1f, d->X2f, g1, g2, 3 * N0, N1, P1, H2, d->tw1f, st, all); }) +
                   timeit([&] { v.inv(d->X2f, d->X1f, g2, g1, 3 * N0, P1, N1, H2, d->tw1f, st, all); });
-        if (t < best) { best = t; d->col32 = v; }
+        if (t < best) { best = t; d->col32 = v; pick[1] = (int)i; }
     }
     best = 1e30f;
-    for (auto& v : vz.z) {
+    for (size_t i = 0; i < vz.z.size(); i++) {
+        auto& v = vz.z[i];
         float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], d->KP, 1.0f, d->tw0f, st, 1, Live{nullptr, 1}); });
-        if (t < best) { best = t; d->z32 = v; }
+        if (t < best) { best = t; d->z32 = v; pick[2] = (int)i; }
     }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_cache()[key] = pick;
+    }
+    apply_force(d, vr, vc, vz);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     hipStreamSynchronize(st);
@@ -640,6 +699,14 @@ extern "C" int hs_demag_kernel_f64(const hs_demag* d, double* host_out) {
 }
 
 extern "C" double hs_demag_max_imag_ratio(const hs_demag* d) { return d ? d->max_imag_ratio : -1; }
+
+extern "C" int hs_demag_variant_count(const hs_demag* d, int* out3) {
+    if (!d || !out3) return HS_EINVAL;
+    out3[0] = (int)hsdm::variants(d->M).rows.size();
+    out3[1] = (int)hsdm::variants(d->P[1]).cols.size();
+    out3[2] = (int)hsdm::variants(d->P[0]).z.size();
+    return 0;
+}
 
 extern "C" int hs_demag_tiles(const hs_demag* d, int* out6) {
     if (!d || !out6) return HS_EINVAL;

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "demag_kernels.hpp"
+#include "fft2.hpp"
 
 namespace hsdm {
 using r2c32_t = void (*)(const float*, const float*, long, cpx<float>*, long, int, int, const cpx<float>*,
@@ -67,6 +68,14 @@ void launch_z32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2
                        srcMul, tw, lv);
 }
 
+template <int L, int S, int W>
+void launch_cols2_32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int npencil, int nin, int nkeep,
+                     int ncol, const cpx<float>* tw, hipStream_t st, Live lv) {
+    dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
+    hipLaunchKernelGGL((k_cols2<float, L, S, W>), g, dim3(W * split2(L).B), 0, st, in, out, gi, go, nin, nkeep, ncol,
+                       tw, lv);
+}
+
 template <int M, int W, int E>
 void add_row(Variants& v) {
     constexpr int NT = clamp_nt((long)M * W / E);
@@ -77,6 +86,30 @@ void add_col(Variants& v) {
     constexpr int NT = clamp_nt((long)L * W / E);
     if constexpr ((long)L * W * 8 <= kLdsMax) v.cols.push_back({W, NT, &launch_cols32<L, -1, W, NT>, &launch_cols32<L, 1, W, NT>});
 }
+template <int L, int W>
+void launch_z2_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
+                  const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
+    hipLaunchKernelGGL((k_zconv2<float, float, L, W>), g, dim3(6 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
+                       srcMul, tw, lv);
+}
+// register-FFT axis-0 pass (fft2.hpp): one thread per pencil, L <= 64
+template <int L, int W>
+void add_z2(Variants& v) {
+    if constexpr (reg_dft_ok(L) && L > 1 && L <= 64 && (long)L * 6 * W * 8 <= kLdsMax)
+        v.z.push_back({W, 6 * W, &launch_z2_32<L, W>});
+}
+
+// two-level register FFT columns (fft2.hpp): W columns, W * B threads
+template <int L, int W>
+void add_col2(Variants& v) {
+    if constexpr (two_level_ok(L) && L > 1) {
+        constexpr int NT = W * split2(L).B;
+        if constexpr ((long)(L * W + L) * 8 <= kLdsMax && NT >= 64 && NT <= 1024)
+            v.cols.push_back({W, NT, &launch_cols2_32<L, -1, W>, &launch_cols2_32<L, 1, W>});
+    }
+}
+
 template <int L, int W, int E>
 void add_z(Variants& v) {  // a block holds a pencil pair: 6W pencils
     constexpr int NT = clamp_nt((long)L * 6 * W / E);
@@ -97,11 +130,16 @@ void register_size() {
     add_col<L, 16, 8>(v);
     add_col<L, 4, 8>(v);
     add_col<L, 4, 4>(v);
+    add_col2<L, 4>(v);
+    add_col2<L, 8>(v);
+    add_col2<L, 16>(v);
     add_z<L, 8, 12>(v);
     add_z<L, 16, 12>(v);
     add_z<L, 32, 12>(v);
     add_z<L, 16, 24>(v);
     add_z<L, 32, 24>(v);
     add_z<L, 8, 6>(v);
+    add_z2<L, 16>(v);
+    add_z2<L, 32>(v);
 }
 }  // namespace hsdm

@@ -18,30 +18,30 @@ struct cpx {
 };
 
 template <typename T>
-__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
-__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>
-__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+__host__ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+__host__ __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
 // (s i) * a  for s = +-1
 template <typename T, int S>
-__device__ __forceinline__ cpx<T> mul_si(cpx<T> a) {
+__host__ __device__ __forceinline__ cpx<T> mul_si(cpx<T> a) {
     return S > 0 ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
 }
 
 // ---- radix-R DFTs in registers; S = -1 forward, +1 backward --------------
 template <typename T, int S>
-__device__ __forceinline__ void dft2(cpx<T>* v) {
+__host__ __device__ __forceinline__ void dft2(cpx<T>* v) {
     cpx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
 }
 template <typename T, int S>
-__device__ __forceinline__ void dft4(cpx<T>* v) {
+__host__ __device__ __forceinline__ void dft4(cpx<T>* v) {
     cpx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
     cpx<T> t2 = cadd(v[1], v[3]), t3 = mul_si<T, S>(csub(v[1], v[3]));
     v[0] = cadd(t0, t2);
@@ -50,7 +50,7 @@ __device__ __forceinline__ void dft4(cpx<T>* v) {
     v[3] = csub(t1, t3);
 }
 template <typename T, int S>
-__device__ __forceinline__ void dft8(cpx<T>* v) {
+__host__ __device__ __forceinline__ void dft8(cpx<T>* v) {
     cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
     cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
     dft4<T, S>(e);
@@ -70,7 +70,7 @@ __device__ __forceinline__ void dft8(cpx<T>* v) {
     v[7] = csub(e[3], o3);
 }
 template <typename T, int S>
-__device__ __forceinline__ void dft3(cpx<T>* v) {
+__host__ __device__ __forceinline__ void dft3(cpx<T>* v) {
     const T c = T(-0.5), sn = T(S) * T(0.86602540378443864676);
     cpx<T> t = cadd(v[1], v[2]);
     cpx<T> d = csub(v[1], v[2]);
@@ -81,7 +81,7 @@ __device__ __forceinline__ void dft3(cpx<T>* v) {
     v[2] = csub(m1, m2);
 }
 template <typename T, int S>
-__device__ __forceinline__ void dft5(cpx<T>* v) {
+__host__ __device__ __forceinline__ void dft5(cpx<T>* v) {
     const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
     const T s1 = T(S) * T(0.95105651629515357212), s2 = T(S) * T(0.58778525229247312917);
     cpx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
@@ -99,7 +99,7 @@ __device__ __forceinline__ void dft5(cpx<T>* v) {
     v[3] = csub(b2, d2);
 }
 template <typename T, int R, int S>
-__device__ __forceinline__ void dftR(cpx<T>* v) {
+__host__ __device__ __forceinline__ void dftR(cpx<T>* v) {
     if constexpr (R == 2) dft2<T, S>(v);
     else if constexpr (R == 3) dft3<T, S>(v);
     else if constexpr (R == 4) dft4<T, S>(v);

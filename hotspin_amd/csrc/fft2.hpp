@@ -1,0 +1,451 @@
+// fft2.hpp -- register-blocked two-level FFT passes.
+//
+// A length L = A * B transform done as A-point DFTs in registers, a twiddle,
+// one exchange through LDS, and B-point DFTs in registers:
+//     X[k1 + A k2] = sum_n2 w_B^(n2 k2) w_L^(n2 k1) sum_n1 x[B n1 + n2] w_A^(n1 k1)
+// so each element is written to and read from LDS once, instead of once per
+// radix stage.  The A- and B-point DFTs are mixed-radix (2,3,4,5,8) in
+// registers with compile-time twiddles.
+#pragma once
+
+#include <type_traits>
+
+#include "demag_kernels.hpp"
+
+// ---- compile-time roots of unity -----------------------------------------
+constexpr double kPi2 = 6.283185307179586476925286766559;
+constexpr double kPi = 3.141592653589793238462643383279;
+constexpr double series_sin(double x) {  // |x| <= pi/4
+    double x2 = x * x, term = x, sum = x;
+    for (int n = 1; n < 14; n++) {
+        term *= -x2 / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double series_cos(double x) {
+    double x2 = x * x, term = 1.0, sum = 1.0;
+    for (int n = 1; n < 14; n++) {
+        term *= -x2 / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+// cos and sin of 2 pi m / R by octant symmetry on exact fractions of the
+// circle, the series evaluated on [0, pi/4] only
+struct CS {
+    double c, s;
+};
+constexpr CS cis_quadrant(long m, long R) {  // angle 2 pi m / R in [0, pi/2]
+    if (8 * m > R) {
+        const double psi = kPi2 * (double)(R - 4 * m) / (4.0 * (double)R);
+        return CS{series_sin(psi), series_cos(psi)};
+    }
+    const double t = kPi2 * (double)m / (double)R;
+    return CS{series_cos(t), series_sin(t)};
+}
+constexpr CS cis2pi(long m, long R) {
+    m %= R;
+    if (m < 0) m += R;
+    bool neg_s = false;
+    if (2 * m > R) {  // theta > pi: reflect
+        m = R - m;
+        neg_s = true;
+    }
+    CS r{0.0, 0.0};
+    if (4 * m > R) {  // theta > pi/2: pi - theta = 2 pi (R - 2m) / (2R)
+        const CS q = cis_quadrant(R - 2 * m, 2 * R);
+        r = CS{-q.c, q.s};
+    } else {
+        r = cis_quadrant(m, R);
+    }
+    if (neg_s) r.s = -r.s;
+    return r;
+}
+
+template <int R>
+struct Roots {
+    double c[R > 0 ? R : 1], s[R > 0 ? R : 1];
+    constexpr Roots() : c(), s() {
+        for (int m = 0; m < R; m++) {
+            CS v = cis2pi(m, R);
+            c[m] = v.c;
+            s[m] = v.s;
+        }
+    }
+};
+template <int R>
+struct RootTable {
+    static constexpr Roots<R> v{};
+};
+
+// w_R^m = exp(S 2 pi i m / R)
+template <typename T, int R, int S>
+__host__ __device__ __forceinline__ cpx<T> root(int m) {
+    return {T(RootTable<R>::v.c[m]), T(S) * T(RootTable<R>::v.s[m])};
+}
+
+constexpr int first_factor(int R) {
+    const int rs[5] = {8, 4, 2, 3, 5};
+    for (int r : rs)
+        if (R % r == 0 && R != r) return r;
+    return R;
+}
+constexpr bool reg_dft_ok(int R) {
+    int m = R;
+    while (m % 2 == 0) m /= 2;
+    while (m % 3 == 0) m /= 3;
+    while (m % 5 == 0) m /= 5;
+    return m == 1 && R >= 1 && R <= 64;
+}
+
+// In-place DFT of v[0..R) in registers, natural order (S = -1 forward).
+template <typename T, int R, int S>
+__host__ __device__ __forceinline__ void dft_reg(cpx<T>* v) {
+    static_assert(reg_dft_ok(R), "register DFT length must be 2^a 3^b 5^c <= 64");
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) {
+        dftR<T, R, S>(v);
+    } else {
+        constexpr int R1 = first_factor(R), R2 = R / R1;
+        cpx<T> u[R];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) {
+            cpx<T> g[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; n1++) g[n1] = v[R2 * n1 + n2];
+            dft_reg<T, R1, S>(g);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; k1++) u[n2 * R1 + k1] = (n2 * k1) ? cmul(g[k1], root<T, R, S>(n2 * k1)) : g[k1];
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) {
+            cpx<T> g[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; n2++) g[n2] = u[n2 * R1 + k1];
+            dft_reg<T, R2, S>(g);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; k2++) v[k1 + R1 * k2] = g[k2];
+        }
+    }
+}
+
+// Split L = A * B with A, B register-DFT lengths, A <= B <= 32 preferred.
+struct Split2 {
+    int A, B;
+};
+constexpr Split2 split2(int L) {
+    Split2 best{0, 0};
+    for (int a = 1; a <= 64; a++) {
+        if (L % a) continue;
+        int b = L / a;
+        if (!reg_dft_ok(a) || !reg_dft_ok(b) || a > b || b > 32) continue;
+        best = Split2{a, b};  // keeps the largest a <= b
+    }
+    return best;
+}
+constexpr bool two_level_ok(int L) { return split2(L).A > 0; }
+
+// ---- column pass (F2 / I2) ------------------------------------------------
+// Pencil a, columns col0..col0+W; input rows t < nin (zero beyond), output
+// rows t < nkeep; in and out may alias only if they are the same buffer with
+// nin == nkeep == L (each block reads its tile completely before writing).
+// Threads: W * B in phase 1 (one A-point DFT over n1 for each (n2, column)),
+// W * A in phase 2 (one B-point DFT over n2 for each (k1, column)).
+template <typename T, int L, int S, int W, int NT = W * split2(L).B>
+__global__ void __launch_bounds__(NT) k_cols2(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
+                                              int ncol, const cpx<T>* __restrict__ tw, Live lv) {
+    constexpr int A = split2(L).A, B = split2(L).B;
+    static_assert(A * B == L && NT == W * B && A <= B, "two-level split");
+    __shared__ cpx<T> lds[L * W];
+    __shared__ cpx<T> tws[L];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int a = lb / gridDim.x;
+    if (all_dead(lv, a, a)) return;
+    const int col0 = (lb % gridDim.x) * W;
+    const long bi = geo_base(gi, a) + col0;
+    const long bo = geo_base(go, a) + col0;
+    for (int e = tid; e < L; e += NT) tws[e] = tw[e];
+    // phase 1: thread (n2, p) gathers x[B n1 + n2], n1 < A
+    const int p = tid % W, n2 = tid / W;
+    const bool pin = col0 + p < ncol;
+    cpx<T> v[B];  // A used in phase 1, B in phase 2
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) {
+        const int t = B * n1 + n2;
+        cpx<T> x = {T(0), T(0)};
+        if (pin && t < nin) x = in[bi + (long)t * gi.ts + p];
+        v[n1] = x;
+    }
+    __syncthreads();  // twiddles visible
+    dft_reg<T, A, S>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) {
+        cpx<T> w = tws[n2 * k1];
+        if (S > 0) w.y = -w.y;
+        lds[(k1 * B + n2) * W + p] = (n2 * k1) ? cmul(v[k1], w) : v[k1];
+    }
+    __syncthreads();
+    // phase 2: thread (k1, p) gathers Y[k1][n2], n2 < B
+    if (tid < W * A) {
+        const int k1 = tid / W;
+#pragma unroll
+        for (int m = 0; m < B; m++) v[m] = lds[(k1 * B + m) * W + p];
+        dft_reg<T, B, S>(v);
+        if (pin) {
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) {
+                const int t = k1 + A * k2;
+                if (t < nkeep) out[bo + (long)t * go.ts + p] = v[k2];
+            }
+        }
+    }
+}
+
+// ---- row passes (F1 R2C / I1 C2R) -----------------------------------------
+// M = P2/2-point complex FFT per row as in demag_kernels.hpp's k_r2c_rows /
+// k_c2r_rows, done two-level.  W rows per block; phase-1 threads take
+// (n2, row) with n2 fastest so a wave reads consecutive pairs of a row as
+// 8-byte words.  Exchange layout [row][k1][n2] with row pitch B + 1 (odd
+// stride in 8-byte words: no bank conflicts in either phase).  Rows of even
+// length n2 move as aligned pairs (one 8-byte word per complex element);
+// odd rows fall back to single words.
+template <int M, int W>
+constexpr int rows2_lds() {
+    constexpr int A = split2(M).A, B = split2(M).B;
+    return (W * A * (B + 1) > W * (M + 1)) ? W * A * (B + 1) : W * (M + 1);
+}
+
+template <typename T, int M, int W, int NT = W * split2(M).B>
+__global__ void __launch_bounds__(NT) k_r2c_rows2(const T* __restrict__ x, const T* __restrict__ msk, long mrows,
+                                                  cpx<T>* __restrict__ out, long nrows, int n2, int H2p,
+                                                  const cpx<T>* __restrict__ twM, const cpx<T>* __restrict__ twP,
+                                                  Live lv) {
+    constexpr int A = split2(M).A, B = split2(M).B, BP = B + 1;
+    static_assert(A * B == M && NT == W * B && A <= B, "two-level split");
+    __shared__ cpx<T> lds[rows2_lds<M, W>()];
+    __shared__ cpx<T> tws[M];
+    const int tid = threadIdx.x;
+    const long row0 = (long)blockIdx.x * W;
+    if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
+    for (int e = tid; e < M; e += NT) tws[e] = twM[e];
+    {  // phase 1: thread (n2, row p) gathers z[B n1 + n2] = x[2n] + i x[2n+1]
+        const int nb = tid % B, p = tid / B;
+        const long r = row0 + p;
+        const bool rin = r < nrows;
+        const T* xr = x + r * n2;
+        const T* mr = msk ? msk + (r % mrows) * n2 : nullptr;
+        cpx<T> v[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) {
+            const int n = B * n1 + nb;
+            cpx<T> z = {T(0), T(0)};
+            if (rin && (n2 & 1) == 0 && 2 * n < n2) {
+                using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+                const V2 q = *(const V2*)(xr + 2 * n);
+                z = {T(q.x), T(q.y)};
+                if (mr) {
+                    const V2 w = *(const V2*)(mr + 2 * n);
+                    z.x *= T(w.x);
+                    z.y *= T(w.y);
+                }
+            } else if (rin && (n2 & 1)) {
+                if (2 * n < n2) z.x = xr[2 * n] * (mr ? mr[2 * n] : T(1));
+                if (2 * n + 1 < n2) z.y = xr[2 * n + 1] * (mr ? mr[2 * n + 1] : T(1));
+            }
+            v[n1] = z;
+        }
+        __syncthreads();  // twiddles visible
+        dft_reg<T, A, -1>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) lds[(p * A + k1) * BP + nb] = (nb * k1) ? cmul(v[k1], tws[nb * k1]) : v[k1];
+    }
+    __syncthreads();
+    cpx<T> u[B];
+    const int k1 = tid % A, p2 = tid / A;
+    if (tid < W * A) {  // phase 2: thread (k1, row) gathers Y[k1][n2]
+#pragma unroll
+        for (int m = 0; m < B; m++) u[m] = lds[(p2 * A + k1) * BP + m];
+        dft_reg<T, B, -1>(u);
+    }
+    __syncthreads();
+    if (tid < W * A) {
+#pragma unroll
+        for (int k2 = 0; k2 < B; k2++) lds[p2 * M + k1 + A * k2] = u[k2];
+    }
+    __syncthreads();
+    // split: X[k] = E + w^k O, k = 0..M
+    for (int e = tid; e < (M + 1) * W; e += NT) {
+        const int p = e / (M + 1), k = e % (M + 1);
+        const long r = row0 + p;
+        if (r >= nrows) continue;
+        cpx<T> zk = lds[p * M + (k % M)];
+        cpx<T> zm = lds[p * M + ((M - k) % M)];
+        cpx<T> E = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+        cpx<T> O = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
+        out[r * H2p + k] = cadd(E, cmul(twP[k], O));
+    }
+}
+
+template <typename T, int M, int W, int NT = W * split2(M).B>
+__global__ void __launch_bounds__(NT) k_c2r_rows2(const cpx<T>* __restrict__ in, T* __restrict__ x, long nrows,
+                                                  int n2, int H2p, const cpx<T>* __restrict__ twM,
+                                                  const cpx<T>* __restrict__ twP, Live lv) {
+    constexpr int A = split2(M).A, B = split2(M).B, BP = B + 1;
+    static_assert(A * B == M && NT == W * B && A <= B, "two-level split");
+    __shared__ cpx<T> lds[rows2_lds<M, W>()];
+    __shared__ cpx<T> tws[M];
+    __shared__ cpx<T> twp[M];
+    const int tid = threadIdx.x;
+    const long row0 = (long)blockIdx.x * W;
+    if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
+    for (int e = tid; e < M; e += NT) {
+        tws[e] = twM[e];
+        twp[e] = twP[e];
+    }
+    for (int e = tid; e < (M + 1) * W; e += NT) {
+        const int p = e / (M + 1), k = e % (M + 1);
+        const long r = row0 + p;
+        cpx<T> v = {T(0), T(0)};
+        if (r < nrows) v = in[r * H2p + k];
+        lds[p * (M + 1) + k] = v;
+    }
+    __syncthreads();
+    const int nb = tid % B, p = tid / B;
+    cpx<T> v[A];
+    {  // phase 1: Z[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]), k = B n1 + n2
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) {
+            const int k = B * n1 + nb;
+            cpx<T> a = lds[p * (M + 1) + k];
+            cpx<T> b = lds[p * (M + 1) + (M - k)];
+            cpx<T> s = {a.x + b.x, a.y - b.y};
+            cpx<T> d = {a.x - b.x, a.y + b.y};
+            cpx<T> w = twp[k];
+            w.y = -w.y;
+            cpx<T> wd = cmul(w, d);
+            v[n1] = {s.x - wd.y, s.y + wd.x};
+        }
+    }
+    __syncthreads();  // X no longer read: the buffer takes the exchange
+    dft_reg<T, A, +1>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) {
+        cpx<T> w = tws[nb * k1];
+        w.y = -w.y;
+        lds[(p * A + k1) * BP + nb] = (nb * k1) ? cmul(v[k1], w) : v[k1];
+    }
+    __syncthreads();
+    if (tid < W * A) {  // phase 2 and the real output pairs x[2n], x[2n+1], n = k1 + A k2
+        const int k1 = tid % A, p2 = tid / A;
+        const long r = row0 + p2;
+        cpx<T> u[B];
+#pragma unroll
+        for (int m = 0; m < B; m++) u[m] = lds[(p2 * A + k1) * BP + m];
+        dft_reg<T, B, +1>(u);
+        if (r < nrows) {
+            T* xr = x + r * n2;
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) {
+                const int n = k1 + A * k2;
+                if ((n2 & 1) == 0) {
+                    if (2 * n < n2) {
+                        using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+                        V2 q;
+                        q.x = u[k2].x;
+                        q.y = u[k2].y;
+                        *(V2*)(xr + 2 * n) = q;
+                    }
+                } else {
+                    if (2 * n < n2) xr[2 * n] = u[k2].x;
+                    if (2 * n + 1 < n2) xr[2 * n + 1] = u[k2].y;
+                }
+            }
+        }
+    }
+}
+
+// ---- axis-0 convolution pass (F3K) with register FFTs ---------------------
+// Same contract as demag_kernels.hpp's k_zconv (pencil j' and its mirror,
+// W columns, 3 components), with one thread per pencil for the transforms:
+// the thread loads the pencil's N0 non-zero values, does the L-point DFT in
+// registers and posts the spectrum to LDS; the 3x3 kernel product runs over
+// the LDS points as in k_zconv; each thread then takes its pencil back,
+// transforms it in registers and stores its N0 rows.  NT = 6 W threads,
+// tid = (s * 3 + c) * W + p, LDS layout [t][s][c][p] as in k_zconv.
+template <typename T, typename TK, int L, int W>
+__global__ void __launch_bounds__(6 * W) k_zconv2(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+                                                  int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
+                                                  Live lv) {
+    constexpr int WP = 3 * W, W2 = 2 * WP, NT = 6 * W;
+    static_assert(reg_dft_ok(L), "register DFT length");
+    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
+    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
+    __shared__ cpx<T> lds[L * W2];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int jp = lb / gridDim.x;
+    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;
+    const int col0 = (lb % gridDim.x) * W;
+    {
+        const int p = tid % W, c = (tid / W) % 3, sm = tid / WP;
+        const int j = sm ? jm : jp;
+        const bool live = j >= 0 && col0 + p < H2;
+        const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
+        cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p;
+        cpx<T> v[L];
+#pragma unroll
+        for (int t = 0; t < L; t++) {
+            cpx<T> x = {T(0), T(0)};
+            if (live && t < N0) x = base[t * tstride];
+            v[t] = x;
+        }
+        dft_reg<T, L, -1>(v);
+#pragma unroll
+        for (int t = 0; t < L; t++) lds[t * W2 + tid] = v[t];
+    }
+    __syncthreads();
+    const long kplane = (long)(L / 2 + 1) * K1 * kp;
+    for (int e = tid; e < L * W; e += NT) {
+        const int t = e / W, p = e % W;
+        const int col = col0 + p;
+        if (col >= H2) continue;
+        const int ii = (t > L / 2) ? L - t : t;
+        const T si = (t > L / 2) ? T(-1) : T(1);
+        const long K = ((long)ii * K1 + jp) * kp + col;
+        const T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
+        const T kyz0 = T(kern[3 * kplane + K]);
+        const T kxz = si * T(kern[4 * kplane + K]);
+        const T kxy0 = si * T(kern[5 * kplane + K]);
+#pragma unroll
+        for (int sm = 0; sm < 2; sm++) {  // mirror pencil: YZ and XY change sign
+            const T sj = sm ? T(-1) : T(1);
+            const T kyz = sj * kyz0, kxy = sj * kxy0;
+            cpx<T>* a = lds + t * W2 + sm * WP + p;
+            cpx<T> mx = cscale(a[0], srcMul);
+            cpx<T> my = cscale(a[W], srcMul);
+            cpx<T> mz = cscale(a[2 * W], srcMul);
+            a[0] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
+            a[W] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
+            a[2 * W] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
+        }
+    }
+    __syncthreads();
+    {
+        const int p = tid % W, c = (tid / W) % 3, sm = tid / WP;
+        const int j = sm ? jm : jp;
+        const bool live = j >= 0 && col0 + p < H2;
+        const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
+        cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p;
+        cpx<T> v[L];
+#pragma unroll
+        for (int t = 0; t < L; t++) v[t] = lds[t * W2 + tid];
+        dft_reg<T, L, +1>(v);
+        if (live) {
+#pragma unroll
+            for (int t = 0; t < L; t++)
+                if (t < N0) base[t * tstride] = v[t];
+        }
+    }
+}

@@ -16,6 +16,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "hs_common.hpp"
@@ -1071,13 +1074,30 @@ StageIO base_io(hs_solver* s, double h, uint32_t step, uint32_t attempt) {
 
 // Pick the fastest stage-kernel variant (marching / one plane per block /
 // two cells per thread) by timing one representative role on the solver's
-// own buffers.  HS_STAGE_KERNEL=march|flat|pair forces one.
+// own buffers.  HS_STAGE_KERNEL=march|flat|pair forces one.  The choice is
+// cached per mesh shape for the life of the process (the variants may round
+// differently, and a solve must give the same result in any batch).
+std::mutex g_stage_mu;
+std::map<std::array<int, 3>, int>& stage_cache() {
+    static std::map<std::array<int, 3>, int> c;
+    return c;
+}
+
 void tune_stage(hs_solver* s) {
     const bool pair_ok = s->phys.N2 % 2 == 0;
     const char* env = getenv("HS_STAGE_KERNEL");
     if (env && !strcmp(env, "flat")) { s->stage_kind = 1; return; }
     if (env && !strcmp(env, "march")) { s->stage_kind = 0; return; }
     if (env && !strcmp(env, "pair") && pair_ok) { s->stage_kind = 2; return; }
+    const std::array<int, 3> key{s->phys.N0, s->phys.N1, s->phys.N2};
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        auto it = stage_cache().find(key);
+        if (it != stage_cache().end()) {
+            s->stage_kind = it->second;
+            return;
+        }
+    }
     StageIO io = base_io(s, 1e-15, 0, 0);
     io.m = s->mA;
     io.B = s->B0;
@@ -1105,6 +1125,10 @@ void tune_stage(hs_solver* s) {
         }
     }
     s->stage_kind = pick;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        stage_cache()[key] = pick;
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
 }

@@ -60,6 +60,10 @@ HS_API double hs_demag_max_imag_ratio(const hs_demag* d);
 /* FP32 tile shapes the plan's autotuner picked: (W, NT) of the row passes
  * (F1/I1), the column passes (F2/I2) and the axis-0 product pass (F3K) */
 HS_API int hs_demag_tiles(const hs_demag* d, int* out6);
+/* Number of registered FP32 variants of the row, column and axis-0 passes
+ * for this plan's lengths (the environment variable HS_DEMAG_FORCE="r,c,z"
+ * selects variants by index at plan creation instead of timing them). */
+HS_API int hs_demag_variant_count(const hs_demag* d, int* out3);
 /* B = mu0 * msat0T0Mul * (K conv (m .* msat0T0)), maxwell_convolution.go:103-131.
  * m, B: 3 device planes of N cells (m and B may not alias); msat0T0: device
  * mask or NULL.  FP64 variant: FP64 FFT passes; FP32 variant: FP32 passes. */

@@ -95,3 +95,34 @@ def test_golden_demag_fp32(pbc, name):
     # and the FP32 field against the FP64 oracle on the same inputs
     want = po.Demag((64, 64, 64), regen.CELL_INTERNAL, pbc).B(m, msk, 800e3).reshape(3, n)
     assert relerr(Bh, want) <= 2e-6
+
+
+def test_every_fp32_pass_variant_matches_fp64(monkeypatch):
+    """Each registered FP32 tile variant of the row, column and axis-0 passes
+    (forced by index) gives the FP64 field to FP32 accuracy."""
+    import ctypes as C
+    from hotspin_amd._lib import lib
+    N, cell = (16, 32, 64), (2e-9, 3e-9, 4e-9)
+    rng = np.random.default_rng(5)
+    ncell = int(np.prod(N))
+    m = rng.standard_normal((3, ncell))
+    ref = Demag(N, cell)
+    Bd = empty((3, ncell))
+    md = dev(m)
+    ref.field_f64(P(md), None, 8e5, P(Bd))
+    B64 = host(Bd)
+    cnt = (C.c_int * 3)()
+    lib().hs_demag_variant_count(ref._h, cnt)
+    assert min(cnt) >= 1
+    mf = dev(m, np.float32)
+    for kind in range(3):
+        for i in range(cnt[kind]):
+            idx = [-1, -1, -1]
+            idx[kind] = i
+            monkeypatch.setenv("HS_DEMAG_FORCE", ",".join(str(x) for x in idx))
+            d = Demag(N, cell)
+            Bf = empty((3, ncell), torch.float32)
+            d.field_f32(P(mf), None, 8e5, P(Bf))
+            err = np.abs(host(Bf) - B64).max() / np.abs(B64).max()
+            assert err <= 2e-6, (kind, i, err)
+            d.close()
