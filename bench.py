@@ -163,6 +163,29 @@ def serving_leg(nreq, plain_steps=20):
     return summ
 
 
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "llbr_512x512x16_rk23_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a pass from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a plain copy); the stage category
+    is the mean of the four RK23 stage roles.  None when not measured."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    table = {}
+    for line in open(TRAFFIC_FILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7:
+            try:
+                table[cells[0]] = float(cells[5]) * 1e6
+            except ValueError:
+                pass
+    if kernel == "STAGE":
+        roles = [table.get(r) for r in ("BS_1", "BS_2", "BS_3", "BS_4")]
+        return None if None in roles else sum(roles) / 4
+    return table.get(kernel)
+
+
 def cpu_baseline(w, kw, m, threads, nsteps=1):
     """FP64 oracle on the host cores: `nsteps` RK23 steps of the same workload."""
     from oracle import pyoracle as po
@@ -298,7 +321,7 @@ def main():
                    "stage_kernel": stage_variant,
                    "state": "f64", "fields": "f32", "reductions": "f64", "final_dt": dt},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom_bytes},
+                     "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "algorithmic_bytes": dom_bytes},
         "kernels": per,
     }
     s.close()
