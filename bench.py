@@ -14,11 +14,12 @@ Two more legs report batched vs plain per-solve throughput:
   time through the plain solver;
 - "serving": Poisson request stream served by the continuous-batching loop
   (p50/p99 latency, cell-steps/s) vs the plain solver per request.
+A "laser" leg runs the laser-heated LLB-r + two-temperature film (5 ps).
 
 Prints one JSON line (see the keys below); progress goes to stderr.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-       [--ens-steps K] [--serve-requests R] [--no-ensemble] [--no-serving]
+       [--ens-steps K] [--serve-requests R] [--no-ensemble] [--no-serving] [--no-laser]
 """
 import argparse
 import json
@@ -163,6 +164,35 @@ def serving_leg(nreq, plain_steps=20):
     return summ
 
 
+def laser_leg(chunk=50):
+    """Laser-heated LLB-r + two-temperature model, 256x256x8, 5 ps of RK23."""
+    import torch
+    from hotspin_amd.api import TMSolver
+    w, kw, tm, m0 = W.laser()
+    ncell = int(np.prod(w["N"]))
+    t0 = time.time()
+    s = TMSolver(w["N"], w["cell"], tm=tm, T0=w["T0"], **kw)
+    s.set_m(m0)
+    torch.cuda.synchronize()
+    setup = time.time() - t0
+    dt, steps, rejected, peak_te, peak_tp = 1e-15, 0, 0, 0.0, 0.0
+    t0 = time.perf_counter()
+    while s.time < w["span"]:
+        dt, st = s.run("rk23", chunk, dt, maxErr=w["tol"], mindt=1e-20, maxdt=5e-14)
+        steps += st.steps
+        rejected += st.rejected
+        te, tp = s.get_T()
+        peak_te, peak_tp = max(peak_te, float(te.max())), max(peak_tp, float(tp.max()))
+    el = time.perf_counter() - t0
+    s.close()
+    log("laser: %d steps (%d rejected) to t = %.3g s in %.2f s, peak Te %.0f K, peak Tp %.0f K" % (
+        steps, rejected, 5e-12, el, peak_te, peak_tp))
+    return {"workload": w["name"], "cells": ncell, "steps": steps, "rejected": rejected,
+            "cell_steps_per_s": ncell * steps / el, "wall_s": el, "setup_s": setup,
+            "peak_Te_K": peak_te, "target_peak_Te_K": w["Tmax"], "peak_Tp_K": peak_tp,
+            "note": "peak temperatures sampled every %d steps (the host reads Te, Tp between runs)" % chunk}
+
+
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "llbr_512x512x16_rk23_hbm_traffic.md")
 
 
@@ -233,6 +263,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-laser", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (copyUnPad3DAsync with equal extents), "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -329,6 +360,8 @@ def main():
         out["ensemble"] = ensemble_leg(args.ens_steps, 5)
     if not args.no_serving:
         out["serving"] = serving_leg(args.serve_requests)
+    if not args.no_laser:
+        out["laser"] = laser_leg()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log("bench: cpu baseline (%d threads)..." % threads)

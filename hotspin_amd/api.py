@@ -200,12 +200,64 @@ class Batch(Solver):
         self.step_count = s0 + int(nsteps)
 
 
+class TmParams(C.Structure):
+    """hs_tm_params of include/hotspin_amd.h."""
+    _fields_ = [(n, C.c_double) for n in ("gammaE", "Cl", "G", "kappa", "Q0", "sigma_r", "sigma_t", "t0", "xc",
+                                          "yc", "maxErrT")]
+
+
+class TMSolver(Solver):
+    """LLB-r coupled to the two-temperature model with a laser source: the
+    LLB temperature is Te per cell; (m, Te, Tp) advance together by RK23."""
+
+    def __init__(self, N, cell, pbc=(0, 0, 0), tm=None, T0=300.0, stream=None, **kw):
+        self.N = tuple(int(x) for x in N)
+        self.ncell = self.N[0] * self.N[1] * self.N[2]
+        self.params, self._keep = make_params(N, cell, pbc, **kw)
+        self.tm = TmParams()
+        for k, v in (tm or {}).items():
+            setattr(self.tm, k, float(v))
+        self._h = lib().hs_solver_create_tm(C.byref(self.params), C.byref(self.tm), float(T0), stream)
+        if not self._h:
+            check("hs_solver_create_tm")
+            raise HotspinError("hs_solver_create_tm failed")
+        self.step_count = 0
+
+    def set_T(self, Te=None, Tp=None):
+        te = None if Te is None else _f64(Te).reshape(-1)
+        tp = None if Tp is None else _f64(Tp).reshape(-1)
+        lib().hs_solver_set_T(self._h, None if te is None else te.ctypes.data, None if tp is None else tp.ctypes.data)
+        check("hs_solver_set_T")
+
+    def get_T(self):
+        te = np.empty(self.ncell)
+        tp = np.empty(self.ncell)
+        lib().hs_solver_get_T(self._h, te.ctypes.data, tp.ctypes.data)
+        check("hs_solver_get_T")
+        return te, tp
+
+    @property
+    def time(self):
+        return lib().hs_solver_time(self._h)
+
+    @time.setter
+    def time(self, t):
+        lib().hs_solver_set_time(self._h, float(t))
+
+    def tm_rhs(self, t):
+        dte = np.empty(self.ncell)
+        dtp = np.empty(self.ncell)
+        lib().hs_solver_tm_rhs(self._h, float(t), dte.ctypes.data, dtp.ctypes.data)
+        check("hs_solver_tm_rhs")
+        return dte, dtp
+
+
 class Slot(C.Structure):
     """hs_slot of include/hotspin_amd.h."""
     _fields_ = [("step", C.c_uint32), ("attempt", C.c_uint32), ("h", C.c_double), ("t", C.c_double),
                 ("err", C.c_double), ("maxErr", C.c_double), ("mindt", C.c_double), ("maxdt", C.c_double),
                 ("end", C.c_uint32), ("state", C.c_int32), ("rejected", C.c_uint32), ("accepted", C.c_uint32),
-                ("home", C.c_uint32), ("retry", C.c_uint32)]
+                ("home", C.c_uint32), ("retry", C.c_uint32), ("maxErrT", C.c_double)]
 
 
 SLOT_IDLE, SLOT_RUNNING, SLOT_DONE, SLOT_FAILED = 0, 1, 2, 3

@@ -39,6 +39,7 @@ struct SolveP {
     double thC;     // 2 kB TMul / (mu0 V gamma Ms0T0Mul); thermal K = thC / max(cutoff_dt, dt)
     double lf_b;    // uniform longitudinal field: x = |m| * lf_b = 3 preS Tc / T |m|
     double lf_c1;   // and H_lf = lf_c1 (B(x) - |m|) / (msk dB/dx |m|) m, lf_c1 = n kB T / (mu0 Ms0T0)
+    double t_origin;  // two-temperature solves: laser time at the start of the current run
     uint64_t seed;
     uint32_t stream;
     uint32_t pad;
@@ -64,7 +65,10 @@ constexpr int kMaxTry = 10;  // solve_rk12.go:44 (maxTry), retries of one step
 // 128-byte line (stride kErrStride), so the per-block atomics of a launch
 // spread over many L2 lines instead of serialising on one address.
 constexpr int kErrLanes = 32, kErrStride = 16;
-constexpr int kErrPerSolve = kErrLanes * kErrStride;
+constexpr int kErrPerEq = kErrLanes * kErrStride;
+// equations with their own error max: m, and for two-temperature solves Te, Tp
+constexpr int kNeq = 3;
+constexpr int kErrPerSolve = kNeq * kErrPerEq;
 
 struct Phys {
     int N0, N1, N2, w0, w1, w2;
@@ -106,6 +110,7 @@ struct StageIO {
     float* kout;      // R_RHS: k
     float* hout;      // R_RHS: H_eff
     unsigned long long* errmax;  // max |e| as double bits: kErrLanes lanes per solve
+    const float* T;   // two-temperature solves: Te of the stage input (FP32, [solve][N]); overrides the T mask
     double h;
     uint32_t step, attempt;
     const Clock* clk;  // when set, step/attempt/h of each solve are read from here
@@ -126,6 +131,7 @@ __device__ __forceinline__ StageIO solve_io(const StageIO& io, int sol, long N) 
     if (r.mout) r.mout += b;
     if (r.kout) r.kout += b;
     if (r.hout) r.hout += b;
+    if (r.T) r.T += (long)sol * N;
     r.errmax += (long)sol * kErrPerSolve;
     if (io.clk) {
         r.step = io.clk[sol].step;
@@ -267,7 +273,7 @@ __device__ __forceinline__ void preload(const Phys& P, const StageIO& io, long I
         q.n = P.n ? P.n[I] : 1.f;
         q.Tc = P.Tc ? P.Tc[I] : 1.f;
     }
-    q.T = P.T ? P.T[I] : 1.f;
+    q.T = io.T ? io.T[I] : (P.T ? P.T[I] : 1.f);
     for (int c = 0; c < 3; c++) q.al[c] = P.alpha[c] ? P.alpha[c][I] : 1.f;
     if constexpr (ROLE == R_RK4_1 || ROLE == R_BS_1 || ROLE == R_RK12_1 || ROLE == R_EULER || ROLE == R_RK4_2 ||
                   ROLE == R_RK4_3 || ROLE == R_BS_2 || ROLE == R_RK4_4 || ROLE == R_BS_3)
@@ -729,7 +735,7 @@ __device__ __forceinline__ void preload2(const Phys& P, const StageIO& io, long 
         v = ld2m(P.Tc, I);
         q[0].Tc = v.x, q[1].Tc = v.y;
     }
-    v = ld2m(P.T, I);
+    v = io.T ? ld2(io.T, I) : ld2m(P.T, I);
     q[0].T = v.x, q[1].T = v.y;
     for (int c = 0; c < 3; c++) {
         v = ld2m(P.alpha[c], I);
@@ -920,7 +926,10 @@ __host__ __device__ inline double controller(double dt, double err, double maxEr
 // Per-slot accept/reject after an RK23 attempt (one thread per slot).  The
 // state of iteration `par` is in buffer set par; an accepted attempt leaves
 // the new state in set par^1, a rejected one is copied there by k_fixup.
-__global__ void k_control(Clock* clk, unsigned long long* errmax, int* live, int nsolve, uint32_t par) {
+// With neq = 3 (two-temperature solves) the step follows the reference's
+// multi-equation rule (solve_rk12.go:150-190): bad if any equation exceeds
+// its bound, factor = the smallest of the equations' factors.
+__global__ void k_control(Clock* clk, unsigned long long* errmax, int* live, int nsolve, uint32_t par, int neq) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nsolve) return;
     Clock c = clk[q];
@@ -928,14 +937,19 @@ __global__ void k_control(Clock* clk, unsigned long long* errmax, int* live, int
         live[q] = 0;
         return;
     }
-    unsigned long long* em = errmax + (long)q * kErrPerSolve;
-    double err = 0.0;
-    for (int l = 0; l < kErrLanes; l++) {
-        err = fmax(err, __longlong_as_double((long long)em[l * kErrStride]));
-        em[l * kErrStride] = 0ull;
+    double errs[kNeq] = {0.0, 0.0, 0.0};
+    bool finite = true;
+    for (int e = 0; e < neq; e++) {
+        unsigned long long* em = errmax + (long)q * kErrPerSolve + e * kErrPerEq;
+        for (int l = 0; l < kErrLanes; l++) {
+            errs[e] = fmax(errs[e], __longlong_as_double((long long)em[l * kErrStride]));
+            em[l * kErrStride] = 0ull;
+        }
+        finite = finite && isfinite(errs[e]);
     }
+    const double err = errs[0];
     c.err = err;
-    if (!isfinite(err)) {  // the state has diverged: stop this slot
+    if (!finite) {  // the state has diverged: stop this slot
         c.state = HS_SLOT_FAILED;
         c.home = par;
         c.retry = 0;
@@ -943,8 +957,19 @@ __global__ void k_control(Clock* clk, unsigned long long* errmax, int* live, int
         live[q] = 0;
         return;
     }
-    bool bad = err > c.maxErr;
-    double nd = controller(c.h, err, c.maxErr, bad, 1.0 / 3.0, 1.0 / 3.0, c.mindt, c.maxdt);
+    bool bad = false;
+    double minf = 2.0;
+    for (int e = 0; e < neq; e++) {
+        const double me = e ? c.maxErrT : c.maxErr;
+        if (errs[e] > me) bad = true;
+        double f = pow(me / errs[e], 1.0 / 3.0) * 0.8;
+        if (f > 1.5) f = 1.5;
+        if (f < 0.1) f = 0.1;
+        minf = fmin(minf, f);
+    }
+    double nd = c.h * minf;
+    if (nd < c.mindt) nd = c.mindt;
+    if (nd > c.maxdt) nd = c.maxdt;
     if (!bad || nd == c.mindt) {
         c.t += c.h;
         c.h = nd;
@@ -988,6 +1013,141 @@ __global__ void k_fixup(const Clock* clk, const double* ycur, const float* mcur,
     }
 }
 
+// ---- two-temperature model ------------------------------------------------
+struct TmP {
+    double gammaE, Cl, G, kappa, Q0, ir2, it2, t0, xc, yc;  // ir2 = 1/(2 sigma_r^2), it2 = 1/(2 sigma_t^2)
+    double c2x, c2y, c2z;  // 1/cell^2 of internal axes 0, 1, 2 (Qspat.cu's x, y, z)
+    double hx, hy;         // cell sizes of the user x (internal 2) and y (internal 1) axes
+};
+struct TmIO {
+    const double *Te, *Tp;        // stage input
+    const double *Te0, *Tp0;      // state at step start (BS_3)
+    double *Teo, *Tpo;            // next stage input (BS_3: the new state)
+    float* Tfo;                   // FP32 copy of the next stage's Te for the LLB fields
+    double *k1e, *k1p, *k2e, *k2p;
+    double *ee, *ep;              // error partials (BS_3 -> BS_4)
+    unsigned long long* errmax;   // solve base; Te lanes at +kErrPerEq, Tp at +2 kErrPerEq
+    const Clock* clk;
+    const SolveP* sp;
+    double t;                     // laser time of stage-1 (no clock: testing)
+};
+
+// Q_e = kappa lap(Te) (Qspat.cu:13-134 with uniform conductance, neighbour
+// order X, Z, Y as there) + G (Tp - Te) (Qinter.cu:13-45) + Q_laser;
+// dTe/dt = Q_e / (gammaE Te) (TM.go:40-59 with mod_ce.go:73-94), dTp/dt = G (Te - Tp) / Cl.
+// ROLE: R_RHS (slopes to k1e/k1p at time t) or the four Bogacki-Shampine stages.
+template <int ROLE>
+__global__ void __launch_bounds__(256) k_tm(Phys P, TmP Q, TmIO io) {
+    __shared__ double wm[2][4];
+    const int sol = blockIdx.y;
+    if (io.clk && io.clk[sol].state != HS_SLOT_RUNNING) return;
+    const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
+    const long N12 = (long)N1 * N2, N = (long)N0 * N12;
+    const long b = (long)sol * N;
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double el = 0.0, pl = 0.0;
+    if (I < N) {
+        const int i = (int)(I / N12), j = (int)((I / N2) % N1), k = (int)(I % N2);
+        const double* Te = io.Te + b;
+        const double T0 = Te[I], Tp = io.Tp[b + I];
+        auto at = [&](int ii, int jj, int kk) { return Te[(long)ii * N12 + (long)jj * N2 + kk]; };
+        int a, c;
+        double ddT;
+        a = i - 1; a = (a < 0 && P.w0) ? N0 + a : a; a = a < 0 ? 0 : a;
+        c = i + 1; c = (c == N0 && P.w0) ? c - N0 : c; c = c > N0 - 1 ? N0 - 1 : c;
+        ddT = Q.c2x * ((at(a, j, k) + at(c, j, k)) - (T0 + T0));
+        a = k - 1; a = (a < 0 && P.w2) ? N2 + a : a; a = a < 0 ? 0 : a;
+        c = k + 1; c = (c == N2 && P.w2) ? c - N2 : c; c = c > N2 - 1 ? N2 - 1 : c;
+        ddT += Q.c2z * ((at(i, j, a) + at(i, j, c)) - (T0 + T0));
+        a = j - 1; a = (a < 0 && P.w1) ? N1 + a : a; a = a < 0 ? 0 : a;
+        c = j + 1; c = (c == N1 && P.w1) ? c - N1 : c; c = c > N1 - 1 ? N1 - 1 : c;
+        ddT += Q.c2y * ((at(i, a, k) + at(i, c, k)) - (T0 + T0));
+        double h = 0.0, ts = io.t;
+        if (io.clk) {
+            const Clock& ck = io.clk[sol];
+            h = ck.h;
+            const double cs = (ROLE == R_BS_2) ? 0.5 : (ROLE == R_BS_3) ? 0.75 : (ROLE == R_BS_4) ? 1.0 : 0.0;
+            ts = io.sp[sol].t_origin + ck.t + cs * h;
+        }
+        const double x = (k + 0.5) * Q.hx - Q.xc, y = (j + 0.5) * Q.hy - Q.yc, dt0 = ts - Q.t0;
+        const double ql = Q.Q0 * exp(-(x * x + y * y) * Q.ir2) * exp(-(dt0 * dt0) * Q.it2);
+        const double qe = Q.kappa * ddT + Q.G * (Tp - T0) + ql;
+        const double ke = qe / (Q.gammaE * T0), kp = Q.G * (T0 - Tp) / Q.Cl;
+        const long J = b + I;
+        if constexpr (ROLE == R_RHS) {
+            io.k1e[J] = ke;
+            io.k1p[J] = kp;
+        } else if constexpr (ROLE == R_BS_1 || ROLE == R_BS_2) {
+            const double cm = (ROLE == R_BS_1) ? 0.5 * h : 0.75 * h;
+            double* ke_ = (ROLE == R_BS_1) ? io.k1e : io.k2e;
+            double* kp_ = (ROLE == R_BS_1) ? io.k1p : io.k2p;
+            ke_[J] = ke;
+            kp_[J] = kp;
+            const double te = io.Te0[J] + cm * ke, tp = io.Tp0[J] + cm * kp;
+            io.Teo[J] = te;
+            io.Tpo[J] = tp;
+            io.Tfo[J] = (float)te;
+        } else if constexpr (ROLE == R_BS_3) {
+            const double k1e = io.k1e[J], k2e = io.k2e[J], k1p = io.k1p[J], k2p = io.k2p[J];
+            const double te = io.Te0[J] + h * ((2.0 / 9.0) * k1e + (1.0 / 3.0) * k2e + (4.0 / 9.0) * ke);
+            const double tp = io.Tp0[J] + h * ((2.0 / 9.0) * k1p + (1.0 / 3.0) * k2p + (4.0 / 9.0) * kp);
+            io.Teo[J] = te;
+            io.Tpo[J] = tp;
+            io.Tfo[J] = (float)te;
+            io.ee[J] = h * ((-5.0 / 72.0) * k1e + (1.0 / 12.0) * k2e + (1.0 / 9.0) * ke);
+            io.ep[J] = h * ((-5.0 / 72.0) * k1p + (1.0 / 12.0) * k2p + (1.0 / 9.0) * kp);
+        } else if constexpr (ROLE == R_BS_4) {
+            el = fabs(io.ee[J] - (h / 8.0) * ke);
+            pl = fabs(io.ep[J] - (h / 8.0) * kp);
+        }
+    }
+    if constexpr (ROLE == R_BS_4) {
+        for (int o = 32; o > 0; o >>= 1) {
+            el = fmax(el, __shfl_xor(el, o, 64));
+            pl = fmax(pl, __shfl_xor(pl, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            wm[0][threadIdx.x >> 6] = el;
+            wm[1][threadIdx.x >> 6] = pl;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+                el = fmax(el, wm[0][w]);
+                pl = fmax(pl, wm[1][w]);
+            }
+            unsigned long long* em = io.errmax + (long)sol * kErrPerSolve;
+            const unsigned lane = blockIdx.x % kErrLanes;
+            atomicMax(em + kErrPerEq + lane * kErrStride, (unsigned long long)__double_as_longlong(el));
+            atomicMax(em + 2 * kErrPerEq + lane * kErrStride, (unsigned long long)__double_as_longlong(pl));
+        }
+    }
+}
+
+// k_fixup / k_gather for the temperature buffers of a buffer set
+__global__ void k_fixup_tm(const Clock* clk, const double* tec, const double* tpc, const float* tfc, double* ten,
+                           double* tpn, float* tfn, long N) {
+    const int q = blockIdx.y;
+    if (clk[q].state != HS_SLOT_RUNNING || !clk[q].retry) return;
+    long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const long J = (long)q * N + I;
+    ten[J] = tec[J];
+    tpn[J] = tpc[J];
+    tfn[J] = tfc[J];
+}
+__global__ void k_gather_tm(const Clock* clk, const double* te1, const double* tp1, const float* tf1, double* te0,
+                            double* tp0, float* tf0, long N) {
+    const int q = blockIdx.y;
+    if (clk[q].home != 1u) return;
+    long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const long J = (long)q * N + I;
+    te0[J] = te1[J];
+    tp0[J] = tp1[J];
+    tf0[J] = tf1[J];
+}
+
 }  // namespace
 
 struct hs_solver {
@@ -1022,6 +1182,16 @@ struct hs_solver {
     std::vector<float*> masks;
     bool b0_valid = false;  // B0 = demag(mA) is current
     int stage_kind = 1;      // 0 marching planes, 1 one plane per block, 2 two cells per thread
+    // two-temperature coupling (tm): Te, Tp in two buffer sets (state / candidate), two stage inputs
+    bool tm = false;
+    hs_tm_params tmp{};
+    TmP tmc{};
+    double tm_time = 0.0;  // laser clock
+    double *Te[2] = {nullptr, nullptr}, *Tp[2] = {nullptr, nullptr};
+    float* Tf[2] = {nullptr, nullptr};
+    double *Tea = nullptr, *Tpa = nullptr, *Teb = nullptr, *Tpb = nullptr;
+    float *Tfa = nullptr, *Tfb = nullptr;
+    double *k1e = nullptr, *k1p = nullptr, *k2e = nullptr, *k2p = nullptr, *ee = nullptr, *ep = nullptr;
 };
 
 namespace {
@@ -1216,27 +1386,56 @@ Set slot_set(hs_solver* s, int par) {
 // One RK23 attempt of every running slot with the state in set par: the
 // stages leave the candidate in set par^1, k_control decides per slot and
 // k_fixup restores the rejected slots' state into set par^1.
+template <int ROLE>
+void launch_tm(hs_solver* s, const double* Te, const double* Tp, int par, double* Teo, double* Tpo, float* Tfo) {
+    TmIO t{};
+    t.Te = Te;
+    t.Tp = Tp;
+    t.Te0 = s->Te[par];
+    t.Tp0 = s->Tp[par];
+    t.Teo = Teo;
+    t.Tpo = Tpo;
+    t.Tfo = Tfo;
+    t.k1e = s->k1e, t.k1p = s->k1p, t.k2e = s->k2e, t.k2p = s->k2p, t.ee = s->ee, t.ep = s->ep;
+    t.errmax = s->errmax;
+    t.clk = s->clk;
+    t.sp = s->sp;
+    hipLaunchKernelGGL(k_tm<ROLE>, dim3(hs::grid1d(s->N), s->nsolve), 256, 0, s->st, s->phys, s->tmc, t);
+}
+
 void slot_iteration(hs_solver* s, int par) {
     const Set cur = slot_set(s, par), nxt = slot_set(s, par ^ 1);
+    const int q = par ^ 1;
     StageIO io = base_io(s, 0.0, 0, 0);
     io.clk = s->clk;
     io.y0 = cur.y;
     io.acc = nxt.y;
     io.m = cur.m; io.B = cur.B; io.mout = nxt.m;
+    if (s->tm) io.T = s->Tf[par];
     launch_stage<R_BS_1>(s, io);
+    if (s->tm) launch_tm<R_BS_1>(s, s->Te[par], s->Tp[par], par, s->Tea, s->Tpa, s->Tfa);
     demag_into(s, nxt.m, nxt.B, s->live);
     io.m = nxt.m; io.B = nxt.B; io.mout = s->mC;
+    if (s->tm) io.T = s->Tfa;
     launch_stage<R_BS_2>(s, io);
+    if (s->tm) launch_tm<R_BS_2>(s, s->Tea, s->Tpa, par, s->Teb, s->Tpb, s->Tfb);
     demag_into(s, s->mC, nxt.B, s->live);
     io.m = s->mC; io.B = nxt.B; io.mout = nxt.m;
+    if (s->tm) io.T = s->Tfb;
     launch_stage<R_BS_3>(s, io);
+    if (s->tm) launch_tm<R_BS_3>(s, s->Teb, s->Tpb, par, s->Te[q], s->Tp[q], s->Tf[q]);
     demag_into(s, nxt.m, nxt.B, s->live);
     io.m = nxt.m; io.B = nxt.B; io.mout = nullptr;
+    if (s->tm) io.T = s->Tf[q];
     launch_stage<R_BS_4>(s, io);
+    if (s->tm) launch_tm<R_BS_4>(s, s->Te[q], s->Tp[q], par, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL(k_control, hs::div_up(s->nsolve, 256), 256, 0, s->st, s->clk, s->errmax, s->live,
-                       s->nsolve, (uint32_t)par);
+                       s->nsolve, (uint32_t)par, s->tm ? 3 : 1);
     hipLaunchKernelGGL(k_fixup, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->clk, cur.y, cur.m,
                        cur.B, nxt.y, nxt.m, nxt.B, s->N, s->phys.demag);
+    if (s->tm)
+        hipLaunchKernelGGL(k_fixup_tm, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->clk, s->Te[par],
+                           s->Tp[par], s->Tf[par], s->Te[q], s->Tp[q], s->Tf[q], s->N);
 }
 
 bool capture(hs_solver* s, hipGraphExec_t* out, const char* who, void (*body)(hs_solver*, int), int arg) {
@@ -1332,6 +1531,11 @@ int run_rk23_device(hs_solver* s, long nsteps, long step0, double* dtio, double 
         c.maxdt = maxdt;
         c.end = (uint32_t)(step0 + nsteps);
         c.state = HS_SLOT_RUNNING;
+        c.maxErrT = s->tm ? s->tmp.maxErrT : 0.0;
+    }
+    if (s->tm) {  // laser clock of this run
+        for (auto& q : s->sp_host) q.t_origin = s->tm_time;
+        hipMemcpyAsync(s->sp, s->sp_host.data(), sizeof(SolveP) * s->nsolve, hipMemcpyHostToDevice, s->st);
     }
     std::vector<int> lv(s->nsolve, 1);
     hipMemcpyAsync(s->clk, hc.data(), sizeof(Clock) * s->nsolve, hipMemcpyHostToDevice, s->st);
@@ -1365,6 +1569,9 @@ int run_rk23_device(hs_solver* s, long nsteps, long step0, double* dtio, double 
     }
     hipLaunchKernelGGL(k_gather, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->clk, s->acc, s->mB,
                        s->Bs, s->y0, s->mA, s->B0, s->N, s->phys.demag);
+    if (s->tm)
+        hipLaunchKernelGGL(k_gather_tm, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->clk, s->Te[1],
+                           s->Tp[1], s->Tf[1], s->Te[0], s->Tp[0], s->Tf[0], s->N);
     hipStreamSynchronize(s->st);
     s->slot_iter = 0;
     s->b0_valid = s->phys.demag != 0;
@@ -1380,6 +1587,7 @@ int run_rk23_device(hs_solver* s, long nsteps, long step0, double* dtio, double 
     }
     const Clock& c0 = s->slot_host[0];
     st.t = c0.t;
+    if (s->tm) s->tm_time += c0.t;
     st.last_err = c0.err;
     st.dt = dtmin;
     *dtio = dtmin;
@@ -1561,6 +1769,10 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
+    void* tmb[] = {s->Te[0], s->Te[1], s->Tp[0], s->Tp[1], s->Tf[0], s->Tf[1], s->Tea, s->Tpa, s->Teb, s->Tpb,
+                   s->Tfa, s->Tfb, s->k1e, s->k1p, s->k2e, s->k2p, s->ee, s->ep};
+    for (void* q : tmb)
+        if (q) hipFree(q);
     if (s->own_stream) hipStreamDestroy(s->st);
     delete s;
 }
@@ -1610,6 +1822,7 @@ extern "C" int hs_solver_rhs(hs_solver* s, double* k_host, double* h_host, uint3
     io.B = s->B0;
     io.kout = s->kout;
     io.hout = s->hout;
+    if (s->tm) io.T = s->Tf[0];
     launch_stage<R_RHS>(s, io);
     std::vector<float> tmp(3 * NN);
     if (k_host) {
@@ -1658,6 +1871,10 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
     if (!s || !dtio || nsteps < 0) return HS_EINVAL;
     if (!whole_batch_ok(s, "hs_solver_run: handle is in slot mode")) return HS_EINVAL;
     if (method == HS_RK23) return run_rk23_device(s, nsteps, step0, dtio, maxErr, mindt, maxdt, stats);
+    if (s->tm) {
+        hs::set_error(HS_EINVAL, "hs_solver_run: two-temperature solves take HS_RK23");
+        return HS_EINVAL;
+    }
     hs_run_stats st{};
     double dt = *dtio;
     const int maxTry = 10;
@@ -1746,8 +1963,8 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
 extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMul, uint64_t seed, uint32_t stream,
                              const double* Bext, long step0, long nsteps, double dt, double maxErr, double mindt,
                              double maxdt) {
-    if (!s || !m || slot < 0 || slot >= s->nsolve || nsteps < 0 || !(dt > 0.0) || !(maxErr > 0.0)) {
-        hs::set_error(HS_EINVAL, "hs_slot_admit: bad argument");
+    if (!s || !m || slot < 0 || slot >= s->nsolve || nsteps < 0 || !(dt > 0.0) || !(maxErr > 0.0) || s->tm) {
+        hs::set_error(HS_EINVAL, "hs_slot_admit: bad argument (or a two-temperature solver)");
         return HS_EINVAL;
     }
     if (!s->slot_mode) {
@@ -1964,4 +2181,101 @@ extern "C" void hs_ensemble_destroy(hs_ensemble* e) {
     for (auto& v : e->ev) hipEventDestroy(v);
     if (e->origin) hipStreamDestroy(e->origin);
     delete e;
+}
+
+// ---------------------------------------------------------------- two-temperature solves
+extern "C" hs_solver* hs_solver_create_tm(const hs_llb_params* p, const hs_tm_params* tm, double T0,
+                                          CUstream stream) {
+    if (!p || !tm || !(tm->gammaE > 0.0) || !(tm->Cl > 0.0) || !(T0 > 0.0) || !(tm->maxErrT > 0.0)) {
+        hs::set_error(HS_EINVAL, "hs_solver_create_tm: bad parameters");
+        return nullptr;
+    }
+    hs_llb_params q = *p;
+    q.TMul = 1.0;  // T = Te per cell
+    q.T = nullptr;
+    hs_solver* s = hs_batch_create(&q, 1, nullptr, nullptr, nullptr, stream);
+    if (!s) return nullptr;
+    s->tm = true;
+    s->tmp = *tm;
+    s->phys.lf.uniform = 0;  // the longitudinal field sees a per-cell temperature
+    TmP& c = s->tmc;
+    c.gammaE = tm->gammaE, c.Cl = tm->Cl, c.G = tm->G, c.kappa = tm->kappa, c.Q0 = tm->Q0;
+    c.ir2 = tm->sigma_r > 0 ? 1.0 / (2.0 * tm->sigma_r * tm->sigma_r) : 0.0;
+    c.it2 = tm->sigma_t > 0 ? 1.0 / (2.0 * tm->sigma_t * tm->sigma_t) : 0.0;
+    c.t0 = tm->t0, c.xc = tm->xc, c.yc = tm->yc;
+    c.c2x = 1.0 / (p->cell[0] * p->cell[0]);
+    c.c2y = 1.0 / (p->cell[1] * p->cell[1]);
+    c.c2z = 1.0 / (p->cell[2] * p->cell[2]);
+    c.hx = p->cell[2], c.hy = p->cell[1];
+    const long N = s->N;
+    double** d64[] = {&s->Te[0], &s->Te[1], &s->Tp[0], &s->Tp[1], &s->Tea, &s->Tpa, &s->Teb, &s->Tpb,
+                      &s->k1e, &s->k1p, &s->k2e, &s->k2p, &s->ee, &s->ep};
+    float** d32[] = {&s->Tf[0], &s->Tf[1], &s->Tfa, &s->Tfb};
+    bool ok = true;
+    for (auto q2 : d64) ok = ok && hipMalloc(q2, sizeof(double) * N) == hipSuccess;
+    for (auto q2 : d32) ok = ok && hipMalloc(q2, sizeof(float) * N) == hipSuccess;
+    if (!ok) {
+        hs::set_error(HS_ENOMEM, "hs_solver_create_tm: out of device memory");
+        hs_solver_destroy(s);
+        return nullptr;
+    }
+    std::vector<double> t(N, T0);
+    if (hs_solver_set_T(s, t.data(), t.data()) != 0) {
+        hs_solver_destroy(s);
+        return nullptr;
+    }
+    return s;
+}
+
+extern "C" int hs_solver_set_T(hs_solver* s, const double* Te, const double* Tp) {
+    if (!s || !s->tm) {
+        hs::set_error(HS_EINVAL, "hs_solver_set_T: not a two-temperature solver");
+        return HS_EINVAL;
+    }
+    if (Te) {
+        hipMemcpyAsync(s->Te[0], Te, sizeof(double) * s->N, hipMemcpyHostToDevice, s->st);
+        hipLaunchKernelGGL(k_d2f, hs::grid1d(s->N), hs::kBlock, 0, s->st, s->Te[0], s->Tf[0], s->N);
+    }
+    if (Tp) hipMemcpyAsync(s->Tp[0], Tp, sizeof(double) * s->N, hipMemcpyHostToDevice, s->st);
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_set_T");
+    return hs_last_error();
+}
+
+extern "C" int hs_solver_get_T(hs_solver* s, double* Te, double* Tp) {
+    if (!s || !s->tm) {
+        hs::set_error(HS_EINVAL, "hs_solver_get_T: not a two-temperature solver");
+        return HS_EINVAL;
+    }
+    if (Te) hipMemcpyAsync(Te, s->Te[0], sizeof(double) * s->N, hipMemcpyDeviceToHost, s->st);
+    if (Tp) hipMemcpyAsync(Tp, s->Tp[0], sizeof(double) * s->N, hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_get_T");
+    return hs_last_error();
+}
+
+extern "C" int hs_solver_set_time(hs_solver* s, double t) {
+    if (!s) return HS_EINVAL;
+    s->tm_time = t;
+    return 0;
+}
+extern "C" double hs_solver_time(hs_solver* s) { return s ? s->tm_time : 0.0; }
+
+extern "C" int hs_solver_tm_rhs(hs_solver* s, double t, double* dTe, double* dTp) {
+    if (!s || !s->tm) {
+        hs::set_error(HS_EINVAL, "hs_solver_tm_rhs: not a two-temperature solver");
+        return HS_EINVAL;
+    }
+    TmIO io{};
+    io.Te = s->Te[0];
+    io.Tp = s->Tp[0];
+    io.k1e = s->k1e;
+    io.k1p = s->k1p;
+    io.t = t;
+    hipLaunchKernelGGL(k_tm<R_RHS>, dim3(hs::grid1d(s->N), 1), 256, 0, s->st, s->phys, s->tmc, io);
+    if (dTe) hipMemcpyAsync(dTe, s->k1e, sizeof(double) * s->N, hipMemcpyDeviceToHost, s->st);
+    if (dTp) hipMemcpyAsync(dTp, s->k1p, sizeof(double) * s->N, hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_tm_rhs");
+    return hs_last_error();
 }

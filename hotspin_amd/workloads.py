@@ -12,6 +12,8 @@ N = (Z, Y, X).
   along +x tilted per cell by up to 10 degrees, an in-plane field of 20-30 mT
   at 150-190 degrees, fixed RK4 at dt = 1e-13 s, lazy normalisation.
 - `single()`: one 512x512x16 solve at 300 K, 2 nm cubes, RK23 tol 1e-5.
+- `laser()`: 256x256x8 film heated by a Gaussian pump, LLB-r coupled to the
+  two-temperature model (T = Te), 5 ps of adaptive RK23.
 - `ensemble()` (material MATERIAL_WIDE_T): 512 solves, seeds 0..511, meshes drawn per seed from
   {32x32x32, 64x64x4, 128x128x2, 96x48x8}, 3 nm cells, T ~ U(100, 1000) K,
   fixed RK4 at dt = 1e-15 s.
@@ -119,6 +121,36 @@ def single():
     me = equilibrium_m(w["T"], MATERIAL["Tc"], MATERIAL["J"])
     m = random_domains(w["N"], w["block"], me, np.random.default_rng(w["seed"]))
     return w, llbr_params(T=w["T"], seed=w["seed"]), m, me
+
+
+def laser(Tc=MATERIAL["Tc"]):
+    """Laser-heated LLB-r + two-temperature model on a 256x256x8 film of
+    4x4x2 nm cells: Ce = gamma Te (gamma = 6e3 J/m^3K^2), Cl = 3.8e6 J/m^3K,
+    G = 1.8e18 W/m^3K, kappa = 90 W/mK, Gaussian pump (sigma_r = 200 nm at the
+    film centre, sigma_t = 100 fs, centred at t0 = 300 fs), T0 = 300 K.  Q0 is
+    set so that the absorbed energy at the spot centre, gamma (Tmax^2 - T0^2)/2
+    with Tmax = 1.3 Tc, equals Q0 sigma_t sqrt(2 pi) (the pulse is much shorter
+    than the electron-phonon time gamma Te / G ~ 3 ps).  Span 5 ps, RK23 with
+    bounds 1e-5 (m) and 1e-3 K (Te, Tp).  Returns (w, kw, tm, m0)."""
+    N = (8, 256, 256)
+    cell = (2e-9, 4e-9, 4e-9)
+    T0, gam = 300.0, 6e3
+    Tmax = 1.3 * Tc
+    sig_t = 100e-15
+    tm = dict(gammaE=gam, Cl=3.8e6, G=1.8e18, kappa=90.0,
+              Q0=gam * (Tmax * Tmax - T0 * T0) / (2.0 * sig_t * math.sqrt(2.0 * math.pi)),
+              sigma_r=200e-9, sigma_t=sig_t, t0=3 * sig_t, xc=N[2] * cell[2] / 2, yc=N[1] * cell[1] / 2,
+              maxErrT=1e-3)
+    me = equilibrium_m(T0, MATERIAL["Tc"], MATERIAL["J"])
+    rng = np.random.default_rng(3)
+    ncell = N[0] * N[1] * N[2]
+    th = np.radians(rng.uniform(0.0, 5.0, ncell))
+    ps = rng.uniform(0.0, 2 * math.pi, ncell)
+    m0 = me * np.stack([np.sin(th) * np.sin(ps), np.sin(th) * np.cos(ps), np.cos(th)])  # user +x
+    kw = llbr_params(T=T0, seed=77)
+    w = dict(name="laser LLB-r + 2TM 256x256x8 cells 4x4x2nm, 5 ps, RK23", N=N, cell=cell, T0=T0, span=5e-12,
+             tol=1e-5, Tmax=Tmax)
+    return w, kw, tm, np.ascontiguousarray(m0)
 
 
 ENSEMBLE_SHAPES = [(32, 32, 32), (4, 64, 64), (2, 128, 128), (8, 48, 96)]  # user 32^3, 64x64x4, 128x128x2, 96x48x8

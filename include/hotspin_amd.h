@@ -170,6 +170,44 @@ HS_API hs_demag* hs_solver_demag(hs_solver* s);
  * HS_STAGE_KERNEL=march|flat|pair overrides */
 HS_API int hs_solver_stage_variant(hs_solver* s);
 
+/* ------------------------------------------------ two-temperature coupling
+ * Laser-heated LLB-r: the LLB temperature is the electron temperature Te of
+ * each cell, and Te, Tp follow the two-temperature model of the reference's
+ * temperature modules (modules/TM.go dT/dt = Q/C, Qspat.go heat diffusion
+ * with harmonic-mean conductances (Qspat.cu:13-134), Qinter.go electron-
+ * phonon exchange G (Tp - Te) (Qinter.cu:13-45), mod_ce.go Drude heat
+ * capacity Ce = gamma Te):
+ *   gamma Te dTe/dt = kappa lap(Te) + G (Tp - Te) + Q_laser(x, y, t)
+ *   Cl dTp/dt       = G (Te - Tp)
+ * Q_laser = Q0 exp(-((x-xc)^2 + (y-yc)^2) / (2 sigma_r^2)) exp(-(t-t0)^2 / (2 sigma_t^2))
+ * with x, y the user in-plane coordinates of the cell centre (internal axes
+ * 2 and 1), uniform through the thickness.  Te, Tp are FP64 state, their
+ * stage values reach the LLB fields as FP32.  hs_solver_run with HS_RK23
+ * integrates (m, Te, Tp) together: one error per equation, the reference's
+ * multi-equation controller (solve_rk12.go:150-190: a step is bad if any
+ * equation is, the factor is the smallest of the equations'), maxErr for m,
+ * maxErrT for Te and Tp. */
+typedef struct {
+    double gammaE;  /* electron heat capacity coefficient, J/(m^3 K^2) */
+    double Cl;      /* lattice heat capacity, J/(m^3 K) */
+    double G;       /* electron-phonon coupling, W/(m^3 K) */
+    double kappa;   /* electron thermal conductivity, W/(m K) */
+    double Q0;      /* laser peak power density, W/m^3 */
+    double sigma_r, sigma_t, t0, xc, yc;
+    double maxErrT; /* K */
+} hs_tm_params;
+/* Resident solver with two-temperature coupling (the LLB T and T mask of p
+ * are ignored: T = Te per cell).  Te, Tp start uniform at T0. */
+HS_API hs_solver* hs_solver_create_tm(const hs_llb_params* p, const hs_tm_params* tm, double T0, CUstream stream);
+/* Te, Tp: N doubles each on the host (either may be NULL) */
+HS_API int hs_solver_set_T(hs_solver* s, const double* Te, const double* Tp);
+HS_API int hs_solver_get_T(hs_solver* s, double* Te, double* Tp);
+/* simulated time of the laser clock (seconds); hs_solver_run advances it */
+HS_API int hs_solver_set_time(hs_solver* s, double t);
+HS_API double hs_solver_time(hs_solver* s);
+/* dTe/dt and dTp/dt at the current state and laser time t (testing) */
+HS_API int hs_solver_tm_rhs(hs_solver* s, double t, double* dTe_host, double* dTp_host);
+
 /* ------------------------------------------------ slots (continuous batching)
  * No counterpart in the reference (it steps one solve per process).  The
  * nsolve solves of a batch become slots that are filled, stepped and emptied
@@ -199,6 +237,7 @@ typedef struct {
     uint32_t accepted; /* accepted steps since admission */
     uint32_t home;     /* internal: buffer set that holds a finished state */
     uint32_t retry;    /* internal: last attempt was rejected */
+    double maxErrT;    /* two-temperature solves: error bound of Te and Tp (K) */
 } hs_slot;
 
 /* Fill slot `slot` (idle, done or failed) with a new solve: m_host is 3 x N

@@ -1241,6 +1241,128 @@ int or_llb_step_rk23(or_llb* s, double* m, double* dtio, double maxErr,
     return 1;
 }
 
+/* ------------------------------------------------------------------ two-temperature model */
+/* Qe summed in the order Qspat, Qe-p, Qlaser; Qspat = or_qspat with a
+ * uniform conductance (Qspat.cu:13-134), Qe-p = or_qinter (Qinter.cu:13-45). */
+void or_tm_rhs(const or_tm_params* q, const int N[3], const double cell[3], const int pbc[3], const double* Te,
+               const double* Tp, double t, double* dTe, double* dTp, double* work) {
+    long n = (long)N[0] * N[1] * N[2];
+    or_qspat(work, Te, NULL, q->kappa, N[0], N[1], N[2], cell[0], cell[1], cell[2], pbc[0], pbc[1], pbc[2]);
+    const double ir2 = q->sigma_r > 0 ? 1.0 / (2.0 * q->sigma_r * q->sigma_r) : 0.0;
+    const double it2 = q->sigma_t > 0 ? 1.0 / (2.0 * q->sigma_t * q->sigma_t) : 0.0;
+    const double d0 = t - q->t0;
+#pragma omp parallel for schedule(static)
+    for (long I = 0; I < n; I++) {
+        const int j = (int)((I / N[2]) % N[1]), k = (int)(I % N[2]);
+        const double x = (k + 0.5) * cell[2] - q->xc, y = (j + 0.5) * cell[1] - q->yc;
+        const double ql = q->Q0 * exp(-(x * x + y * y) * ir2) * exp(-(d0 * d0) * it2);
+        const double qe = work[I] + q->G * (Tp[I] - Te[I]) + ql;
+        dTe[I] = qe / (q->gammaE * Te[I]);
+        dTp[I] = q->G * (Te[I] - Tp[I]) / q->Cl;
+    }
+}
+
+static double maxabs_err(const double* k1, const double* k2, const double* k3, const double* k4, double h, long n) {
+    double e = 0;
+    for (long i = 0; i < n; i++) {
+        double v = h * ((-5.0 / 72.0) * k1[i] + (1.0 / 12.0) * k2[i] + (1.0 / 9.0) * k3[i] - (1.0 / 8.0) * k4[i]);
+        if (fabs(v) > e) e = fabs(v);
+    }
+    return e;
+}
+
+int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, double* Tp, double* t,
+                       double* dtio, double maxErr, double mindt, double maxdt, uint32_t step, double* stats) {
+    const long N = s->N, n3 = 3 * N;
+    const int maxTry = 10;
+    const double headRoom = 0.8;
+    double* buf = (double*)malloc(sizeof(double) * 14 * N);
+    double *Te0 = buf, *Tp0 = buf + N, *Tes = buf + 2 * N, *Tps = buf + 3 * N, *work = buf + 4 * N;
+    double* ke[4] = {buf + 5 * N, buf + 6 * N, buf + 7 * N, buf + 8 * N};
+    double* kp[4] = {buf + 9 * N, buf + 10 * N, buf + 11 * N, buf + 12 * N};
+    memcpy(Te0, Te, sizeof(double) * N);
+    memcpy(Tp0, Tp, sizeof(double) * N);
+    memcpy(s->y0, m, sizeof(double) * n3);
+    const double* Tsave = s->p.T;
+    const double TMsave = s->p.TMul;
+    s->p.TMul = 1.0;
+    int tr = 0, rc = 1;
+    double err[3] = {0, 0, 0};
+    const double cs[4] = {0.0, 0.5, 0.75, 1.0};
+    for (;;) {
+        const double dt = *dtio, h = dt;
+        memcpy(m, s->y0, sizeof(double) * n3);
+        /* stage 1 at (y0, t) */
+        s->p.T = Te0;
+        or_llb_rhs(s, m, s->k1, step, (uint32_t)tr, dt); /* normalises y0 */
+        if (tr == 0) memcpy(s->y0, m, sizeof(double) * n3);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te0, Tp0, *t + cs[0] * h, ke[0], kp[0], work);
+        /* stage 2 at y0 + h/2 k1 */
+        axpy_to(s->ys, m, 0.5 * h, s->k1, n3);
+        axpy_to(Tes, Te0, 0.5 * h, ke[0], N);
+        axpy_to(Tps, Tp0, 0.5 * h, kp[0], N);
+        s->p.T = Tes;
+        or_llb_rhs(s, s->ys, s->k2, step, (uint32_t)tr, dt);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[1] * h, ke[1], kp[1], work);
+        /* stage 3 at y0 + 3h/4 k2 */
+        axpy_to(s->ys, m, 0.75 * h, s->k2, n3);
+        axpy_to(Tes, Te0, 0.75 * h, ke[1], N);
+        axpy_to(Tps, Tp0, 0.75 * h, kp[1], N);
+        s->p.T = Tes;
+        or_llb_rhs(s, s->ys, s->k3, step, (uint32_t)tr, dt);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[2] * h, ke[2], kp[2], work);
+        /* y1 and stage 4 at (y1, t + h) */
+        for (long i = 0; i < n3; i++)
+            m[i] = m[i] + h * ((2.0 / 9.0) * s->k1[i] + (1.0 / 3.0) * s->k2[i] + (4.0 / 9.0) * s->k3[i]);
+        for (long i = 0; i < N; i++) {
+            Tes[i] = Te0[i] + h * ((2.0 / 9.0) * ke[0][i] + (1.0 / 3.0) * ke[1][i] + (4.0 / 9.0) * ke[2][i]);
+            Tps[i] = Tp0[i] + h * ((2.0 / 9.0) * kp[0][i] + (1.0 / 3.0) * kp[1][i] + (4.0 / 9.0) * kp[2][i]);
+        }
+        s->p.T = Tes;
+        or_llb_rhs(s, m, s->k4, step, (uint32_t)tr, dt); /* normalises y1 */
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[3] * h, ke[3], kp[3], work);
+        err[0] = maxabs_err(s->k1, s->k2, s->k3, s->k4, h, n3);
+        err[1] = maxabs_err(ke[0], ke[1], ke[2], ke[3], h, N);
+        err[2] = maxabs_err(kp[0], kp[1], kp[2], kp[3], h, N);
+        /* solve_rk12.go:150-190: bad if any equation exceeds its bound, smallest factor */
+        int bad = 0;
+        double minF = 2.0;
+        for (int e = 0; e < 3; e++) {
+            const double me = e ? q->maxErrT : maxErr;
+            if (err[e] > me) bad = 1;
+            double f = headRoom * pow(me / err[e], 1. / 3.);
+            if (f > 1.5) f = 1.5;
+            if (f < 0.1) f = 0.1;
+            if (f < minF) minF = f;
+        }
+        double newDt = dt * minF;
+        if (newDt < mindt) newDt = mindt;
+        if (newDt > maxdt) newDt = maxdt;
+        *dtio = newDt;
+        if (!bad || newDt == mindt) {
+            memcpy(Te, Tes, sizeof(double) * N);
+            memcpy(Tp, Tps, sizeof(double) * N);
+            *t += h;
+            break;
+        }
+        if (tr > maxTry) {
+            rc = -1;
+            break;
+        }
+        tr++;
+    }
+    s->p.T = Tsave;
+    s->p.TMul = TMsave;
+    free(buf);
+    if (stats) {
+        stats[0] = err[0];
+        stats[1] = err[1];
+        stats[2] = err[2];
+        stats[3] = tr + 1;
+    }
+    return rc;
+}
+
 /* EnergyUpdater.Update, modules/mod_energy.go:98-116 */
 double or_energy_term(const double* m, const double* H, const double mMul[3], const double HMul[3],
                       const double* msat, double msatMul, double weight, long N) {

@@ -184,6 +184,25 @@ int or_llb_step_rk12(or_llb* s, double* m, double* dt, double maxErr,
                      double mindt, double maxdt, uint32_t step, double* stats);
 int or_llb_step_rk23(or_llb* s, double* m, double* dt, double maxErr,
                      double mindt, double maxdt, uint32_t step, double* stats);
+/* Two-temperature model with laser source (TM.go dT/dt = Q/C; Qspat.go,
+ * Qinter.go; mod_ce.go Ce = gammaE Te):
+ *   Qe = kappa lap(Te) + G (Tp - Te) + Q0 exp(-r^2/(2 sr^2)) exp(-(t-t0)^2/(2 st^2))
+ *   dTe/dt = Qe / (gammaE Te),  dTp/dt = G (Te - Tp) / Cl
+ * r = in-plane distance of the cell centre (user x = internal axis 2, y = 1)
+ * from (xc, yc).  Written for the coupled LLB-r run of configs with a laser. */
+typedef struct {
+    double gammaE, Cl, G, kappa, Q0, sigma_r, sigma_t, t0, xc, yc, maxErrT;
+} or_tm_params;
+void or_tm_rhs(const or_tm_params* q, const int N[3], const double cell[3], const int pbc[3], const double* Te,
+               const double* Tp, double t, double* dTe, double* dTp, double* work);
+/* One adaptive RK23 step of (m, Te, Tp) with the LLB temperature T = Te per
+ * cell (the LLB's T mask and TMul are replaced), laser time *t advanced on
+ * acceptance; the reference's multi-equation controller (solve_rk12.go:150-190)
+ * with maxErr for m and q->maxErrT for Te and Tp.  stats[0..2] = errors of
+ * the accepted attempt (m, Te, Tp), stats[3] = attempts. */
+int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, double* Tp, double* t,
+                       double* dtio, double maxErr, double mindt, double maxdt, uint32_t step, double* stats);
+
 /* energy term: weight*Ms0T0Mul*sum((mMul.m).(HMul.H)*msat) -- mod_energy.go:98-116 */
 double or_energy_term(const double* m, const double* H, const double mMul[3], const double HMul[3],
                       const double* msat, double msatMul, double weight, long N);

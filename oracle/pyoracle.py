@@ -69,6 +69,9 @@ def _declare(L):
     L.or_llb_new.restype = v
     L.or_llb_free.argtypes = [v]
     L.or_llb_rhs.argtypes = [v, v, v, u32, u32, d]
+    L.or_tm_rhs.argtypes = [v, v, v, v, v, v, d, v, v, v]
+    L.or_llbtm_step_rk23.argtypes = [v, v, v, v, v, v, v, d, d, d, u32, v]
+    L.or_llbtm_step_rk23.restype = i
     L.or_llb_heff.argtypes = [v]
     L.or_llb_heff.restype = dp
     L.or_llb_bdemag.argtypes = [v]
@@ -181,6 +184,32 @@ class LLBParams(C.Structure):
     ]
 
 
+class TmParams(C.Structure):
+    """or_tm_params of oracle/hs_oracle.h (two-temperature model + laser)."""
+    _fields_ = [(n, C.c_double) for n in ("gammaE", "Cl", "G", "kappa", "Q0", "sigma_r", "sigma_t", "t0", "xc",
+                                          "yc", "maxErrT")]
+
+
+def tm_params(**kw):
+    q = TmParams()
+    for k, v in kw.items():
+        setattr(q, k, float(v))
+    return q
+
+
+def tm_rhs(q, N, cell, pbc, Te, Tp, t):
+    """dTe/dt, dTp/dt of the two-temperature model at time t."""
+    Te = f64(Te).reshape(-1)
+    Tp = f64(Tp).reshape(-1)
+    n = Te.size
+    dTe, dTp, work = np.empty(n), np.empty(n), np.empty(n)
+    Na = (C.c_int * 3)(*N)
+    ca = (C.c_double * 3)(*cell)
+    pa = (C.c_int * 3)(*pbc)
+    lib().or_tm_rhs(C.byref(q), Na, ca, pa, ptr(Te), ptr(Tp), float(t), ptr(dTe), ptr(dTp), ptr(work))
+    return dTe, dTp
+
+
 class LLB:
     """FP64 LLB-r problem: RHS assembly and the drivers (euler/heun/rk4/rk12/rk23)."""
 
@@ -247,6 +276,21 @@ class LLB:
         if rc < 0:
             raise RuntimeError("oracle adaptive step did not converge")
         return float(dtb[0]), st
+
+    def step_tm(self, q, m, Te, Tp, t, dt, step=0, maxErr=1e-5, mindt=1e-38, maxdt=1e38):
+        """One adaptive RK23 step of (m, Te, Tp) with T = Te (arrays modified in
+        place).  Returns (dt_next, t_next, stats[err_m, err_Te, err_Tp, attempts])."""
+        for a in (m, Te, Tp):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        dtb = np.array([dt], np.float64)
+        tb = np.array([t], np.float64)
+        st = np.zeros(4, np.float64)
+        rc = lib().or_llbtm_step_rk23(self._h, C.byref(q), ptr(m), ptr(Te), ptr(Tp), tb.ctypes.data,
+                                      dtb.ctypes.data, float(maxErr), float(mindt), float(maxdt), int(step),
+                                      st.ctypes.data)
+        if rc < 0:
+            raise RuntimeError("oracle adaptive step did not converge")
+        return float(dtb[0]), float(tb[0]), st
 
     def __del__(self):
         if getattr(self, "_h", None):

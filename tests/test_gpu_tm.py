@@ -1,0 +1,87 @@
+"""Laser-heated LLB-r coupled to the two-temperature model, GPU vs the FP64
+oracle: the Te/Tp right-hand side (FP64 on both sides, 1e-12 relative), the
+LLB right-hand side with T = Te (FP32 fields, 2e-5), and adaptive RK23 steps
+of (m, Te, Tp) replayed from the oracle's dt sequence (same accept/reject
+decisions, max|dm| <= 1e-5, |dTe|, |dTp| <= 1e-6 K)."""
+import math
+
+import numpy as np
+import pytest
+
+from hotspin_amd import workloads as W
+from hotspin_amd.api import TMSolver
+from hotspin_amd._lib import check
+from oracle import pyoracle as po
+from tests.gpu_util import relerr, require_gpu
+
+pytestmark = pytest.mark.gpu
+
+N, CELL = (2, 16, 32), (2e-9, 4e-9, 4e-9)
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    require_gpu()
+    yield
+    check("tm")
+
+
+def _problem():
+    gam, T0, Tmax, st = 6e3, 300.0, 900.0, 100e-15
+    tm = dict(gammaE=gam, Cl=3.8e6, G=1.8e18, kappa=90.0,
+              Q0=gam * (Tmax ** 2 - T0 ** 2) / (2 * st * math.sqrt(2 * math.pi)), sigma_r=30e-9, sigma_t=st,
+              t0=200e-15, xc=64e-9, yc=32e-9, maxErrT=1e-3)
+    kw = W.llbr_params(T=T0, seed=5, Bext=(0.0, 0.01, 0.02))
+    rng = np.random.default_rng(9)
+    ncell = N[0] * N[1] * N[2]
+    m = W.random_state(ncell, W.equilibrium_m(T0, 1044.0, 0.5), rng)
+    Te = T0 + 400 * rng.random(ncell)
+    Tp = T0 + 100 * rng.random(ncell)
+    return tm, kw, m, Te, Tp
+
+
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (0, 1, 1)])
+def test_tm_and_llb_rhs_match_oracle(pbc):
+    tm, kw, m, Te, Tp = _problem()
+    g = TMSolver(N, CELL, pbc, tm=tm, T0=300.0, **kw)
+    g.set_m(m)
+    g.set_T(Te, Tp)
+    q = po.tm_params(**tm)
+    for t in (0.0, 180e-15, 400e-15):
+        dte, dtp = g.tm_rhs(t)
+        ote, otp = po.tm_rhs(q, N, CELL, pbc, Te, Tp, t)
+        assert relerr(dte, ote) <= 1e-12
+        assert relerr(dtp, otp) <= 1e-12
+    kwo = dict(kw)
+    kwo.update(TMul=1.0, T=Te)
+    o = po.LLB(N, CELL, pbc, **kwo)
+    _, ko = o.rhs(m, step=4, attempt=0, dt=1e-15)
+    kg, Hg = g.rhs(step=4, attempt=0, dt=1e-15)
+    assert relerr(Hg, o.heff()) <= 2e-5
+    assert relerr(kg, ko) <= 2e-5
+
+
+def test_coupled_rk23_replays_oracle_steps():
+    tm, kw, m, _, _ = _problem()
+    T0 = 300.0
+    ncell = N[0] * N[1] * N[2]
+    Te, Tp = np.full(ncell, T0), np.full(ncell, T0)
+    q = po.tm_params(**tm)
+    o = po.LLB(N, CELL, **kw)
+    g = TMSolver(N, CELL, tm=tm, T0=T0, **kw)
+    mo, t, dt = m.copy(), 0.0, 1e-15
+    for i in range(30):
+        m_b, Te_b, Tp_b, t_b, dt_in = mo.copy(), Te.copy(), Tp.copy(), t, dt
+        dt, t, st = o.step_tm(q, mo, Te, Tp, t, dt, i, maxErr=1e-5, mindt=1e-20, maxdt=5e-14)
+        g.set_m(m_b)
+        g.set_T(Te_b, Tp_b)
+        g.time = t_b
+        dtg, stg = g.run("rk23", 1, dt_in, maxErr=1e-5, mindt=1e-20, maxdt=5e-14, step0=i)
+        assert stg.rejected == int(st[3]) - 1, (i, stg.rejected, st)
+        assert abs(dtg / dt - 1) <= 2e-2, (i, dtg, dt)
+        assert abs(g.time - t) <= 1e-9 * t
+        teg, tpg = g.get_T()
+        assert np.abs(g.get_m() - mo).max() <= 1e-5, i
+        assert np.abs(teg - Te).max() <= 1e-6, (i, np.abs(teg - Te).max())
+        assert np.abs(tpg - Tp).max() <= 1e-6, i
+    assert Te.max() > 450.0  # the pump has heated the spot
