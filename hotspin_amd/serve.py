@@ -47,7 +47,7 @@ class Server:
     // cells of the class, at least min_slots)."""
 
     def __init__(self, shapes=W.SERVE_SHAPES, cell=W.SERVE_CELL, material=W.MATERIAL_WIDE_T,
-                 hbm_budget=64 << 30, slots_cells=1 << 22, min_slots=4, max_slots=512, chunk=8, tol=1e-5,
+                 hbm_budget=64 << 30, slots_cells=1 << 21, min_slots=4, max_slots=64, chunk=8, tol=1e-5,
                  dt0=1e-15, mindt=1e-20, maxdt=1e-12, keep_results=True):
         self.shapes = list(shapes)
         self.cell = cell
@@ -83,7 +83,7 @@ class Server:
         return W.random_domains(n, 8, me, np.random.default_rng(1_000_000 + r["seed"]))
 
     def _admit(self, g, q, r):
-        m = self._initial_state(r)
+        m = r["m0"] if r.get("m0") is not None else self._initial_state(r)
         g.batch.admit(q, m, r["T"], r["seed"], r["steps"], self.dt0, maxErr=self.tol, mindt=self.mindt,
                       maxdt=self.maxdt, Bext=r["Bext"])
         g.req[q] = r
