@@ -3,6 +3,8 @@
 // threads NT) are template parameters so the plan can time variants.
 #pragma once
 
+#include <type_traits>
+
 #include "fft.hpp"
 
 // ------------------------------------------------------------------ tiling
@@ -83,6 +85,7 @@ __global__ void __launch_bounds__(NT) k_r2c_rows(const TI* __restrict__ x, const
     const int tid = threadIdx.x;
     long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
+    const bool pairs = (n2 & 1) == 0;  // rows of even length: each complex input is one aligned pair
     for (int e = tid; e < M * W; e += NT) {
         int p = e / M, n = e % M;
         long r = row0 + p;
@@ -91,8 +94,21 @@ __global__ void __launch_bounds__(NT) k_r2c_rows(const TI* __restrict__ x, const
             long base = r * n2;
             long mb = (r % mrows) * n2;
             int i0 = 2 * n, i1 = 2 * n + 1;
-            if (i0 < n2) v.x = T(x[base + i0]) * (msk ? T(msk[mb + i0]) : T(1));
-            if (i1 < n2) v.y = T(x[base + i1]) * (msk ? T(msk[mb + i1]) : T(1));
+            if (pairs) {
+                if (i0 < n2) {
+                    using V2 = typename std::conditional<sizeof(TI) == 4, float2, double2>::type;
+                    const V2 a = *(const V2*)(x + base + i0);
+                    v = {T(a.x), T(a.y)};
+                    if (msk) {
+                        const V2 w = *(const V2*)(msk + mb + i0);
+                        v.x *= T(w.x);
+                        v.y *= T(w.y);
+                    }
+                }
+            } else {
+                if (i0 < n2) v.x = T(x[base + i0]) * (msk ? T(msk[mb + i0]) : T(1));
+                if (i1 < n2) v.y = T(x[base + i1]) * (msk ? T(msk[mb + i1]) : T(1));
+            }
         }
         lds[n * W + p] = v;
     }
@@ -156,14 +172,25 @@ __global__ void __launch_bounds__(NT) k_c2r_rows(const cpx<T>* __restrict__ in, 
     }
     __syncthreads();
     fft_lds<T, M, W, NT, +1>(lds, twM, tid);
+    const bool pairs = (n2 & 1) == 0;
     for (int e = tid; e < M * W; e += NT) {
         int p = e / M, n = e % M;
         long r = row0 + p;
         if (r >= nrows) continue;
         cpx<T> v = lds[n * W + p];
         int i0 = 2 * n, i1 = 2 * n + 1;
-        if (i0 < n2) x[r * n2 + i0] = TO(v.x);
-        if (i1 < n2) x[r * n2 + i1] = TO(v.y);
+        if (pairs) {
+            if (i0 < n2) {
+                using V2 = typename std::conditional<sizeof(TO) == 4, float2, double2>::type;
+                V2 o;
+                o.x = TO(v.x);
+                o.y = TO(v.y);
+                *(V2*)(x + r * n2 + i0) = o;
+            }
+        } else {
+            if (i0 < n2) x[r * n2 + i0] = TO(v.x);
+            if (i1 < n2) x[r * n2 + i1] = TO(v.y);
+        }
     }
 }
 
