@@ -139,6 +139,51 @@ HS_API void tensSYMMVecMul(double* dstX, double* dstY, double* dstZ,
 /* ---- div.h:20 */
 HS_API void divAsync(double* dst, double* a, double* b, CUstream stream, int Npart);
 
+/* ---- divMulPow.h:21: dst = a/b * pow(c, p), the power taken in single
+ * precision as the reference does (divMulPow.cu:27). */
+HS_API void divMulPowAsync(double* dst, double* a, double* b, double* c, double p, CUstream stream, int Npart);
+
+/* ---- wavg.h:20-28 */
+HS_API void wavgAsync(double* dst, double* a, double* b, double* w0, double* w1, double* R,
+                      double w0Mul, double w1Mul, double RMul, CUstream stream, int Npart);
+
+/* ---- kappa.h:20-32 (MFA longitudinal susceptibility) */
+HS_API void kappaAsync(double* kappa, double* msat0, double* msat0T0, double* T, double* Tc, double* S,
+                       double* n, const double msat0Mul, const double msat0T0Mul, const double TcMul,
+                       const double SMul, int Npart, CUstream stream);
+
+/* ---- Ts.h:20-30 (spin temperature from |m|, inverse Brillouin) */
+HS_API void tsAsync(double* Ts, double* msat, double* msat0T0, double* Tc, double* S, const double msatMul,
+                    const double msat0T0Mul, const double TcMul, const double SMul, int Npart,
+                    CUstream stream);
+
+/* ---- Cp.h:22-28 (Debye lattice heat capacity) */
+HS_API void cpAsync(double* Cp, double* T, double* Td, double* n, const double TdMul, const int Npart,
+                    CUstream stream);
+
+/* ---- energy_flow.h:20-30 */
+HS_API void energyFlowAsync(double* w, double* mx, double* my, double* mz, double* Rx, double* Ry, double* Rz,
+                            double* Tc, double* S, double* n, double SMul, int Npart, CUstream stream);
+
+/* ---- uniaxialanisotropy.h:21-27 */
+HS_API void uniaxialAnisotropyAsync(double* hx, double* hy, double* hz, double* mx, double* my, double* mz,
+                                    double* Ku1_map, double* MSat_map, double Ku2_Mu0Msat_mul,
+                                    double* anisU_mapx, double anisU_mulx, double* anisU_mapy,
+                                    double anisU_muly, double* anisU_mapz, double anisU_mulz, CUstream stream,
+                                    int Npart);
+
+/* ---- llbar-nonlocal00nc.h:21-34 (-lambda_e Laplacian(H); the +1 neighbour
+ * along axis 1 uses the index i*N1*N2 + (j+1)*N2 + k, where the reference
+ * computes i*N1*N1 + (j+1)*N1 + k, llbar-nonlocal00nc.cu:183; both agree
+ * when N1 == N2). */
+HS_API void llbar_nonlocal00nc_async(double* tx, double* ty, double* tz, double* hx, double* hy, double* hz,
+                                     double* msat0T0, double* lambda_e_xx, double* lambda_e_yy,
+                                     double* lambda_e_zz, const double lambda_eMul_xx,
+                                     const double lambda_eMul_yy, const double lambda_eMul_zz, const int sx,
+                                     const int sy, const int sz, const double csx, const double csy,
+                                     const double csz, const int pbc_x, const int pbc_y, const int pbc_z,
+                                     CUstream stream);
+
 /* ---- dot.h:21-31 */
 HS_API void dotMaskAsync(double* dst,
                          double* ax, double* ay, double* az,

@@ -418,6 +418,202 @@ void or_qinter(double* Qi, const double* Ti, const double* Tj, const double* G, 
     }
 }
 
+/* ------------------------------------------------------------------ MFA helpers, anisotropy, nonlocal */
+/* kappaKern, src/libhotspin/kappa.cu:15-61.  n is its mask alone (no multiplier). */
+void or_kappa(double* kappa, const double* ms0Msk, const double* ms0T0Msk, const double* T,
+              const double* TcMsk, const double* SMsk, const double* nMsk,
+              double ms0Mul, double ms0T0Mul, double TcMul, double SMul, int N) {
+    for (int i = 0; i < N; i++) {
+        double u = mask1(ms0T0Msk, i);
+        double ms0T0 = ms0T0Mul * u;
+        double Temp = T[i];
+        if (ms0T0 == 0.0 || Temp == 0.0) { kappa[i] = 0.0; continue; }
+        double S = SMul * mask1(SMsk, i);
+        double Tc = TcMul * mask1(TcMsk, i);
+        double ms0 = ms0Mul * mask1(ms0Msk, i);
+        double preS = (S > OR_INFSPIN) ? 1.0 : S / (S + 1.0);
+        double J0 = 3.0 * Tc;
+        double n = mask1(nMsk, i);
+        double mul = u * u / (preS * J0 * n);
+        double me = ms0 / ms0T0;
+        double b = preS * J0 / Temp;
+        double f = b * ((S > OR_INFSPIN) ? dLdx(me * b) : dBjdx(S, me * b));
+        kappa[i] = mul * (f / (1.0 - f));
+    }
+}
+
+/* ModelTs + findroot_Ridders_Ts + tsKern, src/libhotspin/Ts.cu:13-120: the spin
+ * temperature T in [0, Tc] with B_S(pre/T) = m, pre = S^2 J0 m, J0 = 3Tc/(S(S+1)). */
+static double ts_model(double T, double J, double pre, double m) {
+    double x = (T == 0.0) ? 1.0e38 : pre / T;
+    return Bj(J, x) - m;
+}
+static double ts_ridders(double J, double pre, double m, double xa, double xb) {
+    double ya = ts_model(xa, J, pre, m);
+    if (fabs(ya) < OR_ZERO) return xa;
+    double yb = ts_model(xb, J, pre, m);
+    if (fabs(yb) < OR_ZERO) return xb;
+    double y1 = ya, x1 = xa, y2 = yb, x2 = xb;
+    double x = 1.0e10, y, tx = x, teps = x;
+    double x3, y3, dx, dy;
+    int iter = 0;
+    while (teps > OR_EPS && iter < 1000) {
+        x3 = 0.5 * (x2 + x1);
+        y3 = ts_model(x3, J, pre, m);
+        dy = (y3 * y3 - y1 * y2);
+        if (dy == 0.0) { x = x3; break; }
+        dx = (x3 - x1) * sgn(y1 - y2) * y3 / (sqrt(dy));
+        x = x3 + dx;
+        y = ts_model(x, J, pre, m);
+        int sy = signbit(y) != 0, sy3 = signbit(y3) != 0, sy1 = signbit(y1) != 0;
+        double ny2 = (sy == sy3) ? y2 : y3;
+        double nx2 = (sy == sy3) ? x2 : x3;
+        y2 = ny2; x2 = nx2;
+        ny2 = (sy == sy1 || x2 == x3) ? y2 : y1;
+        nx2 = (sy == sy1 || x2 == x3) ? x2 : x1;
+        y2 = ny2; x2 = nx2;
+        y1 = y; x1 = x;
+        teps = fabs((x - tx) / (tx + x));
+        tx = x;
+        iter++;
+    }
+    return x;
+}
+void or_ts(double* Ts, const double* msMsk, const double* ms0T0Msk, const double* TcMsk, const double* SMsk,
+           double msMul, double ms0T0Mul, double TcMul, double SMul, int N) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < N; i++) {
+        double ms0T0 = ms0T0Mul * mask1(ms0T0Msk, i);
+        if (ms0T0 == 0.0) { Ts[i] = 0.0; continue; }
+        double ms = msMul * mask1(msMsk, i);
+        if (ms == ms0T0) { Ts[i] = 0.0; continue; }
+        double Tc = TcMul * mask1(TcMsk, i);
+        if (ms == 0.0) { Ts[i] = Tc; continue; }
+        double S = SMul * mask1(SMsk, i);
+        double J0 = 3.0 * Tc / (S * (S + 1.0));
+        double m = ms / ms0T0;
+        double pre = S * S * J0 * m;
+        Ts[i] = ts_ridders(S, pre, m, 0.0, Tc);
+    }
+}
+
+/* Debye + cpKern, src/libhotspin/common_func.h:168-173, Cp.cu:14-83:
+ * n (T/Td)^3 int_0^{Td/T} x^4 e^x/(e^x-1)^2 dx by the 61-interval trapezoid rule
+ * (INTMAXSTEPS, common_func.h:27); below T/Td = 0.01 the T^3 limit 4 pi^4/15. */
+static double debye(double x) {
+    double e = exp(x);
+    double em1 = e - 1.0;
+    return (x <= OR_ZERO) ? 0.0 : (x * x * x * x * e) / (em1 * em1);
+}
+void or_cp(double* Cp, const double* T, const double* TdMsk, const double* nMsk, double TdMul, int N) {
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < N; i++) {
+        double n = mask1(nMsk, i);
+        double Td = TdMul * mask1(TdMsk, i);
+        double Temp = T[i];
+        if (Td == 0.0) { Cp[i] = 0.0; continue; }
+        double xx = Td / Temp, ixx = Temp / Td;
+        if (ixx < 0.01) {
+            Cp[i] = n * (12.0 / 5.0) * (1.0 / 9.0) * (pi * pi * pi * pi) * (ixx * ixx * ixx);
+            continue;
+        }
+        double h = xx / 61.0, inner = 0.0;
+        for (int j = 1; j < 61; j++) inner += debye(j * h);
+        double v = (debye(0.0) + debye(xx) + 2.0 * inner) * (0.5 * h);
+        Cp[i] = n * v / (xx * xx * xx);
+    }
+}
+
+/* energyFlowKern, src/libhotspin/energy_flow.cu:15-48.  Tc and n are masks alone. */
+void or_energy_flow(double* w, const double* mx, const double* my, const double* mz,
+                    const double* Rx, const double* Ry, const double* Rz,
+                    const double* TcMsk, const double* SMsk, const double* nMsk, double SMul, int N) {
+    for (int i = 0; i < N; i++) {
+        double S = SMul * mask1(SMsk, i), Tc = mask1(TcMsk, i), n = mask1(nMsk, i);
+        if (n == 0.0 || S == 0.0 || Tc == 0.0) { w[i] = 0.0; continue; }
+        double rW = mx[i] * Rx[i] + my[i] * Ry[i] + mz[i] * Rz[i];
+        w[i] = (Tc * S * S * n / (S * (S + 1.0))) * rW;
+    }
+}
+
+/* uniaxialAnisotropyKern, src/libhotspin/uniaxialanisotropy.cu:12-52: m.u with the
+ * normalized axis, the field along the axis as given. */
+void or_uniaxial_anisotropy(double* hx, double* hy, double* hz,
+                            const double* mx, const double* my, const double* mz,
+                            const double* KuMsk, const double* msMsk, double K2Mul,
+                            const double* uxMsk, double uxMul, const double* uyMsk, double uyMul,
+                            const double* uzMsk, double uzMul, int N) {
+    for (int i = 0; i < N; i++) {
+        double ms = mask1(msMsk, i);
+        if (ms == 0.0) { hx[i] = 0.0; hy[i] = 0.0; hz[i] = 0.0; continue; }
+        double K = (K2Mul / ms) * mask1(KuMsk, i);
+        double ux = uxMul * mask1(uxMsk, i), uy = uyMul * mask1(uyMsk, i), uz = uzMul * mask1(uzMsk, i);
+        double len = sqrt(ux * ux + uy * uy + uz * uz);
+        double inv = (len != 0.0) ? 1.0 / len : 0.0;
+        double mu = mx[i] * (ux * inv) + my[i] * (uy * inv) + mz[i] * (uz * inv);
+        hx[i] = K * mu * ux; hy[i] = K * mu * uy; hz[i] = K * mu * uz;
+    }
+}
+
+/* llbarNonlocal00ncKern, src/libhotspin/llbar-nonlocal00nc.cu:11-208:
+ * t_c = -(1/mu0) sum_d (lmul_c/cs_d^2) sum_+- avgGeom(l0_c, l+-_c)(mu0 H+-_c - mu0 H0_c),
+ * neighbour order X(i), Z(k), Y(j).  The +1 neighbour along j is addressed
+ * i*sy*sz + idx*sz + k (the reference's line 183 has i*sy*sy + idx*sy + k). */
+void or_llbar_nonlocal00nc(double* t[3], double* const H[3], const double* msk, double* const lam[3],
+                           const double lmul[3], int sx, int sy, int sz,
+                           double csx, double csy, double csz, int px, int py, int pz) {
+    double c2[3] = {1.0 / (csx * csx), 1.0 / (csy * csy), 1.0 / (csz * csz)};
+    int n[3] = {sx, sy, sz}, w[3] = {px, py, pz};
+    long str[3] = {(long)sy * sz, (long)sz, 1L};
+    const int order[3] = {0, 2, 1};
+    for (int i = 0; i < sx; i++)
+        for (int j = 0; j < sy; j++)
+            for (int k = 0; k < sz; k++) {
+                long I = i * str[0] + j * str[1] + k;
+                if (mask1(msk, I) == 0.0) { t[0][I] = 0.0; t[1][I] = 0.0; t[2][I] = 0.0; continue; }
+                int pos[3] = {i, j, k};
+                double R[3] = {0.0, 0.0, 0.0};
+                for (int q = 0; q < 3; q++) {
+                    int ax = order[q], c = pos[ax];
+                    int lo = c - 1; lo = (lo < 0 && w[ax]) ? n[ax] + lo : lo; if (lo < 0) lo = 0;
+                    int hi = c + 1; hi = (hi == n[ax] && w[ax]) ? hi - n[ax] : hi; if (hi > n[ax] - 1) hi = n[ax] - 1;
+                    long a1 = I + (long)(lo - c) * str[ax], a2 = I + (long)(hi - c) * str[ax];
+                    for (int cc = 0; cc < 3; cc++) {
+                        double l0 = mask1(lam[cc], I);
+                        double p1 = avg_geom_zero(l0, mask1(lam[cc], a1));
+                        double p2 = avg_geom_zero(l0, mask1(lam[cc], a2));
+                        double H0 = OR_MU0 * H[cc][I], H1 = OR_MU0 * H[cc][a1], H2 = OR_MU0 * H[cc][a2];
+                        R[cc] += (lmul[cc] * c2[ax]) * ((p1 * H1 + p2 * H2) - H0 * (p1 + p2));
+                    }
+                }
+                double pre = 1.0 / OR_MU0;
+                for (int cc = 0; cc < 3; cc++) t[cc][I] = -pre * R[cc];
+            }
+}
+
+/* divMulPowKern, src/libhotspin/divMulPow.cu:13-30: the power in single precision
+ * (powf on the values rounded to float, :27). */
+void or_div_mul_pow(double* dst, const double* a, const double* b, const double* c, double p, int N) {
+    for (int i = 0; i < N; i++) {
+        double bb = b ? b[i] : 1.0;
+        double val = (bb == 0.0) ? 0.0 : a[i] / bb;
+        double cc = c ? c[i] : 1.0;
+        cc = (cc == 0.0) ? 0.0 : (double)powf((float)cc, (float)p);
+        dst[i] = val * cc;
+    }
+}
+
+/* wavgKern + weightedAvgZero, src/libhotspin/wavg.cu:13-30, common_func.h:162-166 */
+void or_wavg(double* dst, const double* a, const double* b, const double* w0Msk, const double* w1Msk,
+             const double* RMsk, double w0Mul, double w1Mul, double RMul, int N) {
+    for (int i = 0; i < N; i++) {
+        double w0 = w0Mul * mask1(w0Msk, i), w1 = w1Mul * mask1(w1Msk, i), R = RMul * mask1(RMsk, i);
+        double x0 = a[i], x1 = b[i];
+        double den = w0 + w1 + 2.0 * R * sqrt(w0 * w1);
+        dst[i] = (den == 0.0) ? 0.0 : (w0 * x0 + w1 * x1 + 2.0 * R * sqrt(w0 * x0 * w1 * x1)) / den;
+    }
+}
+
 /* ------------------------------------------------------------------ linear algebra, add.cu */
 void or_madd(double* dst, const double* a, const double* b, double mulB, int N) {
     for (int i = 0; i < N; i++) dst[i] = b ? a[i] + mulB * b[i] : a[i] + mulB; /* add.cu:41-56,113-126 */

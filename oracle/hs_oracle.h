@@ -74,6 +74,26 @@ void or_qspat(double* Q, const double* T, const double* k, double kMul,
               int px, int py, int pz);
 void or_qinter(double* Qi, const double* Ti, const double* Tj, const double* G,
                double GMul, int N);
+void or_kappa(double* kappa, const double* ms0Msk, const double* ms0T0Msk, const double* T,
+              const double* TcMsk, const double* SMsk, const double* nMsk,
+              double ms0Mul, double ms0T0Mul, double TcMul, double SMul, int N);
+void or_ts(double* Ts, const double* msMsk, const double* ms0T0Msk, const double* TcMsk, const double* SMsk,
+           double msMul, double ms0T0Mul, double TcMul, double SMul, int N);
+void or_cp(double* Cp, const double* T, const double* TdMsk, const double* nMsk, double TdMul, int N);
+void or_energy_flow(double* w, const double* mx, const double* my, const double* mz,
+                    const double* Rx, const double* Ry, const double* Rz,
+                    const double* TcMsk, const double* SMsk, const double* nMsk, double SMul, int N);
+void or_uniaxial_anisotropy(double* hx, double* hy, double* hz,
+                            const double* mx, const double* my, const double* mz,
+                            const double* KuMsk, const double* msMsk, double K2Mul,
+                            const double* uxMsk, double uxMul, const double* uyMsk, double uyMul,
+                            const double* uzMsk, double uzMul, int N);
+void or_llbar_nonlocal00nc(double* t[3], double* const H[3], const double* msk, double* const lam[3],
+                           const double lmul[3], int sx, int sy, int sz,
+                           double csx, double csy, double csz, int px, int py, int pz);
+void or_div_mul_pow(double* dst, const double* a, const double* b, const double* c, double p, int N);
+void or_wavg(double* dst, const double* a, const double* b, const double* w0Msk, const double* w1Msk,
+             const double* RMsk, double w0Mul, double w1Mul, double RMul, int N);
 void or_madd(double* dst, const double* a, const double* b, double mulB, int N);
 void or_madd2(double* a, const double* b, double mulB, const double* c, double mulC, int N);
 void or_vec_madd(double* dst[3], double* const a[3], double* const b[3],

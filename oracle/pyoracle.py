@@ -41,6 +41,14 @@ def _declare(L):
     L.or_qspat.argtypes = [v] * 3 + [d] + [i] * 3 + [d] * 3 + [i] * 3
     L.or_qinter.argtypes = [v] * 4 + [d, i]
     L.or_tens_symm_vec_mul.argtypes = [v] * 12 + [d] + [i] * 3
+    L.or_kappa.argtypes = [v] * 7 + [d] * 4 + [i]
+    L.or_ts.argtypes = [v] * 5 + [d] * 4 + [i]
+    L.or_cp.argtypes = [v] * 4 + [d, i]
+    L.or_energy_flow.argtypes = [v] * 10 + [d, i]
+    L.or_uniaxial_anisotropy.argtypes = [v] * 8 + [d, v, d, v, d, v, d, i]
+    L.or_llbar_nonlocal00nc.argtypes = [v] * 5 + [i] * 3 + [d] * 3 + [i] * 3
+    L.or_div_mul_pow.argtypes = [v] * 4 + [d, i]
+    L.or_wavg.argtypes = [v] * 6 + [d] * 3 + [i]
     for name in ("or_sum", "or_max", "or_min", "or_maxabs"):
         getattr(L, name).argtypes = [v, C.c_long]
         getattr(L, name).restype = d
@@ -306,3 +314,85 @@ def exchange6(m, lex, N, pbc, msat0T0Mul, pre):
                        int(N[0]), int(N[1]), int(N[2]), int(pbc[0]), int(pbc[1]), int(pbc[2]),
                        float(msat0T0Mul), float(pre[0]), float(pre[1]), float(pre[2]))
     return h
+
+
+def _planes(a):
+    """(pointer array of the 3 planes, the planes) of a (3, n) float64 array."""
+    a = f64(a).reshape(3, -1)
+    return (C.c_void_p * 3)(*[p.ctypes.data for p in a]), a
+
+
+def kappa(T, Tc, S, n, msat0, msat0T0, msat0Mul, msat0T0Mul, TcMul, SMul):
+    """kappaAsync (kappa.cu:15-61); mask arguments may be None."""
+    T = f64(T)
+    out = np.empty_like(T)
+    lib().or_kappa(ptr(out), ptr(f64(msat0)), ptr(f64(msat0T0)), ptr(T), ptr(f64(Tc)), ptr(f64(S)), ptr(f64(n)),
+                   msat0Mul, msat0T0Mul, TcMul, SMul, T.size)
+    return out
+
+
+def ts(N, msat, msat0T0, Tc, S, msatMul, msat0T0Mul, TcMul, SMul):
+    """tsAsync (Ts.cu:84-120); mask arguments may be None, N cells."""
+    out = np.empty(N)
+    lib().or_ts(ptr(out), ptr(f64(msat)), ptr(f64(msat0T0)), ptr(f64(Tc)), ptr(f64(S)), msatMul, msat0T0Mul,
+                TcMul, SMul, N)
+    return out
+
+
+def cp(T, Td, n, TdMul):
+    """cpAsync (Cp.cu:14-83)."""
+    T = f64(T)
+    out = np.empty_like(T)
+    lib().or_cp(ptr(out), ptr(T), ptr(f64(Td)), ptr(f64(n)), TdMul, T.size)
+    return out
+
+
+def energy_flow(m, R, Tc, S, n, SMul):
+    """energyFlowAsync (energy_flow.cu:15-48); m, R of shape (3, N)."""
+    m, R = f64(m).reshape(3, -1), f64(R).reshape(3, -1)
+    out = np.empty(m.shape[1])
+    lib().or_energy_flow(ptr(out), ptr(m[0]), ptr(m[1]), ptr(m[2]), ptr(R[0]), ptr(R[1]), ptr(R[2]),
+                         ptr(f64(Tc)), ptr(f64(S)), ptr(f64(n)), SMul, out.size)
+    return out
+
+
+def uniaxial_anisotropy(m, Ku, Ms, K2Mul, u_masks, u_muls):
+    """uniaxialAnisotropyAsync (uniaxialanisotropy.cu:12-52)."""
+    m = f64(m).reshape(3, -1)
+    h = np.empty_like(m)
+    um = [f64(x) for x in u_masks]
+    lib().or_uniaxial_anisotropy(ptr(h[0]), ptr(h[1]), ptr(h[2]), ptr(m[0]), ptr(m[1]), ptr(m[2]), ptr(f64(Ku)),
+                                 ptr(f64(Ms)), K2Mul, ptr(um[0]), u_muls[0], ptr(um[1]), u_muls[1], ptr(um[2]),
+                                 u_muls[2], m.shape[1])
+    return h
+
+
+def llbar_nonlocal00nc(H, msat0T0, lam, lmul, N, cell, pbc):
+    """llbar_nonlocal00nc_async (llbar-nonlocal00nc.cu:11-208); lam: 3 masks or Nones."""
+    Hp, Ha = _planes(H)
+    t = np.empty_like(Ha)
+    tp = (C.c_void_p * 3)(*[p.ctypes.data for p in t])
+    lm = [f64(x) for x in lam]
+    lp = (C.c_void_p * 3)(*[ptr(x) for x in lm])
+    lmu = (C.c_double * 3)(*lmul)
+    lib().or_llbar_nonlocal00nc(tp, Hp, ptr(f64(msat0T0)), lp, lmu, int(N[0]), int(N[1]), int(N[2]),
+                                float(cell[0]), float(cell[1]), float(cell[2]), int(pbc[0]), int(pbc[1]),
+                                int(pbc[2]))
+    return t
+
+
+def div_mul_pow(a, b, c, p):
+    """divMulPowAsync (divMulPow.cu:13-30)."""
+    a = f64(a)
+    out = np.empty_like(a)
+    lib().or_div_mul_pow(ptr(out), ptr(a), ptr(f64(b)), ptr(f64(c)), p, a.size)
+    return out
+
+
+def wavg(a, b, w0, w1, R, w0Mul, w1Mul, RMul):
+    """wavgAsync (wavg.cu:13-30)."""
+    a = f64(a)
+    out = np.empty_like(a)
+    lib().or_wavg(ptr(out), ptr(a), ptr(f64(b)), ptr(f64(w0)), ptr(f64(w1)), ptr(f64(R)), w0Mul, w1Mul, RMul,
+                  a.size)
+    return out

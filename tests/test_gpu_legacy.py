@@ -299,3 +299,107 @@ def test_thermal_normals_match_oracle_stream():
     lib().hs_thermal_normals_f64(P(xi), n, 1234, 3, 17, 2, None)
     want = po.thermal_normals(1234, 3, 17, 2, n).ravel()
     assert np.abs(host(xi) - want).max() <= 1e-12
+
+
+def test_mfa_helpers_kappa_ts_cp():
+    n = 3000
+    rng = np.random.default_rng(50)
+    T = rng.uniform(1.0, 1200.0, n)
+    T[:3] = 0.0
+    ms0 = rng.uniform(0.05, 1.0, n)
+    ms0T0 = rng.uniform(0.5, 1.0, n)
+    ms0T0[5] = 0.0
+    Tc = rng.uniform(0.9, 1.1, n)
+    S = rng.choice([0.5, 1.0, 3.5], n)
+    nm = rng.uniform(0.5, 2.0, n)
+    want = po.kappa(T, Tc, S, nm, ms0, ms0T0, 1.0, 1.0, 1044.0, 1.0)
+    dT, dTc, dS, dn, dms0, dms0T0 = (dev(x) for x in (T, Tc, S, nm, ms0, ms0T0))
+    k = empty(n)
+    lib().kappaAsync(P(k), P(dms0), P(dms0T0), P(dT), P(dTc), P(dS), P(dn), 1.0, 1.0, 1044.0, 1.0, n, None)
+    # f/(1-f) amplifies rounding of dB/dx near f = 1 (T -> Tc)
+    assert relerr(host(k), want) <= 1e-10
+    # Ts: the inverse-Brillouin root; Ridders iterations keep the two within ~eps
+    ms = rng.uniform(0.0, 1.0, n)
+    ms[7] = 0.0
+    ms[8] = 1.0
+    want = po.ts(n, ms, None, Tc, S, 1.0, 1.0, 1044.0, 1.0)
+    dms = dev(ms)
+    t = empty(n)
+    lib().tsAsync(P(t), P(dms), None, P(dTc), P(dS), 1.0, 1.0, 1044.0, 1.0, n, None)
+    assert relerr(host(t), want) <= 1e-8
+    # Cp over both branches (T/Td below and above 0.01)
+    Tp = rng.uniform(0.5, 2000.0, n)
+    Td = rng.uniform(0.5, 1.5, n)
+    Td[11] = 0.0
+    want = po.cp(Tp, Td, nm, 470.0)
+    dTp, dTd = dev(Tp), dev(Td)
+    c = empty(n)
+    lib().cpAsync(P(c), P(dTp), P(dTd), P(dn), 470.0, n, None)
+    assert relerr(host(c), want) <= TOL
+
+
+def test_energy_flow_and_uniaxial_anisotropy():
+    n = 2500
+    m, R = _rand((3, n), 60), _rand((3, n), 61) * 1e12
+    Tc, S, nm = _rand(n, 62, 500, 1100), _rand(n, 63, 0, 4), _rand(n, 64, 0, 2)
+    S[4] = 0.0
+    nm[9] = 0.0
+    want = po.energy_flow(m, R, Tc, S, nm, 1.0)
+    dm, dR, dTc, dS, dn = dev(m), dev(R), dev(Tc), dev(S), dev(nm)
+    w = empty(n)
+    lib().energyFlowAsync(P(w), P(dm[0]), P(dm[1]), P(dm[2]), P(dR[0]), P(dR[1]), P(dR[2]), P(dTc), P(dS), P(dn),
+                          1.0, n, None)
+    assert relerr(host(w), want) <= TOL
+    Ku, Ms = _rand(n, 65, 0, 1), _rand(n, 66, 0.2, 1)
+    Ms[2] = 0.0
+    u = [_rand(n, 67 + c) for c in range(3)]
+    u[0][6] = u[1][6] = u[2][6] = 0.0
+    want = po.uniaxial_anisotropy(m, Ku, Ms, 2 * 5e5 / 1.7e6, u, (1.0, 0.5, 2.0))
+    dKu, dMs = dev(Ku), dev(Ms)
+    du = [dev(x) for x in u]
+    h = empty((3, n))
+    lib().uniaxialAnisotropyAsync(P(h[0]), P(h[1]), P(h[2]), P(dm[0]), P(dm[1]), P(dm[2]), P(dKu), P(dMs),
+                                  2 * 5e5 / 1.7e6, P(du[0]), 1.0, P(du[1]), 0.5, P(du[2]), 2.0, None, n)
+    assert relerr(host(h), want) <= TOL
+
+
+@pytest.mark.parametrize("N,pbc,masks", [((3, 5, 70), (0, 0, 0), True), ((4, 9, 130), (1, 1, 1), True),
+                                         ((2, 16, 8), (0, 1, 0), False), ((1, 3, 65), (1, 1, 1), True)])
+def test_llbar_nonlocal00nc(N, pbc, masks):
+    rng = np.random.default_rng(N[1] * 7 + N[2])
+    n = N[0] * N[1] * N[2]
+    H = rng.standard_normal((3, n)) * 1e5
+    msk = rng.uniform(0.2, 1.0, n)
+    msk[::13] = 0.0
+    lam = [rng.uniform(0.5, 1.5, n) for _ in range(3)] if masks else [None] * 3
+    lmul = (2e-19, 3e-19, 5e-19)
+    cell = (2e-9, 3e-9, 4e-9)
+    want = po.llbar_nonlocal00nc(H, msk, lam, lmul, N, cell, pbc)
+    dH, dmsk = dev(H), dev(msk)
+    dl = [dev(x) if x is not None else None for x in lam]
+    t = empty((3, n))
+    lib().llbar_nonlocal00nc_async(P(t[0]), P(t[1]), P(t[2]), P(dH[0]), P(dH[1]), P(dH[2]), P(dmsk),
+                                   *[P(x) for x in dl], *lmul, *N, *cell, *pbc, None)
+    assert relerr(host(t), want) <= TOL
+
+
+def test_div_mul_pow_and_wavg():
+    n = 4000
+    a, b, c = _rand(n, 70), _rand(n, 71, 0.1, 2), _rand(n, 72, 0.0, 3.0)
+    b[::17] = 0.0
+    c[::19] = 0.0
+    want = po.div_mul_pow(a, b, c, 1.7)
+    da, db, dc = dev(a), dev(b), dev(c)
+    d = empty(n)
+    lib().divMulPowAsync(P(d), P(da), P(db), P(dc), 1.7, None, n)
+    # single-precision power: the two powf implementations may differ by an ulp of float
+    assert relerr(host(d), want) <= 3e-7
+    lib().divMulPowAsync(P(d), P(da), None, None, 1.7, None, n)
+    assert relerr(host(d), po.div_mul_pow(a, None, None, 1.7)) <= TOL
+    x0, x1 = _rand(n, 73, 100, 1000), _rand(n, 74, 100, 1000)
+    w0, w1, R = _rand(n, 75, 0, 2), _rand(n, 76, 0, 2), _rand(n, 77, 0, 1)
+    w0[5] = w1[5] = 0.0
+    want = po.wavg(x0, x1, w0, w1, R, 6e3, 3.8e6, 0.5)
+    dx0, dx1, dw0, dw1, dR = dev(x0), dev(x1), dev(w0), dev(w1), dev(R)
+    lib().wavgAsync(P(d), P(dx0), P(dx1), P(dw0), P(dw1), P(dR), 6e3, 3.8e6, 0.5, None, n)
+    assert relerr(host(d), want) <= TOL
