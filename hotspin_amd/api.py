@@ -17,8 +17,9 @@ import numpy as np
 from . import _lib
 from ._lib import HotspinError, check, lib
 
-EULER, HEUN, RK4, RK12, RK23 = 0, 1, 2, 3, 4
-METHODS = {"euler": EULER, "heun": HEUN, "rk4": RK4, "rk12": RK12, "rk23": RK23}
+EULER, HEUN, RK4, RK12, RK23, AM12, BDF_EULER = 0, 1, 2, 3, 4, 5, 6
+METHODS = {"euler": EULER, "heun": HEUN, "rk4": RK4, "rk12": RK12, "rk23": RK23, "am12": AM12,
+           "bdf-euler": BDF_EULER}
 
 
 class LLBParams(C.Structure):
@@ -41,7 +42,7 @@ class LLBParams(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("steps", C.c_long), ("rejected", C.c_long), ("rhs_evals", C.c_long),
-                ("t", C.c_double), ("last_err", C.c_double), ("dt", C.c_double)]
+                ("t", C.c_double), ("last_err", C.c_double), ("dt", C.c_double), ("iterations", C.c_long)]
 
 
 def _f64(a):
@@ -121,6 +122,15 @@ class Solver:
         if rc:
             raise HotspinError("hs_solver_run_graph returned %d" % rc)
         self.step_count = s0 + int(nsteps)
+
+    def set_implicit(self, maxAbsErr=1e-4, maxRelErr=1e-3, maxIterErr=1e-5, maxIter=3):
+        """Tolerances of "am12" (m_maxAbsError, m_maxRelError) and "bdf-euler"
+        (m_maxIterError, m_maxIterations); the defaults are the reference's."""
+        rc = lib().hs_solver_set_implicit(self._h, float(maxAbsErr), float(maxRelErr), float(maxIterErr),
+                                          int(maxIter))
+        check("hs_solver_set_implicit")
+        if rc:
+            raise HotspinError("hs_solver_set_implicit returned %d" % rc)
 
     def run(self, method, nsteps, dt, maxErr=1e-5, mindt=1e-38, maxdt=1e38, step0=None):
         meth = METHODS[method] if isinstance(method, str) else int(method)

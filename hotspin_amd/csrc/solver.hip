@@ -885,6 +885,71 @@ __global__ void k_rk12_predict(const double* y0, const float* k1, double* acc, f
     for (int c = 0; c < 3; c++) mout[c * N + I] = (float)v[c];
 }
 
+// Fixed-point update of the implicit drivers (solve_am01.go:118-140 and
+// 188-206, solve_euler_backward.go:66-78): from the RHS k of the current
+// iterate y (acc; normalised first when norm_in is set: the reference's RHS
+// update normalises m in place, so every iterate whose RHS was evaluated is
+// compared normalised),
+//   PRED (dy0 == null): y_new = y0 + c k          (Madd)
+//   TRAP:               y_new = y0 + c (k + dy0)  (AddMadd)
+// acc <- y_new, mout <- the FP32 stage input of y_new.  Maxima over every
+// cell of every solve: red[p] = |y_p - y_new,p|, red[3+p] = |y_new,p|,
+// red[6+p] = |y_new,p - y1,p| (y1 given), red[9] = |y + y_new| over all
+// components, red[10] = 1 when a non-finite value appeared.  Each maximum
+// has kImpLanes lanes (128-byte apart) that blocks spread their atomics over.
+constexpr int kImpVals = 11, kImpLanes = 16, kImpStride = 16;
+__global__ void __launch_bounds__(256) k_imp(const double* __restrict__ y0, double* __restrict__ acc,
+                                             const float* __restrict__ k, const float* __restrict__ dy0,
+                                             const double* __restrict__ y1, float* __restrict__ mout, double c,
+                                             int norm_in, int norm_out, unsigned long long* __restrict__ red,
+                                             long N) {
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long b = (long)blockIdx.y * 3 * N;
+    double r[kImpVals];
+    for (int v = 0; v < kImpVals; v++) r[v] = 0.0;
+    if (I < N) {
+        double yc[3], yn[3];
+        for (int p = 0; p < 3; p++) yc[p] = acc[b + p * N + I];
+        if (norm_in) {
+            double l = sqrt(yc[0] * yc[0] + yc[1] * yc[1] + yc[2] * yc[2]);
+            double s = (l == 0.0) ? 0.0 : 1.0 / l;
+            for (int p = 0; p < 3; p++) yc[p] *= s;
+        }
+        for (int p = 0; p < 3; p++) {
+            const long a = b + p * N + I;
+            const double kv = (double)k[a];
+            yn[p] = dy0 ? y0[a] + c * (kv + (double)dy0[a]) : y0[a] + c * kv;
+            r[p] = fabs(yc[p] - yn[p]);
+            r[3 + p] = fabs(yn[p]);
+            if (y1) r[6 + p] = fabs(yn[p] - y1[a]);
+            r[9] = fmax(r[9], fabs(yc[p] + yn[p]));
+            if (!isfinite(yn[p]) || !isfinite(yc[p])) r[10] = 1.0;
+            acc[a] = yn[p];
+        }
+        double w[3] = {yn[0], yn[1], yn[2]};
+        if (norm_out) {
+            double l = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+            double s = (l == 0.0) ? 0.0 : 1.0 / l;
+            for (int p = 0; p < 3; p++) w[p] *= s;
+        }
+        for (int p = 0; p < 3; p++) mout[b + p * N + I] = (float)w[p];
+    }
+    __shared__ double wm[4][kImpVals];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int v = 0; v < kImpVals; v++) {
+        double x = r[v];
+        for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+        if (lane == 0) wm[wave][v] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kImpVals) {
+        const int v = threadIdx.x;
+        const double x = fmax(fmax(wm[0][v], wm[1][v]), fmax(wm[2][v], wm[3][v]));
+        const unsigned ln = (blockIdx.x + gridDim.x * blockIdx.y) % kImpLanes;
+        atomicMax(red + ((long)v * kImpLanes + ln) * kImpStride, (unsigned long long)__double_as_longlong(x));
+    }
+}
+
 __global__ void k_set_state(const double* y, float* mf, int normalize, long N) {
     long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
@@ -1192,6 +1257,12 @@ struct hs_solver {
     double *Tea = nullptr, *Tpa = nullptr, *Teb = nullptr, *Tpb = nullptr;
     float *Tfa = nullptr, *Tfb = nullptr;
     double *k1e = nullptr, *k1p = nullptr, *k2e = nullptr, *k2p = nullptr, *ee = nullptr, *ep = nullptr;
+    // implicit drivers (AM12, backward Euler): fixed-point maxima, the predictor
+    // result y1, the per-equation tolerances (reference defaults)
+    unsigned long long* imp = nullptr;
+    double* y1 = nullptr;
+    double imp_abs = 1e-4, imp_rel = 1e-3, imp_iter_err = 1e-5;
+    int imp_max_iter = 3;
 };
 
 namespace {
@@ -1765,7 +1836,7 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
     if (s->slot_host) hipHostFree(s->slot_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
-                  s->errmax, s->sp, s->clk, s->live};
+                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
@@ -1866,6 +1937,257 @@ extern "C" int hs_solver_run_graph(hs_solver* s, long nsteps, long step0, double
     return hs_last_error();
 }
 
+// ---------------------------------------------------------------- implicit drivers
+namespace {
+struct ImpRed {
+    double d[3], a[3], d1[3], sum;
+    bool bad;
+};
+
+bool ensure_imp(hs_solver* s) {
+    if (!s->imp &&
+        hipMalloc(&s->imp, sizeof(unsigned long long) * kImpVals * kImpLanes * kImpStride) != hipSuccess)
+        s->imp = nullptr;
+    if (!s->y1 && hipMalloc(&s->y1, sizeof(double) * 3 * s->N * s->nsolve) != hipSuccess) s->y1 = nullptr;
+    if (!s->imp || !s->y1) hs::set_error(HS_ENOMEM, "implicit driver: out of device memory");
+    return s->imp && s->y1;
+}
+
+// One fixed-point update of every solve: the RHS of the iterate (FP32 stage
+// input mB, demag first) into k_out, unless k_in already holds the slope,
+// then k_imp; the maxima come back to the host (one sync, as the reference's
+// reductor does per update).
+ImpRed imp_update(hs_solver* s, const float* k_in, float* k_out, const float* dy0, const double* y1, double c,
+                  bool norm_in, uint32_t step, uint32_t attempt, double dt) {
+    if (!k_in) {
+        demag_into(s, s->mB, s->Bs);
+        StageIO io = base_io(s, dt, step, attempt);
+        io.m = s->mB;
+        io.B = s->Bs;
+        io.kout = k_out;
+        io.hout = nullptr;
+        io.mout = nullptr;
+        launch_stage<R_RHS>(s, io);
+        k_in = k_out;
+    }
+    const long nred = (long)kImpVals * kImpLanes * kImpStride;
+    hipMemsetAsync(s->imp, 0, sizeof(unsigned long long) * nred, s->st);
+    hipLaunchKernelGGL(k_imp, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->y0, s->acc, k_in, dy0,
+                       y1, s->mB, c, norm_in ? s->p.normalize : 0, s->p.normalize, s->imp, s->N);
+    std::vector<unsigned long long> b(nred);
+    hipMemcpyAsync(b.data(), s->imp, sizeof(unsigned long long) * nred, hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    double v[kImpVals];
+    for (int q = 0; q < kImpVals; q++) {
+        v[q] = 0.0;
+        for (int l = 0; l < kImpLanes; l++) {
+            double x;
+            memcpy(&x, &b[((long)q * kImpLanes + l) * kImpStride], sizeof(x));
+            v[q] = std::max(v[q], x);
+        }
+    }
+    ImpRed r;
+    for (int p = 0; p < 3; p++) {
+        r.d[p] = v[p];
+        r.a[p] = v[3 + p];
+        r.d1[p] = v[6 + p];
+    }
+    r.sum = v[9];
+    r.bad = v[10] != 0.0;
+    return r;
+}
+
+// solve_am01.go:122-132: (1/sqrt(3)) sqrt(sum_p (maxdiff_p / (absErr + maxabs_p relErr))^2)
+double am_norm(const double d[3], const double a[3], double absErr, double relErr) {
+    double t = 0.0;
+    for (int p = 0; p < 3; p++) t += pow(d[p] / (absErr + a[p] * relErr), 2.0);
+    return (1.0 / sqrt(3.0)) * sqrt(t);
+}
+
+int imp_fail(hs_solver* s, const char* what, double dt, double* dtio, hs_run_stats* stats, const hs_run_stats& st) {
+    hs::set_error(HS_ECONV, what);
+    *dtio = dt;
+    if (stats) *stats = st;
+    return HS_ECONV;
+}
+
+// Explicit-Euler start of every step: dy0 = f(y0) into e (y0 normalised in
+// place with the normaliser), acc = y0 + dt dy0, mB its stage input.
+void implicit_start(hs_solver* s, double dt, uint32_t step) {
+    ensure_b0(s);
+    StageIO io = base_io(s, dt, step, 0);
+    io.m = s->mA;
+    io.B = s->B0;
+    io.mout = s->mB;
+    launch_stage<R_RK12_1>(s, io);
+}
+
+void implicit_accept(hs_solver* s) {
+    std::swap(s->y0, s->acc);
+    std::swap(s->mA, s->mB);
+    drop_graphs(s);
+    s->b0_valid = false;
+}
+
+// BDFAM12.Step, engine/solve_am01.go:45-300, one equation (m) for the whole
+// batch: the norms take the maxima over every solve.  Noise attempt a counts
+// the dt changes within a step.
+int run_am12(hs_solver* s, long nsteps, long step0, double* dtio, double mindt, double maxdt, hs_run_stats* stats) {
+    const int maxTry = 10, maxIter = 5;
+    const double headRoom = 0.8, maxIterErr = 0.1, alpha_ref = 0.6;
+    const double ab = s->imp_abs, rl = s->imp_rel;
+    hs_run_stats st{};
+    double dt = *dtio;
+    for (long n = 0; n < nsteps; n++) {
+        const uint32_t step = (uint32_t)(step0 + n);
+        uint32_t a = 0;
+        implicit_start(s, dt, step);
+        st.rhs_evals++;
+        int tr = 0;
+        bool restrict_step = false, first = true;
+        double saved_alpha = 0.0, tErr = 0.0;
+        for (;;) {
+            const double h = dt;
+            if (!first)
+                hipLaunchKernelGGL(k_rk12_predict, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->y0,
+                                   s->e, s->acc, s->mB, h, s->p.normalize, s->N);
+            first = false;
+            st.iterations++;
+            // predictor: implicit Euler, y <- y0 + h f(y); the first slope f(y_EE) kept in k1
+            double er = maxIterErr, alpha = 0.0;
+            bool badIt = false;
+            ImpRed r{};
+            for (int iter = 0;;) {
+                r = imp_update(s, nullptr, iter == 0 ? s->k1 : s->k2, nullptr, nullptr, h, true, step, a, dt);
+                st.rhs_evals++;
+                if (r.bad) return imp_fail(s, "hs_solver_run: AM12 iterate is not finite", dt, dtio, stats, st);
+                const double t = am_norm(r.d, r.a, ab, rl);
+                alpha = t / er;
+                saved_alpha = alpha;
+                er = t;
+                iter++;
+                st.iterations++;
+                if (er <= maxIterErr) break;
+                if (alpha >= 1.0 || iter > maxIter) {
+                    badIt = true;
+                    break;
+                }
+            }
+            if (badIt && tr == maxTry - 1)
+                return imp_fail(s, "hs_solver_run: AM12 predictor iteration did not converge", dt, dtio, stats, st);
+            if (badIt) {
+                dt = (alpha > alpha_ref) ? dt * pow(alpha_ref / alpha, 0.5) : 0.5 * dt;
+                restrict_step = true;
+                a++;
+                st.rejected++;
+                continue;
+            }
+            double a1[3] = {r.a[0], r.a[1], r.a[2]};  // max|y1_p|
+            hipMemcpyAsync(s->y1, s->acc, sizeof(double) * 3 * s->N * s->nsolve, hipMemcpyDeviceToDevice, s->st);
+            // corrector: trapezoidal, y <- y0 + h/2 (f(y) + dy0), starting from the slope f(y_EE)
+            er = maxIterErr;
+            for (int iter = 0;;) {
+                r = imp_update(s, iter == 0 ? s->k1 : nullptr, s->k2, s->e, s->y1, 0.5 * h, iter > 0, step, a, dt);
+                if (iter > 0) st.rhs_evals++;
+                if (r.bad) return imp_fail(s, "hs_solver_run: AM12 iterate is not finite", dt, dtio, stats, st);
+                const double t = am_norm(r.d, r.a, ab, rl);
+                alpha = t / er;
+                er = t;
+                iter++;
+                st.iterations++;
+                if (er <= maxIterErr) break;
+                if (alpha >= 1.0 || iter > maxIter) {
+                    badIt = true;
+                    break;
+                }
+            }
+            if (badIt && tr == maxTry - 1)
+                return imp_fail(s, "hs_solver_run: AM12 corrector iteration did not converge", dt, dtio, stats, st);
+            if (badIt) {
+                dt = (alpha > alpha_ref) ? dt * pow(alpha_ref / alpha, 0.5) : 0.5 * dt;
+                a++;
+                st.rejected++;
+                continue;
+            }
+            // step error: corrector - predictor, scaled by the predictor
+            tErr = am_norm(r.d1, a1, ab, rl);
+            const bool bad = tErr > 1.0;
+            double nd = dt * pow(headRoom / tErr, 0.5);
+            if (restrict_step) {
+                nd = std::min(nd, dt * pow(alpha_ref / saved_alpha, 0.5));
+                restrict_step = false;
+            }
+            nd = std::max(nd, mindt);
+            nd = std::min(nd, maxdt);
+            const double used = dt;
+            dt = nd;
+            if (!bad || nd == mindt) {
+                st.t += used;
+                break;
+            }
+            if (tr > maxTry) return imp_fail(s, "hs_solver_run: AM12 did not converge", dt, dtio, stats, st);
+            tr++;
+            a++;
+            st.rejected++;
+        }
+        implicit_accept(s);
+        st.steps++;
+        st.last_err = tErr;
+    }
+    *dtio = dt;
+    st.dt = dt;
+    if (stats) *stats = st;
+    return hs_last_error();
+}
+
+// BDFEuler.Step, engine/solve_euler_backward.go:29-86, one equation (m) for
+// the whole batch: y <- y0 + dt f(y) until max|y - yb| / max|y + yb| <=
+// iterErr over every value; more than maxIter updates fail.
+int run_bdf_euler(hs_solver* s, long nsteps, long step0, double* dtio, hs_run_stats* stats) {
+    hs_run_stats st{};
+    const double dt = *dtio;
+    for (long n = 0; n < nsteps; n++) {
+        const uint32_t step = (uint32_t)(step0 + n);
+        implicit_start(s, dt, step);
+        st.rhs_evals++;
+        st.iterations++;
+        double err = 1.0e38;
+        int iter = 0;
+        while (err > s->imp_iter_err) {
+            ImpRed r = imp_update(s, nullptr, s->k2, nullptr, nullptr, dt, true, step, 0, dt);
+            st.rhs_evals++;
+            if (r.bad) return imp_fail(s, "hs_solver_run: backward-Euler iterate is not finite", dt, dtio, stats, st);
+            err = std::max(r.d[0], std::max(r.d[1], r.d[2]));
+            if (r.sum > 0.0) err = err / r.sum;
+            iter++;
+            st.iterations++;
+            if (iter > s->imp_max_iter)
+                return imp_fail(s, "hs_solver_run: backward-Euler iteration did not converge", dt, dtio, stats, st);
+        }
+        implicit_accept(s);
+        st.steps++;
+        st.t += dt;
+        st.last_err = err;
+    }
+    st.dt = dt;
+    if (stats) *stats = st;
+    return hs_last_error();
+}
+}  // namespace
+
+extern "C" int hs_solver_set_implicit(hs_solver* s, double maxAbsErr, double maxRelErr, double maxIterErr,
+                                      int maxIter) {
+    if (!s || !(maxAbsErr > 0.0) || !(maxRelErr > 0.0) || !(maxIterErr > 0.0) || maxIter < 1) {
+        hs::set_error(HS_EINVAL, "hs_solver_set_implicit: tolerances must be positive");
+        return HS_EINVAL;
+    }
+    s->imp_abs = maxAbsErr;
+    s->imp_rel = maxRelErr;
+    s->imp_iter_err = maxIterErr;
+    s->imp_max_iter = maxIter;
+    return 0;
+}
+
 extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, double* dtio, double maxErr,
                              double mindt, double maxdt, hs_run_stats* stats) {
     if (!s || !dtio || nsteps < 0) return HS_EINVAL;
@@ -1874,6 +2196,14 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
     if (s->tm) {
         hs::set_error(HS_EINVAL, "hs_solver_run: two-temperature solves take HS_RK23");
         return HS_EINVAL;
+    }
+    if (method == HS_AM12 || method == HS_BDF_EULER) {
+        if (!ensure_imp(s)) return HS_ENOMEM;
+        int rc = method == HS_AM12 ? run_am12(s, nsteps, step0, dtio, mindt, maxdt, stats)
+                                   : run_bdf_euler(s, nsteps, step0, dtio, stats);
+        hipStreamSynchronize(s->st);
+        HS_LAUNCH_CHECK("hs_solver_run");
+        return rc ? rc : hs_last_error();
     }
     hs_run_stats st{};
     double dt = *dtio;

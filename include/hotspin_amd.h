@@ -124,6 +124,8 @@ typedef struct {
 #define HS_RK4 2
 #define HS_RK12 3
 #define HS_RK23 4
+#define HS_AM12 5      /* solver/am12: implicit-Euler predictor + trapezoidal corrector, adaptive */
+#define HS_BDF_EULER 6 /* solver/bdf-euler: fixed-step backward Euler */
 
 typedef struct {
     long steps;          /* accepted steps */
@@ -132,6 +134,7 @@ typedef struct {
     double t;            /* simulated time advanced */
     double last_err;     /* error estimate of the last accepted step */
     double dt;           /* dt for the next step */
+    long iterations;     /* fixed-point iterations (implicit drivers) */
 } hs_run_stats;
 
 typedef struct hs_solver hs_solver;
@@ -159,6 +162,16 @@ HS_API int hs_solver_get_m(hs_solver* s, double* m_host);
  * step (it keys the thermal noise).  stats may be NULL. */
 HS_API int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, double* dt, double maxErr,
                          double mindt, double maxdt, hs_run_stats* stats);
+/* Tolerances of the implicit drivers, the reference's per-equation
+ * quantities: AM12 m_maxAbsError and m_maxRelError (defaults 1e-4, 1e-3,
+ * engine/solve_am01.go:380-384), backward Euler m_maxIterError and
+ * m_maxIterations (1e-5, 3, engine/solve_euler_backward.go:128-129).
+ * hs_solver_run with HS_AM12 adapts *dt like solve_am01.go:45-300 (its
+ * maxErr argument is unused); HS_BDF_EULER keeps *dt.  Both treat a batch as
+ * one system (maxima over every solve) and fail with HS_ECONV where the
+ * reference panics.  Not for two-temperature solves. */
+HS_API int hs_solver_set_implicit(hs_solver* s, double maxAbsErr, double maxRelErr, double maxIterErr,
+                                  int maxIter);
 /* One RHS evaluation at the current state (no step): k and H_eff to host. */
 HS_API int hs_solver_rhs(hs_solver* s, double* k_host, double* heff_host, uint32_t step, uint32_t attempt,
                          double dt);

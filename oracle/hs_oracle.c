@@ -1437,6 +1437,157 @@ int or_llb_step_rk23(or_llb* s, double* m, double* dtio, double maxErr,
     return 1;
 }
 
+/* ------------------------------------------------------------------ implicit drivers */
+/* Error of one fixed-point update (solve_am01.go:122-132): with the maxima
+ * over cells per component p,
+ *   tErr = (1/sqrt(3)) sqrt(sum_p (max|y_p - yb_p| / (absErr + max|yb_p| relErr))^2). */
+static double am_norm(const double* y, const double* yb, long N, double absErr, double relErr) {
+    double t = 0.0;
+    for (int p = 0; p < 3; p++) {
+        double d = or_maxdiff(y + p * N, yb + p * N, N);
+        double mx = or_maxabs(yb + p * N, N);
+        t += pow(d / (absErr + mx * relErr), 2.0);
+    }
+    return (1.0 / sqrt(3.0)) * sqrt(t);
+}
+
+/* BDFAM12.Step, engine/solve_am01.go:45-300, for the single equation m:
+ * explicit-Euler start, implicit-Euler predictor and trapezoidal corrector,
+ * each iterated to a fixed point (tErr <= 0.1, at most 6 updates, stop when
+ * the contraction alpha = tErr/tErr_prev reaches 1); the step error is the
+ * corrector-predictor difference; dt *= (0.8/err)^(1/2), restricted to
+ * (0.6/alpha)^(1/2) after an iterator failure; at most 10 retries.  Noise
+ * attempt a increments whenever dt changes within the step (the reference
+ * redraws H_therm when dt changes, mod_brown.go:71).
+ * stats: [0] step error, [1] attempts (rejected steps + 1), [2] fixed-point
+ * iterations, [3] RHS evaluations, [4] dt of the accepted step. */
+int or_llb_step_am12(or_llb* s, double* m, double* dtio, double absErr, double relErr,
+                     double mindt, double maxdt, uint32_t step, double* stats) {
+    const long N = s->N, n3 = 3 * N;
+    const int maxTry = 10, maxIter = 5;
+    const double headRoom = 0.8, maxIterErr = 0.1, alpha_ref = 0.6;
+    double *dy0 = s->k1, *dy = s->k2, *dyE = s->k3, *y1 = s->k4, *yb = s->ys;
+    uint32_t a = 0;
+    long evals = 0, iterations = 0;
+    or_llb_rhs(s, m, dy0, step, a, *dtio); /* normalises m = y0 */
+    evals++;
+    memcpy(s->y0, m, sizeof(double) * n3);
+    int tr = 0, restrict_step = 0;
+    double saved_alpha = 0.0, tErr = 0.0, used_dt = 0.0;
+    for (;;) {
+        double dt = *dtio, h = dt;
+        int badIt = 0;
+        double er, alpha = 0.0;
+        for (long i = 0; i < n3; i++) m[i] = s->y0[i] + h * dy0[i];
+        iterations++;
+        or_llb_rhs(s, m, dy, step, a, dt);
+        evals++;
+        memcpy(dyE, dy, sizeof(double) * n3);
+        er = maxIterErr;
+        for (int iter = 0;;) {
+            for (long i = 0; i < n3; i++) yb[i] = s->y0[i] + h * dy[i];
+            double t = am_norm(m, yb, N, absErr, relErr);
+            alpha = t / er;
+            saved_alpha = alpha;
+            er = t;
+            memcpy(m, yb, sizeof(double) * n3);
+            iter++;
+            iterations++;
+            if (er <= maxIterErr) break;
+            if (alpha >= 1.0 || iter > maxIter) { badIt = 1; break; }
+            or_llb_rhs(s, m, dy, step, a, dt);
+            evals++;
+        }
+        if (badIt && tr == maxTry - 1) return -1;
+        if (badIt) {
+            double ha = 0.5 * dt;
+            if (alpha > alpha_ref) ha = dt * pow(alpha_ref / alpha, 0.5);
+            *dtio = ha;
+            restrict_step = 1;
+            a++;
+            continue;
+        }
+        memcpy(y1, m, sizeof(double) * n3);
+        memcpy(dy, dyE, sizeof(double) * n3);
+        er = maxIterErr;
+        for (int iter = 0;;) {
+            for (long i = 0; i < n3; i++) yb[i] = s->y0[i] + (0.5 * h) * (dy[i] + dy0[i]);
+            double t = am_norm(m, yb, N, absErr, relErr);
+            alpha = t / er;
+            er = t;
+            memcpy(m, yb, sizeof(double) * n3);
+            iter++;
+            iterations++;
+            if (er <= maxIterErr) break;
+            if (alpha >= 1.0 || iter > maxIter) { badIt = 1; break; }
+            or_llb_rhs(s, m, dy, step, a, dt);
+            evals++;
+        }
+        if (badIt && tr == maxTry - 1) return -1;
+        if (badIt) {
+            double ha = 0.5 * dt;
+            if (alpha > alpha_ref) ha = dt * pow(alpha_ref / alpha, 0.5);
+            *dtio = ha;
+            a++;
+            continue;
+        }
+        tErr = am_norm(m, y1, N, absErr, relErr);
+        used_dt = dt;
+        int bad = tErr > 1.0;
+        double new_dt = dt * pow(headRoom / tErr, 0.5);
+        if (restrict_step) {
+            double ha = dt * pow(alpha_ref / saved_alpha, 0.5);
+            new_dt = fmin(new_dt, ha);
+            restrict_step = 0;
+        }
+        if (new_dt < mindt) new_dt = mindt;
+        if (new_dt > maxdt) new_dt = maxdt;
+        *dtio = new_dt;
+        if (!bad || new_dt == mindt) break;
+        if (tr > maxTry) return -1;
+        tr++;
+        a++;
+    }
+    if (stats) {
+        stats[0] = tErr; stats[1] = tr + 1; stats[2] = (double)iterations; stats[3] = (double)evals;
+        stats[4] = used_dt;
+    }
+    return 1;
+}
+
+/* BDFEuler.Step, engine/solve_euler_backward.go:29-86, for the single equation
+ * m: explicit-Euler start, then y <- y0 + dt f(y) until
+ * max|y - yb| / max|y + yb| <= maxIterErr (over all 3N values); more than
+ * maxIter updates fail.  The RHS of the last iterate is not needed and not
+ * evaluated.  stats: [0] last iterator error, [1] iterations, [2] RHS evaluations. */
+int or_llb_step_bdf_euler(or_llb* s, double* m, double dt, uint32_t step, double maxIterErr, int maxIter,
+                          double* stats) {
+    const long n3 = 3 * s->N;
+    double *dy = s->k1, *yb = s->ys;
+    long evals = 0, iterations = 0;
+    or_llb_rhs(s, m, dy, step, 0, dt); /* normalises m = y0 */
+    evals++;
+    memcpy(s->y0, m, sizeof(double) * n3);
+    for (long i = 0; i < n3; i++) m[i] = s->y0[i] + dt * dy[i];
+    iterations++;
+    double err = 1.0e38;
+    int iter = 0;
+    while (err > maxIterErr) {
+        or_llb_rhs(s, m, dy, step, 0, dt);
+        evals++;
+        for (long i = 0; i < n3; i++) yb[i] = s->y0[i] + dt * dy[i];
+        err = or_maxdiff(m, yb, n3);
+        double sum = or_maxsum(m, yb, n3);
+        if (sum > 0.0) err = err / sum;
+        iter++;
+        iterations++;
+        memcpy(m, yb, sizeof(double) * n3);
+        if (iter > maxIter) return -1;
+    }
+    if (stats) { stats[0] = err; stats[1] = (double)iterations; stats[2] = (double)evals; }
+    return 1;
+}
+
 /* ------------------------------------------------------------------ two-temperature model */
 /* Qe summed in the order Qspat, Qe-p, Qlaser; Qspat = or_qspat with a
  * uniform conductance (Qspat.cu:13-134), Qe-p = or_qinter (Qinter.cu:13-45). */

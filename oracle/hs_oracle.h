@@ -204,6 +204,13 @@ int or_llb_step_rk12(or_llb* s, double* m, double* dt, double maxErr,
                      double mindt, double maxdt, uint32_t step, double* stats);
 int or_llb_step_rk23(or_llb* s, double* m, double* dt, double maxErr,
                      double mindt, double maxdt, uint32_t step, double* stats);
+/* implicit drivers (solve_am01.go, solve_euler_backward.go); -1 where the
+ * reference panics.  AM12 stats[5]: err, attempts, iterations, RHS evals, dt used;
+ * BDF-Euler stats[3]: iterator err, iterations, RHS evals. */
+int or_llb_step_am12(or_llb* s, double* m, double* dt, double absErr, double relErr,
+                     double mindt, double maxdt, uint32_t step, double* stats);
+int or_llb_step_bdf_euler(or_llb* s, double* m, double dt, uint32_t step, double maxIterErr, int maxIter,
+                          double* stats);
 /* Two-temperature model with laser source (TM.go dT/dt = Q/C; Qspat.go,
  * Qinter.go; mod_ce.go Ce = gammaE Te):
  *   Qe = kappa lap(Te) + G (Tp - Te) + Q0 exp(-r^2/(2 sr^2)) exp(-(t-t0)^2/(2 st^2))

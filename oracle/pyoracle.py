@@ -90,6 +90,10 @@ def _declare(L):
     for name in ("or_llb_step_rk12", "or_llb_step_rk23"):
         getattr(L, name).argtypes = [v, v, v, d, d, d, u32, v]
         getattr(L, name).restype = i
+    L.or_llb_step_am12.argtypes = [v, v, v, d, d, d, d, u32, v]
+    L.or_llb_step_am12.restype = i
+    L.or_llb_step_bdf_euler.argtypes = [v, v, d, u32, d, i, v]
+    L.or_llb_step_bdf_euler.restype = i
     L.or_energy_term.argtypes = [v, v, v, v, v, d, d, C.c_long]
     L.or_energy_term.restype = d
     L.or_set_threads.argtypes = [i]
@@ -284,6 +288,29 @@ class LLB:
         if rc < 0:
             raise RuntimeError("oracle adaptive step did not converge")
         return float(dtb[0]), st
+
+    def step_am12(self, m, dt, step=0, absErr=1e-4, relErr=1e-3, mindt=1e-38, maxdt=1e38):
+        """One adaptive AM12 step (m modified in place).  Returns (dt_next,
+        stats[err, attempts, iterations, rhs_evals, dt_used])."""
+        assert m.dtype == np.float64 and m.flags["C_CONTIGUOUS"]
+        dtb = np.array([dt], np.float64)
+        st = np.zeros(5, np.float64)
+        rc = lib().or_llb_step_am12(self._h, ptr(m), dtb.ctypes.data, float(absErr), float(relErr), float(mindt),
+                                    float(maxdt), int(step), st.ctypes.data)
+        if rc < 0:
+            raise RuntimeError("oracle AM12 step did not converge")
+        return float(dtb[0]), st
+
+    def step_bdf_euler(self, m, dt, step=0, maxIterErr=1e-5, maxIter=3):
+        """One backward-Euler step (m modified in place).  Returns
+        stats[iterator err, iterations, rhs_evals]."""
+        assert m.dtype == np.float64 and m.flags["C_CONTIGUOUS"]
+        st = np.zeros(3, np.float64)
+        rc = lib().or_llb_step_bdf_euler(self._h, ptr(m), float(dt), int(step), float(maxIterErr), int(maxIter),
+                                         st.ctypes.data)
+        if rc < 0:
+            raise RuntimeError("oracle backward-Euler iterator did not converge")
+        return st
 
     def step_tm(self, q, m, Te, Tp, t, dt, step=0, maxErr=1e-5, mindt=1e-38, maxdt=1e38):
         """One adaptive RK23 step of (m, Te, Tp) with T = Te (arrays modified in
