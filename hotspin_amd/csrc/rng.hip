@@ -14,6 +14,17 @@ __global__ void k_thermal_normals_f64(double* xi, int N, uint64_t seed, uint32_t
     xi[2L * N + I] = c;
 }
 
+__global__ void k_thermal_normals_f32(float* xi, int N, uint64_t seed, uint32_t stream, uint32_t step,
+                                      uint32_t attempt) {
+    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    float a, b, c;
+    thermal_normals_hw((uint32_t)I, step, attempt, stream, seed, a, b, c);
+    xi[I] = a;
+    xi[(long)N + I] = b;
+    xi[2L * N + I] = c;
+}
+
 __global__ void k_philox_raw(uint32_t* out, const uint32_t* ctr, int n, uint64_t key) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -35,6 +46,18 @@ extern "C" void hs_thermal_normals_f64(double* xi, int N, uint64_t seed, uint32_
     hipLaunchKernelGGL(k_thermal_normals_f64, hs::grid1d(N), hs::kBlock, 0, hs::S(s), xi, N, seed, stream, step,
                        attempt);
     HS_LAUNCH_CHECK("hs_thermal_normals_f64");
+}
+
+extern "C" void hs_thermal_normals_f32(float* xi, int N, uint64_t seed, uint32_t stream, uint32_t step,
+                                       uint32_t attempt, CUstream s) {
+    if (!xi || N < 0) {
+        hs::set_error(HS_EINVAL, "hs_thermal_normals_f32");
+        return;
+    }
+    if (N == 0) return;
+    hipLaunchKernelGGL(k_thermal_normals_f32, hs::grid1d(N), hs::kBlock, 0, hs::S(s), xi, N, seed, stream, step,
+                       attempt);
+    HS_LAUNCH_CHECK("hs_thermal_normals_f32");
 }
 
 extern "C" void hs_philox4x32_10(uint32_t* out, const uint32_t* ctr, int n, uint64_t key, CUstream s) {

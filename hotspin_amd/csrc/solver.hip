@@ -52,6 +52,7 @@ struct SolveP {
 struct LfConst {
     int uniform;   // fast path usable
     int langevin;  // S > 1e5
+    int gpow;      // 2S+1 when it is an integer <= 16: exp(-2gx) = exp(-2lx)^(2S+1); else 0
     double l, g, lim;
     double a1, a3, a5, a7;  // series of B: x (a1 - x^2 (a3 - x^2 (a5 - x^2 a7)))
     double d0, d2, d4, d6;  // series of dB/dx
@@ -86,6 +87,7 @@ struct Phys {
     const float *J, *n, *Tc, *T;
     float gamma;
     float aMul[3];
+    float sqa[3];  // sqrt(aMul[c]): the thermal amplitude of a component without an alpha mask
     const float* alpha[3];
     LfConst lf;
 };
@@ -195,10 +197,22 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
     const long N12 = (long)N1 * N2;
     const float l02 = l0 * l0;
     float e0[3], e1[3], e2[3];
+    // harmonic-mean weights of the two neighbours; exactly 1 without a lex mask
+    // (a uniform branch: no divisions on the common path)
+    auto weights = [&](float l1, float l2, float& pre1, float& pre2) {
+        if (P.lex) {
+            pre1 = avg_geom_zero(l02, l1 * l1);
+            pre2 = avg_geom_zero(l02, l2 * l2);
+        } else {
+            pre1 = 1.f;
+            pre2 = 1.f;
+        }
+    };
     {  // axis 0 (X): planes i-1, i+1; clamped at open ends = self
         bool lo_self = (i == 0 && !P.w0), hi_self = (i == N0 - 1 && !P.w0);
         float l1 = lo_self ? l0 : llo, l2 = hi_self ? l0 : lhi;
-        float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+        float pre1, pre2;
+        weights(l1, l2, pre1, pre2);
         for (int c = 0; c < 3; c++) {
             float v1 = lo_self ? m0[c] : lo[c], v2 = hi_self ? m0[c] : hi[c];
             e0[c] = P.ex0 * (pre1 * (v1 - m0[c]) + pre2 * (v2 - m0[c]));
@@ -224,7 +238,8 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
             for (int c = 0; c < 3; c++) v2[c] = tile[c][ty + 1][col];
             l2 = P.lex ? ltile[ty + 1][col] : 1.f;
         }
-        float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+        float pre1, pre2;
+        weights(l1, l2, pre1, pre2);
         for (int c = 0; c < 3; c++) e2[c] = P.ex2 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
     }
     {  // axis 1 (Y)
@@ -247,7 +262,8 @@ __device__ __forceinline__ void exchange_cell(const Phys& P, const float* m, lon
             for (int c = 0; c < 3; c++) v2[c] = tile[c][row][cc];
             l2 = P.lex ? ltile[row][cc] : 1.f;
         }
-        float pre1 = avg_geom_zero(l02, l1 * l1), pre2 = avg_geom_zero(l02, l2 * l2);
+        float pre1, pre2;
+        weights(l1, l2, pre1, pre2);
         for (int c = 0; c < 3; c++) e1[c] = P.ex1 * (pre1 * (v1[c] - m0[c]) + pre2 * (v2[c] - m0[c]));
     }
     for (int c = 0; c < 3; c++) H[c] = P.ms0T0Mul * (e0[c] + e2[c] + e1[c]);
@@ -311,8 +327,18 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
     if (P.longfield && P.lf.uniform) {
         const LfConst& C = P.lf;
         double mx = m0[0], my = m0[1], mz = m0[2];
-        double am = sqrt(mx * mx + my * my + mz * mz);
-        if (!(ms == 0.f || am <= HS_ZERO)) {
+        double am2 = mx * mx + my * my + mz * mz;
+        if (!(ms == 0.f || am2 <= HS_ZERO * HS_ZERO)) {
+            // |m| and 1/|m|: hardware rsqrt refined by one FP64 Newton step (rel. error ~1e-14)
+            double ia, am;
+            if (am2 > 1e-30) {
+                ia = (double)__builtin_amdgcn_rsqf((float)am2);
+                ia = ia * (1.5 - 0.5 * am2 * ia * ia);
+                am = am2 * ia;
+            } else {
+                am = sqrt(am2);
+                ia = 1.0 / am;
+            }
             double x = am * sp.lf_b;
             double W, dW;
             if (x < C.lim) {
@@ -320,8 +346,17 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
                 W = x * (C.a1 - x2 * (C.a3 - x2 * (C.a5 - x2 * C.a7)));
                 dW = C.d0 - x2 * (C.d2 - x2 * (C.d4 - x2 * C.d6));
             } else if (!C.langevin) {
-                double eg = exp(-2.0 * C.g * x), el = exp(-2.0 * C.l * x);
-                double ig = 1.0 / (1.0 - eg), il = 1.0 / (1.0 - el);
+                // coth(gx), coth(lx) through e^{-2gx}, e^{-2lx}; g = (2S+1) l, so for
+                // integer 2S+1 the first is a power of the second (one exp)
+                double el = exp(-2.0 * C.l * x), eg;
+                if (C.gpow) {
+                    eg = el;
+                    for (int q2 = 1; q2 < C.gpow; q2++) eg *= el;
+                } else {
+                    eg = exp(-2.0 * C.g * x);
+                }
+                double inv = 1.0 / ((1.0 - eg) * (1.0 - el));
+                double ig = (1.0 - el) * inv, il = (1.0 - eg) * inv;
                 W = C.g * (1.0 + eg) * ig - C.l * (1.0 + el) * il;
                 dW = 4.0 * (C.l * C.l) * el * (il * il) - 4.0 * (C.g * C.g) * eg * (ig * ig);
             } else {
@@ -329,7 +364,7 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
                 W = (1.0 + e) * i1 - 1.0 / x;
                 dW = 1.0 / (x * x) - 4.0 * e * (i1 * i1);
             }
-            double mult = sp.lf_c1 * (W - am) / ((double)ms * dW * am);
+            double mult = sp.lf_c1 * (W - am) * ia / ((double)ms * dW);
             H[0] += (float)(mult * mx);
             H[1] += (float)(mult * my);
             H[2] += (float)(mult * mz);
@@ -356,12 +391,12 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
     }
     if (P.thermal && ms != 0.f) {
         float x0, x1, x2;
-        thermal_normals_f((uint32_t)I, io.step, io.attempt, sp.stream, sp.seed, x0, x1, x2);
+        thermal_normals_hw((uint32_t)I, io.step, io.attempt, sp.stream, sp.seed, x0, x1, x2);
         float thK = (float)(sp.thC / fmax((double)P.cutoff_dt, io.h));
-        float pre = sqrtf((q.T * thK) / ms);
-        float s0 = sqrtf(P.aMul[0] * q.al[0]);
-        float s1 = sqrtf(P.aMul[1] * q.al[1]);
-        float s2 = sqrtf(P.aMul[2] * q.al[2]);
+        float pre = __builtin_amdgcn_sqrtf((q.T * thK) * __builtin_amdgcn_rcpf(ms));
+        float s0 = P.alpha[0] ? __builtin_amdgcn_sqrtf(P.aMul[0] * q.al[0]) : P.sqa[0];
+        float s1 = P.alpha[1] ? __builtin_amdgcn_sqrtf(P.aMul[1] * q.al[1]) : P.sqa[1];
+        float s2 = P.alpha[2] ? __builtin_amdgcn_sqrtf(P.aMul[2] * q.al[2]) : P.sqa[2];
         H[0] += pre * (s0 * x0);
         H[1] += pre * (s1 * x1);
         H[2] += pre * (s2 * x2);
@@ -1684,6 +1719,8 @@ LfConst lf_constants(const hs_llb_params* p) {
         return C;
     }
     const double l = 1.0 / (2.0 * S), g = (2.0 * S + 1.0) * l;
+    const double tp = 2.0 * S + 1.0;
+    C.gpow = (tp == floor(tp) && tp >= 1.0 && tp <= 16.0) ? (int)tp : 0;
     const double l2 = l * l, g2 = g * g, l4 = l2 * l2, g4 = g2 * g2, l6 = l4 * l2, g6 = g4 * g2;
     const double l8 = l4 * l4, g8 = g4 * g4;
     C.l = l, C.g = g;
@@ -1803,6 +1840,7 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     P.lf = lf_constants(p);
     for (int c = 0; c < 3; c++) {
         P.aMul[c] = (float)p->alphaMul[c];
+        P.sqa[c] = sqrtf(P.aMul[c]);
         P.alpha[c] = upload_mask(s, p->alpha[c]);
     }
     if (p->demag) {

@@ -83,6 +83,11 @@ HS_API int hs_demag_reserve(hs_demag* d, int nsolve);
  * curandGenerateNormalDouble(3N) of mod_brown.go:122. */
 HS_API void hs_thermal_normals_f64(double* xi, int N, uint64_t seed, uint32_t stream, uint32_t step,
                                    uint32_t attempt, CUstream s);
+/* The FP32 normals the resident solver's stage kernels draw in registers
+ * (same Philox words, Box-Muller from the hardware log2 / sqrt / sin / cos):
+ * within ~1e-6 absolute of hs_thermal_normals_f64. */
+HS_API void hs_thermal_normals_f32(float* xi, int N, uint64_t seed, uint32_t stream, uint32_t step,
+                                   uint32_t attempt, CUstream s);
 /* raw Philox words for n counters (device arrays, 4 words each) */
 HS_API void hs_philox4x32_10(uint32_t* out, const uint32_t* ctr, int n, uint64_t key, CUstream s);
 

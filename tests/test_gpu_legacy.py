@@ -299,6 +299,13 @@ def test_thermal_normals_match_oracle_stream():
     lib().hs_thermal_normals_f64(P(xi), n, 1234, 3, 17, 2, None)
     want = po.thermal_normals(1234, 3, 17, 2, n).ravel()
     assert np.abs(host(xi) - want).max() <= 1e-12
+    # the FP32 normals of the stage kernels (hardware log2/sqrt/sin/cos): a wrong
+    # argument unit of v_sin/v_cos (radians for revolutions) would be off by O(1)
+    xf = empty(3 * n, dtype=torch.float32)
+    lib().hs_thermal_normals_f32(P(xf), n, 1234, 3, 17, 2, None)
+    got = host(xf).astype(np.float64)
+    assert np.abs(got - want).max() <= 2e-5
+    assert np.abs(got - want).mean() <= 1e-6
 
 
 def test_mfa_helpers_kappa_ts_cp():
