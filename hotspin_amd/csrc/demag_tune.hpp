@@ -110,6 +110,28 @@ void add_col2(Variants& v) {
     }
 }
 
+// two-level register FFT rows (fft2.hpp): W rows, W * B threads
+template <int M, int W>
+void launch_r2c2_32(const float* x, const float* msk, long mrows, cpx<float>* out, long nrows, int n2, int H2p,
+                    const cpx<float>* twM, const cpx<float>* twP, hipStream_t st, Live lv) {
+    hipLaunchKernelGGL((k_r2c_rows2<float, M, W>), dim3((unsigned)((nrows + W - 1) / W)), dim3(W * split2(M).B), 0,
+                       st, x, msk, mrows, out, nrows, n2, H2p, twM, twP, lv);
+}
+template <int M, int W>
+void launch_c2r2_32(const cpx<float>* in, float* x, long nrows, int n2, int H2p, const cpx<float>* twM,
+                    const cpx<float>* twP, hipStream_t st, Live lv) {
+    hipLaunchKernelGGL((k_c2r_rows2<float, M, W>), dim3((unsigned)((nrows + W - 1) / W)), dim3(W * split2(M).B), 0,
+                       st, in, x, nrows, n2, H2p, twM, twP, lv);
+}
+template <int M, int W>
+void add_row2(Variants& v) {
+    if constexpr (two_level_ok(M) && M > 1) {
+        constexpr int NT = W * split2(M).B;
+        if constexpr (((long)rows2_lds<M, W>() + 2L * M) * 8 <= kLdsMax && NT >= 64 && NT <= 1024)
+            v.rows.push_back({W, NT, &launch_r2c2_32<M, W>, &launch_c2r2_32<M, W>});
+    }
+}
+
 template <int L, int W, int E>
 void add_z(Variants& v) {  // a block holds a pencil pair: 6W pencils
     constexpr int NT = clamp_nt((long)L * 6 * W / E);
@@ -124,6 +146,9 @@ void register_size() {
     add_row<L, 8, 8>(v);
     add_row<L, 4, 4>(v);
     add_row<L, 16, 8>(v);
+    add_row2<L, 2>(v);
+    add_row2<L, 4>(v);
+    add_row2<L, 8>(v);
     add_col<L, 8, 8>(v);
     add_col<L, 8, 16>(v);
     add_col<L, 16, 16>(v);

@@ -97,12 +97,14 @@ def test_golden_demag_fp32(pbc, name):
     assert relerr(Bh, want) <= 2e-6
 
 
-def test_every_fp32_pass_variant_matches_fp64(monkeypatch):
+@pytest.mark.parametrize("N", [(16, 32, 64), (4, 8, 512)])
+def test_every_fp32_pass_variant_matches_fp64(monkeypatch, N):
     """Each registered FP32 tile variant of the row, column and axis-0 passes
-    (forced by index) gives the FP64 field to FP32 accuracy."""
+    (forced by index) gives the FP64 field to FP32 accuracy (row length 512
+    is the 512x512x16 workload's)."""
     import ctypes as C
     from hotspin_amd._lib import lib
-    N, cell = (16, 32, 64), (2e-9, 3e-9, 4e-9)
+    cell = (2e-9, 3e-9, 4e-9)
     rng = np.random.default_rng(5)
     ncell = int(np.prod(N))
     m = rng.standard_normal((3, ncell))
