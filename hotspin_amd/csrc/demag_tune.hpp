@@ -110,6 +110,30 @@ void add_col2(Variants& v) {
     }
 }
 
+// half-length column pass (fft2.hpp k_cols2h) for a zero-padded axis: the
+// forward pass needs nin <= L/2 and the inverse nkeep <= L/2; otherwise (a
+// periodic axis, no padding) the launcher runs the length-L k_cols2
+template <int L, int S, int W>
+void launch_cols2h_32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int npencil, int nin, int nkeep,
+                      int ncol, const cpx<float>* tw, hipStream_t st, Live lv) {
+    dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
+    const bool half = (S < 0) ? (2 * nin <= L) : (2 * nkeep <= L);
+    if (half)
+        hipLaunchKernelGGL((k_cols2h<float, L, S, W>), g, dim3(W * split2(L / 2).B), 0, st, in, out, gi, go, nin,
+                           nkeep, ncol, tw, lv);
+    else
+        hipLaunchKernelGGL((k_cols2<float, L, S, W>), g, dim3(W * split2(L).B), 0, st, in, out, gi, go, nin, nkeep,
+                           ncol, tw, lv);
+}
+template <int L, int W>
+void add_col2h(Variants& v) {
+    if constexpr (L % 2 == 0 && L >= 4 && two_level_ok(L) && two_level_ok(L / 2)) {
+        constexpr int NTh = W * split2(L / 2).B, NT = W * split2(L).B;
+        if constexpr ((long)(L * W + L) * 8 <= kLdsMax && NTh >= 64 && NTh <= 1024 && NT >= 64 && NT <= 1024)
+            v.cols.push_back({W, NTh, &launch_cols2h_32<L, -1, W>, &launch_cols2h_32<L, 1, W>});
+    }
+}
+
 // two-level register FFT rows (fft2.hpp): W rows, W * B threads
 template <int M, int W>
 void launch_r2c2_32(const float* x, const float* msk, long mrows, cpx<float>* out, long nrows, int n2, int H2p,
@@ -158,6 +182,8 @@ void register_size() {
     add_col2<L, 4>(v);
     add_col2<L, 8>(v);
     add_col2<L, 16>(v);
+    add_col2h<L, 8>(v);
+    add_col2h<L, 16>(v);
     add_z<L, 8, 12>(v);
     add_z<L, 16, 12>(v);
     add_z<L, 32, 12>(v);

@@ -203,6 +203,96 @@ __global__ void __launch_bounds__(NT) k_cols2(const cpx<T>* in, cpx<T>* out, Geo
     }
 }
 
+// ---- half-length column pass (F2 / I2 of a zero-padded axis) -------------
+// L = 2H with the data on one side of the padding: the forward transform of
+// an input with rows t < nin <= H, or the inverse transform whose output rows
+// t < nkeep <= H are kept.  The length-L transform splits into two length-H
+// transforms (even and odd frequencies):
+//   forward: X[2k] = DFT_H(x)[k],  X[2k+1] = DFT_H(x[n] w_L^n)[k]
+//   inverse: x[n] = IDFT_H(X[2k])[n] + w_L^-n IDFT_H(X[2k+1])[n]
+// each done two-level (H = A B) through an LDS tile of H x W points, so a
+// block holds half the LDS of the length-L pass.  tw: w_L^j, j < L
+// (w_H^j = w_L^2j).  Threads: W * B (phase 1), W * A (phase 2).
+template <typename T, int L, int S, int W, int NT = W * split2(L / 2).B>
+__global__ void __launch_bounds__(NT) k_cols2h(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
+                                               int ncol, const cpx<T>* __restrict__ tw, Live lv) {
+    constexpr int H = L / 2, A = split2(H).A, B = split2(H).B;
+    static_assert(A * B == H && NT == W * B && A <= B, "two-level split");
+    __shared__ cpx<T> lds[H * W];
+    __shared__ cpx<T> tws[L];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int a = lb / gridDim.x;
+    if (all_dead(lv, a, a)) return;
+    const int col0 = (lb % gridDim.x) * W;
+    const long bi = geo_base(gi, a) + col0;
+    const long bo = geo_base(go, a) + col0;
+    for (int e = tid; e < L; e += NT) tws[e] = tw[e];
+    const int p = tid % W, n2 = tid / W;
+    const bool pin = col0 + p < ncol;
+    auto twl = [&](int j) {  // w_L^j in the transform's direction
+        cpx<T> w = tws[j];
+        if (S > 0) w.y = -w.y;
+        return w;
+    };
+    // phase 1 inputs: forward x[B n1 + n2]; inverse X[2(B n1 + n2)] and X[2(B n1 + n2) + 1]
+    cpx<T> ve[A], vo[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) {
+        const int t = B * n1 + n2;
+        cpx<T> x0 = {T(0), T(0)}, x1 = {T(0), T(0)};
+        if (S < 0) {
+            if (pin && t < nin) x0 = in[bi + (long)t * gi.ts + p];
+        } else {
+            if (pin && 2 * t < nin) x0 = in[bi + (long)(2 * t) * gi.ts + p];
+            if (pin && 2 * t + 1 < nin) x1 = in[bi + (long)(2 * t + 1) * gi.ts + p];
+        }
+        ve[n1] = x0;
+        vo[n1] = x1;
+    }
+    __syncthreads();  // twiddles visible
+    if (S < 0) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) vo[n1] = (B * n1 + n2) ? cmul(ve[n1], twl(B * n1 + n2)) : ve[n1];
+    }
+    cpx<T> acc[B];  // inverse: the even half of the output rows
+    const int k1 = tid / W;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        cpx<T>* v = half ? vo : ve;
+        dft_reg<T, A, S>(v);
+        if (half) __syncthreads();  // the previous half's phase 2 has read the tile
+#pragma unroll
+        for (int q = 0; q < A; q++) lds[(q * B + n2) * W + p] = (n2 * q) ? cmul(v[q], twl(2 * n2 * q)) : v[q];
+        __syncthreads();
+        if (tid < W * A) {
+            cpx<T> u[B];
+#pragma unroll
+            for (int m = 0; m < B; m++) u[m] = lds[(k1 * B + m) * W + p];
+            dft_reg<T, B, S>(u);
+            if (S < 0) {  // forward: frequencies 2k + half
+                if (pin) {
+#pragma unroll
+                    for (int k2 = 0; k2 < B; k2++) {
+                        const int t = 2 * (k1 + A * k2) + half;
+                        if (t < nkeep) out[bo + (long)t * go.ts + p] = u[k2];
+                    }
+                }
+            } else if (half == 0) {
+#pragma unroll
+                for (int k2 = 0; k2 < B; k2++) acc[k2] = u[k2];
+            } else if (pin) {  // inverse: x[n] = even[n] + w_L^-n odd[n], n < nkeep
+#pragma unroll
+                for (int k2 = 0; k2 < B; k2++) {
+                    const int t = k1 + A * k2;
+                    const cpx<T> o = t ? cmul(u[k2], twl(t)) : u[k2];
+                    if (t < nkeep) out[bo + (long)t * go.ts + p] = cadd(acc[k2], o);
+                }
+            }
+        }
+    }
+}
+
 // ---- row passes (F1 R2C / I1 C2R) -----------------------------------------
 // M = P2/2-point complex FFT per row as in demag_kernels.hpp's k_r2c_rows /
 // k_c2r_rows, done two-level.  W rows per block; phase-1 threads take
