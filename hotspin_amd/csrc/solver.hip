@@ -42,7 +42,7 @@ struct SolveP {
     double t_origin;  // two-temperature solves: laser time at the start of the current run
     uint64_t seed;
     uint32_t stream;
-    uint32_t pad;
+    float sthC;  // sqrt(thC): H_therm amplitude = sqrt(T / msk) sqrt(thC) / sqrt(max(cutoff_dt, dt))
 };
 
 // Material constants of the longitudinal field when J, n, Tc and T are
@@ -392,8 +392,8 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
     if (P.thermal && ms != 0.f) {
         float x0, x1, x2;
         thermal_normals_hw((uint32_t)I, io.step, io.attempt, sp.stream, sp.seed, x0, x1, x2);
-        float thK = (float)(sp.thC / fmax((double)P.cutoff_dt, io.h));
-        float pre = __builtin_amdgcn_sqrtf((q.T * thK) * __builtin_amdgcn_rcpf(ms));
+        const float rdt = __builtin_amdgcn_rsqf((float)fmax((double)P.cutoff_dt, io.h));
+        float pre = __builtin_amdgcn_sqrtf(q.T * __builtin_amdgcn_rcpf(ms)) * (sp.sthC * rdt);
         float s0 = P.alpha[0] ? __builtin_amdgcn_sqrtf(P.aMul[0] * q.al[0]) : P.sqa[0];
         float s1 = P.alpha[1] ? __builtin_amdgcn_sqrtf(P.aMul[1] * q.al[1]) : P.sqa[1];
         float s2 = P.alpha[2] ? __builtin_amdgcn_sqrtf(P.aMul[2] * q.al[2]) : P.sqa[2];
@@ -1737,6 +1737,7 @@ void solve_constants(const hs_llb_params* p, double T, SolveP& q) {
     const double preS = (S > HS_INFSPIN) ? 1.0 : S / (S + 1.0);
     q.TMul = (float)T;
     q.thC = (HS_KB * 2.0 * T) / (HS_MU0 * V * p->gammaLL * p->msat0T0Mul);
+    q.sthC = (float)sqrt(q.thC);
     q.lf_b = 3.0 * preS * p->TcMul / T;
     q.lf_c1 = p->nMul * (HS_KB / HS_MU0) * T / p->msat0T0Mul;
 }
@@ -1791,7 +1792,6 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
         for (int c = 0; c < 3; c++) hsp[q].bext[c] = (float)(1.0 / HS_MU0 * (Bext ? Bext[3 * q + c] : p->Bext[c]));
         hsp[q].seed = seed ? seed[q] : p->seed;
         hsp[q].stream = p->stream + (uint32_t)(seed ? 0 : q);
-        hsp[q].pad = 0;
     }
     hipMemcpy(s->sp, hsp.data(), sizeof(SolveP) * nsolve, hipMemcpyHostToDevice);
     s->sp_host = hsp;
