@@ -86,31 +86,44 @@ __global__ void __launch_bounds__(NT) k_r2c_rows(const TI* __restrict__ x, const
     long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
     const bool pairs = (n2 & 1) == 0;  // rows of even length: each complex input is one aligned pair
-    for (int e = tid; e < M * W; e += NT) {
-        int p = e / M, n = e % M;
-        long r = row0 + p;
-        cpx<T> v = {T(0), T(0)};
-        if (r < nrows) {
-            long base = r * n2;
-            long mb = (r % mrows) * n2;
-            int i0 = 2 * n, i1 = 2 * n + 1;
-            if (pairs) {
-                if (i0 < n2) {
-                    using V2 = typename std::conditional<sizeof(TI) == 4, float2, double2>::type;
-                    const V2 a = *(const V2*)(x + base + i0);
-                    v = {T(a.x), T(a.y)};
-                    if (msk) {
-                        const V2 w = *(const V2*)(msk + mb + i0);
-                        v.x *= T(w.x);
-                        v.y *= T(w.y);
+    // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
+    constexpr int NE = M * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+    for (int q0 = 0; q0 < IT; q0 += CH) {
+        cpx<T> g[CH];
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            const int e = tid + (q0 + q) * NT;
+            int p = e / M, n = e % M;
+            long r = row0 + p;
+            cpx<T> v = {T(0), T(0)};
+            if (e < NE && r < nrows) {
+                long base = r * n2;
+                long mb = (r % mrows) * n2;
+                int i0 = 2 * n, i1 = 2 * n + 1;
+                if (pairs) {
+                    if (i0 < n2) {
+                        using V2 = typename std::conditional<sizeof(TI) == 4, float2, double2>::type;
+                        const V2 a = *(const V2*)(x + base + i0);
+                        v = {T(a.x), T(a.y)};
+                        if (msk) {
+                            const V2 w = *(const V2*)(msk + mb + i0);
+                            v.x *= T(w.x);
+                            v.y *= T(w.y);
+                        }
                     }
+                } else {
+                    if (i0 < n2) v.x = T(x[base + i0]) * (msk ? T(msk[mb + i0]) : T(1));
+                    if (i1 < n2) v.y = T(x[base + i1]) * (msk ? T(msk[mb + i1]) : T(1));
                 }
-            } else {
-                if (i0 < n2) v.x = T(x[base + i0]) * (msk ? T(msk[mb + i0]) : T(1));
-                if (i1 < n2) v.y = T(x[base + i1]) * (msk ? T(msk[mb + i1]) : T(1));
             }
+            g[q] = v;
         }
-        lds[n * W + p] = v;
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            const int e = tid + (q0 + q) * NT;
+            const int p = e / M, n = e % M;
+            if (e < NE) lds[n * W + p] = g[q];
+        }
     }
     __syncthreads();
     fft_lds<T, M, W, NT, -1>(lds, twM, tid);
@@ -139,12 +152,25 @@ __global__ void __launch_bounds__(NT) k_c2r_rows(const cpx<T>* __restrict__ in, 
     const int tid = threadIdx.x;
     long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
-    for (int e = tid; e < (M + 1) * W; e += NT) {
-        int p = e / (M + 1), k = e % (M + 1);
-        long r = row0 + p;
-        cpx<T> v = {T(0), T(0)};
-        if (r < nrows) v = in[r * H2p + k];
-        lds[k * W + p] = v;
+    {  // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
+        constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+        for (int q0 = 0; q0 < IT; q0 += CH) {
+            cpx<T> g[CH];
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int e = tid + (q0 + q) * NT;
+                const int p = e / (M + 1), k = e % (M + 1);
+                const long r = row0 + p;
+                g[q] = {T(0), T(0)};
+                if (e < NE && r < nrows) g[q] = in[r * H2p + k];
+            }
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int e = tid + (q0 + q) * NT;
+                const int p = e / (M + 1), k = e % (M + 1);
+                if (e < NE) lds[k * W + p] = g[q];
+            }
+        }
     }
     __syncthreads();
     constexpr int PER = (M * W + NT - 1) / NT;
@@ -208,11 +234,23 @@ __global__ void __launch_bounds__(NT) k_cols(const cpx<T>* in, cpx<T>* out, Geo 
     const int col0 = (lb % gridDim.x) * W;
     const long bi = geo_base(gi, a) + col0;
     const long bo = geo_base(go, a) + col0;
-    for (int e = tid; e < L * W; e += NT) {
-        int t = e / W, p = e % W;
-        cpx<T> v = {T(0), T(0)};
-        if (t < nin && col0 + p < ncol) v = in[bi + t * gi.ts + p];
-        lds[e] = v;
+    {  // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
+        constexpr int NE = L * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+        for (int q0 = 0; q0 < IT; q0 += CH) {
+            cpx<T> g[CH];
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int e = tid + (q0 + q) * NT;
+                const int t = e / W, p = e % W;
+                g[q] = {T(0), T(0)};
+                if (e < NE && t < nin && col0 + p < ncol) g[q] = in[bi + t * gi.ts + p];
+            }
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int e = tid + (q0 + q) * NT;
+                if (e < NE) lds[e] = g[q];
+            }
+        }
     }
     __syncthreads();
     fft_lds<T, L, W, NT, S>(lds, tw, tid);

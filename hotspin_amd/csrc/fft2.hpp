@@ -394,12 +394,24 @@ __global__ void __launch_bounds__(NT) k_c2r_rows2(const cpx<T>* __restrict__ in,
         tws[e] = twM[e];
         twp[e] = twP[e];
     }
-    for (int e = tid; e < (M + 1) * W; e += NT) {
-        const int p = e / (M + 1), k = e % (M + 1);
-        const long r = row0 + p;
-        cpx<T> v = {T(0), T(0)};
-        if (r < nrows) v = in[r * H2p + k];
-        lds[p * (M + 1) + k] = v;
+    {  // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
+        constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+        for (int q0 = 0; q0 < IT; q0 += CH) {
+            cpx<T> g[CH];
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int e = tid + (q0 + q) * NT;
+                const int p = e / (M + 1), k = e % (M + 1);
+                const long r = row0 + p;
+                g[q] = {T(0), T(0)};
+                if (e < NE && r < nrows) g[q] = in[r * H2p + k];
+            }
+#pragma unroll
+            for (int q = 0; q < CH; q++) {
+                const int e = tid + (q0 + q) * NT;
+                if (e < NE) lds[e] = g[q];  // e = p (M + 1) + k
+            }
+        }
     }
     __syncthreads();
     const int nb = tid % B, p = tid / B;
