@@ -1354,8 +1354,8 @@ StageIO base_io(hs_solver* s, double h, uint32_t step, uint32_t attempt) {
 // cached per mesh shape for the life of the process (the variants may round
 // differently, and a solve must give the same result in any batch).
 std::mutex g_stage_mu;
-std::map<std::array<int, 3>, int>& stage_cache() {
-    static std::map<std::array<int, 3>, int> c;
+std::map<std::array<int, 3>, std::array<int, 2>>& stage_cache() {  // shape -> (variant, march chunk)
+    static std::map<std::array<int, 3>, std::array<int, 2>> c;
     return c;
 }
 
@@ -1370,7 +1370,8 @@ void tune_stage(hs_solver* s) {
         std::lock_guard<std::mutex> lk(g_stage_mu);
         auto it = stage_cache().find(key);
         if (it != stage_cache().end()) {
-            s->stage_kind = it->second;
+            s->stage_kind = it->second[0];
+            if (s->stage_kind == 0) s->phys.chunk = std::min(it->second[1], s->phys.N0);
             return;
         }
     }
@@ -1384,26 +1385,42 @@ void tune_stage(hs_solver* s) {
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     float best = -1.f;
-    int pick = 1;
-    for (int v = 0; v < 3; v++) {
-        if ((v == 0 && s->phys.N0 < 2) || (v == 2 && !pair_ok)) continue;
-        s->stage_kind = v;
+    int pick = 1, pick_chunk = s->phys.chunk;
+    const int chunk0 = s->phys.chunk;
+    // the marching variant at its default chunk and at 2, 4, 8 planes per block
+    std::vector<std::array<int, 2>> cand;
+    if (s->phys.N0 >= 2) {
+        cand.push_back({0, chunk0});
+        for (int c : {2, 4, 8})
+            if (c != chunk0 && c < s->phys.N0) cand.push_back({0, c});
+    }
+    cand.push_back({1, chunk0});
+    if (pair_ok) cand.push_back({2, chunk0});
+    for (const auto& cv : cand) {
+        s->stage_kind = cv[0];
+        s->phys.chunk = cv[1];
+        float t = 1e30f;
         launch_stage<R_RK4_2>(s, io);
-        hipEventRecord(e0, s->st);
-        for (int r = 0; r < 3; r++) launch_stage<R_RK4_2>(s, io);
-        hipEventRecord(e1, s->st);
-        hipEventSynchronize(e1);
-        float t;
-        hipEventElapsedTime(&t, e0, e1);
+        for (int round = 0; round < 2; round++) {  // best of two rounds of 3 launches
+            hipEventRecord(e0, s->st);
+            for (int r = 0; r < 3; r++) launch_stage<R_RK4_2>(s, io);
+            hipEventRecord(e1, s->st);
+            hipEventSynchronize(e1);
+            float tr;
+            hipEventElapsedTime(&tr, e0, e1);
+            t = std::min(t, tr);
+        }
         if (best < 0.f || t < best) {
             best = t;
-            pick = v;
+            pick = cv[0];
+            pick_chunk = cv[1];
         }
     }
     s->stage_kind = pick;
+    s->phys.chunk = pick == 0 ? pick_chunk : chunk0;
     {
         std::lock_guard<std::mutex> lk(g_stage_mu);
-        stage_cache()[key] = pick;
+        stage_cache()[key] = {pick, pick_chunk};
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
