@@ -132,6 +132,13 @@ class Solver:
         if rc:
             raise HotspinError("hs_solver_set_implicit returned %d" % rc)
 
+    def set_implicit_T(self, maxAbsErr=1e-4, maxRelErr=1e-3):
+        """Te and Tp tolerances of "am12" on a two-temperature solve."""
+        rc = lib().hs_solver_set_implicit_T(self._h, float(maxAbsErr), float(maxRelErr))
+        check("hs_solver_set_implicit_T")
+        if rc:
+            raise HotspinError("hs_solver_set_implicit_T returned %d" % rc)
+
     def run(self, method, nsteps, dt, maxErr=1e-5, mindt=1e-38, maxdt=1e38, step0=None):
         meth = METHODS[method] if isinstance(method, str) else int(method)
         d = C.c_double(dt)

@@ -985,6 +985,53 @@ __global__ void __launch_bounds__(256) k_imp(const double* __restrict__ y0, doub
     }
 }
 
+// The same fixed-point update for a one-component FP64 equation (Te, Tp):
+// red[0] = |y - y_new|, red[1] = |y_new|, red[2] = |y_new - y1| (y1 given),
+// red[3] = 1 on a non-finite value; fout (optional) takes the FP32 copy of
+// y_new (the LLB stage's temperature).
+constexpr int kImp1Vals = 4;
+__global__ void __launch_bounds__(256) k_imp1(const double* __restrict__ y0, double* __restrict__ acc,
+                                              const double* __restrict__ k, const double* __restrict__ dy0,
+                                              const double* __restrict__ y1, float* __restrict__ fout, double c,
+                                              unsigned long long* __restrict__ red, long N) {
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double r[kImp1Vals] = {0.0, 0.0, 0.0, 0.0};
+    if (I < N) {
+        const double yc = acc[I];
+        const double yn = dy0 ? y0[I] + c * (k[I] + dy0[I]) : y0[I] + c * k[I];
+        r[0] = fabs(yc - yn);
+        r[1] = fabs(yn);
+        if (y1) r[2] = fabs(yn - y1[I]);
+        if (!isfinite(yn) || !isfinite(yc)) r[3] = 1.0;
+        acc[I] = yn;
+        if (fout) fout[I] = (float)yn;
+    }
+    __shared__ double wm[4][kImp1Vals];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int v = 0; v < kImp1Vals; v++) {
+        double x = r[v];
+        for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+        if (lane == 0) wm[wave][v] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kImp1Vals) {
+        const int v = threadIdx.x;
+        const double x = fmax(fmax(wm[0][v], wm[1][v]), fmax(wm[2][v], wm[3][v]));
+        const unsigned ln = blockIdx.x % kImpLanes;
+        atomicMax(red + ((long)v * kImpLanes + ln) * kImpStride, (unsigned long long)__double_as_longlong(x));
+    }
+}
+
+// y = y0 + c k (one component), with its FP32 copy
+__global__ void k_axpy1(const double* __restrict__ y0, const double* __restrict__ k, double* __restrict__ y,
+                        float* __restrict__ fout, double c, long N) {
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const double v = y0[I] + c * k[I];
+    y[I] = v;
+    if (fout) fout[I] = (float)v;
+}
+
 __global__ void k_set_state(const double* y, float* mf, int normalize, long N) {
     long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
@@ -1298,6 +1345,7 @@ struct hs_solver {
     double* y1 = nullptr;
     double imp_abs = 1e-4, imp_rel = 1e-3, imp_iter_err = 1e-5;
     int imp_max_iter = 3;
+    double imp_absT = 1e-4, imp_relT = 1e-3;  // Te and Tp (two-temperature solves)
 };
 
 namespace {
@@ -2000,12 +2048,27 @@ struct ImpRed {
 };
 
 bool ensure_imp(hs_solver* s) {
-    if (!s->imp &&
-        hipMalloc(&s->imp, sizeof(unsigned long long) * kImpVals * kImpLanes * kImpStride) != hipSuccess)
+    // region 0: the m maxima (kImpVals), regions 1, 2: Te, Tp (kImp1Vals each)
+    if (!s->imp && hipMalloc(&s->imp, sizeof(unsigned long long) * (kImpVals + 2 * kImp1Vals) * kImpLanes *
+                                          kImpStride) != hipSuccess)
         s->imp = nullptr;
     if (!s->y1 && hipMalloc(&s->y1, sizeof(double) * 3 * s->N * s->nsolve) != hipSuccess) s->y1 = nullptr;
     if (!s->imp || !s->y1) hs::set_error(HS_ENOMEM, "implicit driver: out of device memory");
     return s->imp && s->y1;
+}
+
+// LLB RHS of the iterate (FP32 stage input mB, demag first) into k_out; T:
+// the FP32 temperature of a two-temperature solve's iterate
+void imp_rhs(hs_solver* s, float* k_out, uint32_t step, uint32_t attempt, double dt, const float* T) {
+    demag_into(s, s->mB, s->Bs);
+    StageIO io = base_io(s, dt, step, attempt);
+    io.m = s->mB;
+    io.B = s->Bs;
+    io.kout = k_out;
+    io.hout = nullptr;
+    io.mout = nullptr;
+    io.T = T;
+    launch_stage<R_RHS>(s, io);
 }
 
 // One fixed-point update of every solve: the RHS of the iterate (FP32 stage
@@ -2015,14 +2078,7 @@ bool ensure_imp(hs_solver* s) {
 ImpRed imp_update(hs_solver* s, const float* k_in, float* k_out, const float* dy0, const double* y1, double c,
                   bool norm_in, uint32_t step, uint32_t attempt, double dt) {
     if (!k_in) {
-        demag_into(s, s->mB, s->Bs);
-        StageIO io = base_io(s, dt, step, attempt);
-        io.m = s->mB;
-        io.B = s->Bs;
-        io.kout = k_out;
-        io.hout = nullptr;
-        io.mout = nullptr;
-        launch_stage<R_RHS>(s, io);
+        imp_rhs(s, k_out, step, attempt, dt, nullptr);
         k_in = k_out;
     }
     const long nred = (long)kImpVals * kImpLanes * kImpStride;
@@ -2068,12 +2124,13 @@ int imp_fail(hs_solver* s, const char* what, double dt, double* dtio, hs_run_sta
 
 // Explicit-Euler start of every step: dy0 = f(y0) into e (y0 normalised in
 // place with the normaliser), acc = y0 + dt dy0, mB its stage input.
-void implicit_start(hs_solver* s, double dt, uint32_t step) {
+void implicit_start(hs_solver* s, double dt, uint32_t step, const float* T = nullptr) {
     ensure_b0(s);
     StageIO io = base_io(s, dt, step, 0);
     io.m = s->mA;
     io.B = s->B0;
     io.mout = s->mB;
+    io.T = T;
     launch_stage<R_RK12_1>(s, io);
 }
 
@@ -2228,7 +2285,263 @@ int run_bdf_euler(hs_solver* s, long nsteps, long step0, double* dtio, hs_run_st
     if (stats) *stats = st;
     return hs_last_error();
 }
+// ---- AM12 of a two-temperature solve: equations Te, Tp, m -------------------
+// BDFAM12.Step (solve_am01.go:45-300) for the laser recipe's equations in
+// their registration order (examples/laser-demag.py:35-60: temperature/ETM,
+// temperature/PTM, then llbar), the LLB temperature T = Te of the iterate.
+// Buffers: state Te[0], Tp[0], Tf[0]; iterate Te[1], Tp[1], Tf[1]; dy0 k1e,
+// k1p; dyE k2e, k2p; the current RHS ee, ep; the predictor result Tea, Tpa.
+struct TmRed {
+    ImpRed m;
+    double d[2], a[2], d1[2];
+    bool bad;
+};
+
+// RHS of all three equations at the iterate (time t): m's into km, Te's and Tp's into ke, kp
+void tm_rhs_all(hs_solver* s, float* km, double* ke, double* kp, uint32_t step, uint32_t attempt, double dt,
+                double t) {
+    imp_rhs(s, km, step, attempt, dt, s->Tf[1]);
+    TmIO io{};
+    io.Te = s->Te[1];
+    io.Tp = s->Tp[1];
+    io.k1e = ke;
+    io.k1p = kp;
+    io.t = t;
+    hipLaunchKernelGGL(k_tm<R_RHS>, dim3(hs::grid1d(s->N), 1), 256, 0, s->st, s->phys, s->tmc, io);
+}
+
+// One fixed-point update of the three equations from the slopes km, ke, kp
+// (trapezoidal with dy0 when trap); the maxima of all three in one readback.
+TmRed tm_updates(hs_solver* s, const float* km, const double* ke, const double* kp, bool trap, bool norm_in,
+                 bool with_y1, double h) {
+    const long region = (long)kImpLanes * kImpStride;
+    const long nred = (kImpVals + 2 * kImp1Vals) * region;
+    unsigned long long* rT = s->imp + kImpVals * region;
+    unsigned long long* rP = rT + kImp1Vals * region;
+    const double c = trap ? 0.5 * h : h;
+    const unsigned g = hs::grid1d(s->N);
+    hipMemsetAsync(s->imp, 0, sizeof(unsigned long long) * nred, s->st);
+    hipLaunchKernelGGL(k_imp, dim3(g, 1), hs::kBlock, 0, s->st, s->y0, s->acc, km, trap ? s->e : nullptr,
+                       with_y1 ? s->y1 : nullptr, s->mB, c, norm_in ? s->p.normalize : 0, s->p.normalize, s->imp,
+                       s->N);
+    hipLaunchKernelGGL(k_imp1, g, hs::kBlock, 0, s->st, s->Te[0], s->Te[1], ke, trap ? s->k1e : nullptr,
+                       with_y1 ? s->Tea : nullptr, s->Tf[1], c, rT, s->N);
+    hipLaunchKernelGGL(k_imp1, g, hs::kBlock, 0, s->st, s->Tp[0], s->Tp[1], kp, trap ? s->k1p : nullptr,
+                       with_y1 ? s->Tpa : nullptr, (float*)nullptr, c, rP, s->N);
+    std::vector<unsigned long long> b(nred);
+    hipMemcpyAsync(b.data(), s->imp, sizeof(unsigned long long) * nred, hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    auto mx = [&](long q) {
+        double v = 0.0;
+        for (int l = 0; l < kImpLanes; l++) {
+            double x;
+            memcpy(&x, &b[q * region + (long)l * kImpStride], sizeof(x));
+            v = std::max(v, x);
+        }
+        return v;
+    };
+    TmRed r{};
+    for (int p = 0; p < 3; p++) {
+        r.m.d[p] = mx(p);
+        r.m.a[p] = mx(3 + p);
+        r.m.d1[p] = mx(6 + p);
+    }
+    r.m.sum = mx(9);
+    r.m.bad = mx(10) != 0.0;
+    for (int e = 0; e < 2; e++) {
+        const long o = kImpVals + e * kImp1Vals;
+        r.d[e] = mx(o);
+        r.a[e] = mx(o + 1);
+        r.d1[e] = mx(o + 2);
+        r.bad = r.bad || mx(o + 3) != 0.0;
+    }
+    r.bad = r.bad || r.m.bad;
+    return r;
+}
+
+// one-component error (solve_am01.go:122-132 with COMP = 1)
+double t1_norm(double d, double a, double absErr, double relErr) {
+    return (1.0 / sqrt(1.0)) * sqrt(pow(d / (absErr + a * relErr), 2.0));
+}
+
+int run_am12_tm(hs_solver* s, long nsteps, long step0, double* dtio, double mindt, double maxdt,
+                hs_run_stats* stats) {
+    const int maxTry = 10, maxIter = 5;
+    const double headRoom = 0.8, maxIterErr = 0.1, alpha_ref = 0.6;
+    const double ab[3] = {s->imp_absT, s->imp_absT, s->imp_abs}, rl[3] = {s->imp_relT, s->imp_relT, s->imp_rel};
+    const unsigned g = hs::grid1d(s->N);
+    hs_run_stats st{};
+    double dt = *dtio;
+    // the errors of one update, equation order Te, Tp, m
+    auto errs = [&](const TmRed& r, double out[3]) {
+        out[0] = t1_norm(r.d[0], r.a[0], ab[0], rl[0]);
+        out[1] = t1_norm(r.d[1], r.a[1], ab[1], rl[1]);
+        out[2] = am_norm(r.m.d, r.m.a, ab[2], rl[2]);
+    };
+    auto sort3 = [](double* v) { std::sort(v, v + 3); };
+    for (long n = 0; n < nsteps; n++) {
+        const uint32_t step = (uint32_t)(step0 + n);
+        const double t0 = s->tm_time;
+        uint32_t a = 0;
+        // RHS of the state (time t0): m's into e (and the Euler start of m), Te's and Tp's into k1e, k1p
+        implicit_start(s, dt, step, s->Tf[0]);
+        {
+            TmIO io{};
+            io.Te = s->Te[0];
+            io.Tp = s->Tp[0];
+            io.k1e = s->k1e;
+            io.k1p = s->k1p;
+            io.t = t0;
+            hipLaunchKernelGGL(k_tm<R_RHS>, dim3(g, 1), 256, 0, s->st, s->phys, s->tmc, io);
+        }
+        st.rhs_evals++;
+        int tr = 0;
+        bool restrict_step = false, first = true;
+        double salpha[3] = {0, 0, 0}, tErr[3] = {0, 0, 0}, used = 0.0;
+        for (;;) {
+            const double h = dt;
+            if (!first)
+                hipLaunchKernelGGL(k_rk12_predict, dim3(g, 1), hs::kBlock, 0, s->st, s->y0, s->e, s->acc, s->mB, h,
+                                   s->p.normalize, s->N);
+            first = false;
+            hipLaunchKernelGGL(k_axpy1, g, hs::kBlock, 0, s->st, s->Te[0], s->k1e, s->Te[1], s->Tf[1], h, s->N);
+            hipLaunchKernelGGL(k_axpy1, g, hs::kBlock, 0, s->st, s->Tp[0], s->k1p, s->Tp[1], (float*)nullptr, h,
+                               s->N);
+            st.iterations++;
+            double er[3] = {maxIterErr, maxIterErr, maxIterErr}, alp[3], alpha = 0.0, te[3];
+            bool badIt = false;
+            TmRed r{};
+            // predictor: implicit Euler; the first slopes (at the Euler point) kept as dyE
+            for (int iter = 0;;) {
+                float* km = iter == 0 ? s->k1 : s->k2;
+                double* ke = iter == 0 ? s->k2e : s->ee;
+                double* kp = iter == 0 ? s->k2p : s->ep;
+                tm_rhs_all(s, km, ke, kp, step, a, dt, t0 + dt);
+                st.rhs_evals++;
+                r = tm_updates(s, km, ke, kp, false, true, false, h);
+                if (r.bad) return imp_fail(s, "hs_solver_run: AM12 iterate is not finite", dt, dtio, stats, st);
+                errs(r, te);
+                for (int e = 0; e < 3; e++) {
+                    alp[e] = te[e] / er[e];
+                    salpha[e] = alp[e];
+                    er[e] = te[e];
+                }
+                sort3(er);
+                sort3(alp);
+                alpha = alp[2];
+                iter++;
+                st.iterations++;
+                if (er[2] <= maxIterErr) break;
+                if (alpha >= 1.0 || iter > maxIter) {
+                    badIt = true;
+                    break;
+                }
+            }
+            if (badIt && tr == maxTry - 1)
+                return imp_fail(s, "hs_solver_run: AM12 predictor iteration did not converge", dt, dtio, stats, st);
+            if (badIt) {
+                dt = (alpha > alpha_ref) ? dt * pow(alpha_ref / alpha, 0.5) : 0.5 * dt;
+                restrict_step = true;
+                a++;
+                st.rejected++;
+                continue;
+            }
+            const double a1m[3] = {r.m.a[0], r.m.a[1], r.m.a[2]}, a1T[2] = {r.a[0], r.a[1]};
+            hipMemcpyAsync(s->y1, s->acc, sizeof(double) * 3 * s->N, hipMemcpyDeviceToDevice, s->st);
+            hipMemcpyAsync(s->Tea, s->Te[1], sizeof(double) * s->N, hipMemcpyDeviceToDevice, s->st);
+            hipMemcpyAsync(s->Tpa, s->Tp[1], sizeof(double) * s->N, hipMemcpyDeviceToDevice, s->st);
+            for (int e = 0; e < 3; e++) er[e] = maxIterErr;
+            // corrector: trapezoidal, starting from the slopes at the Euler point
+            for (int iter = 0;;) {
+                const float* km = s->k1;
+                const double *ke = s->k2e, *kp = s->k2p;
+                if (iter > 0) {
+                    tm_rhs_all(s, s->k2, s->ee, s->ep, step, a, dt, t0 + dt);
+                    st.rhs_evals++;
+                    km = s->k2;
+                    ke = s->ee;
+                    kp = s->ep;
+                }
+                r = tm_updates(s, km, ke, kp, true, iter > 0, true, h);
+                if (r.bad) return imp_fail(s, "hs_solver_run: AM12 iterate is not finite", dt, dtio, stats, st);
+                errs(r, te);
+                for (int e = 0; e < 3; e++) {
+                    alp[e] = te[e] / er[e];
+                    er[e] = te[e];
+                }
+                sort3(er);
+                sort3(alp);
+                alpha = alp[2];
+                iter++;
+                st.iterations++;
+                if (er[2] <= maxIterErr) break;
+                if (alpha >= 1.0 || iter > maxIter) {
+                    badIt = true;
+                    break;
+                }
+            }
+            if (badIt && tr == maxTry - 1)
+                return imp_fail(s, "hs_solver_run: AM12 corrector iteration did not converge", dt, dtio, stats, st);
+            if (badIt) {
+                dt = (alpha > alpha_ref) ? dt * pow(alpha_ref / alpha, 0.5) : 0.5 * dt;
+                a++;
+                st.rejected++;
+                continue;
+            }
+            // step errors (corrector - predictor) and the controller; the alpha
+            // restriction applies to the first equation only (solve_am01.go:253-258)
+            tErr[0] = t1_norm(r.d1[0], a1T[0], ab[0], rl[0]);
+            tErr[1] = t1_norm(r.d1[1], a1T[1], ab[1], rl[1]);
+            tErr[2] = am_norm(r.m.d1, a1m, ab[2], rl[2]);
+            bool bad = false;
+            double nd[3];
+            for (int e = 0; e < 3; e++) {
+                if (tErr[e] > 1.0) bad = true;
+                double x = dt * pow(headRoom / tErr[e], 0.5);
+                if (restrict_step) {
+                    x = std::min(x, dt * pow(alpha_ref / salpha[e], 0.5));
+                    restrict_step = false;
+                }
+                x = std::max(x, mindt);
+                x = std::min(x, maxdt);
+                nd[e] = x;
+            }
+            sort3(nd);
+            used = dt;
+            dt = nd[0];
+            if (!bad || nd[0] == mindt) {
+                st.t += used;
+                break;
+            }
+            if (tr > maxTry) return imp_fail(s, "hs_solver_run: AM12 did not converge", dt, dtio, stats, st);
+            tr++;
+            a++;
+            st.rejected++;
+        }
+        implicit_accept(s);
+        hipMemcpyAsync(s->Te[0], s->Te[1], sizeof(double) * s->N, hipMemcpyDeviceToDevice, s->st);
+        hipMemcpyAsync(s->Tp[0], s->Tp[1], sizeof(double) * s->N, hipMemcpyDeviceToDevice, s->st);
+        hipMemcpyAsync(s->Tf[0], s->Tf[1], sizeof(float) * s->N, hipMemcpyDeviceToDevice, s->st);
+        s->tm_time = t0 + used;
+        st.steps++;
+        st.last_err = std::max(tErr[0], std::max(tErr[1], tErr[2]));
+    }
+    *dtio = dt;
+    st.dt = dt;
+    if (stats) *stats = st;
+    return hs_last_error();
+}
 }  // namespace
+
+extern "C" int hs_solver_set_implicit_T(hs_solver* s, double maxAbsErr, double maxRelErr) {
+    if (!s || !(maxAbsErr > 0.0) || !(maxRelErr > 0.0)) {
+        hs::set_error(HS_EINVAL, "hs_solver_set_implicit_T: tolerances must be positive");
+        return HS_EINVAL;
+    }
+    s->imp_absT = maxAbsErr;
+    s->imp_relT = maxRelErr;
+    return 0;
+}
 
 extern "C" int hs_solver_set_implicit(hs_solver* s, double maxAbsErr, double maxRelErr, double maxIterErr,
                                       int maxIter) {
@@ -2248,8 +2561,15 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
     if (!s || !dtio || nsteps < 0) return HS_EINVAL;
     if (!whole_batch_ok(s, "hs_solver_run: handle is in slot mode")) return HS_EINVAL;
     if (method == HS_RK23) return run_rk23_device(s, nsteps, step0, dtio, maxErr, mindt, maxdt, stats);
+    if (s->tm && method == HS_AM12) {
+        if (!ensure_imp(s)) return HS_ENOMEM;
+        int rc = run_am12_tm(s, nsteps, step0, dtio, mindt, maxdt, stats);
+        hipStreamSynchronize(s->st);
+        HS_LAUNCH_CHECK("hs_solver_run");
+        return rc ? rc : hs_last_error();
+    }
     if (s->tm) {
-        hs::set_error(HS_EINVAL, "hs_solver_run: two-temperature solves take HS_RK23");
+        hs::set_error(HS_EINVAL, "hs_solver_run: two-temperature solves take HS_RK23 or HS_AM12");
         return HS_EINVAL;
     }
     if (method == HS_AM12 || method == HS_BDF_EULER) {

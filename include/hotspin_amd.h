@@ -174,9 +174,13 @@ HS_API int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, doub
  * hs_solver_run with HS_AM12 adapts *dt like solve_am01.go:45-300 (its
  * maxErr argument is unused); HS_BDF_EULER keeps *dt.  Both treat a batch as
  * one system (maxima over every solve) and fail with HS_ECONV where the
- * reference panics.  Not for two-temperature solves. */
+ * reference panics.  A two-temperature solve takes HS_AM12 (not
+ * HS_BDF_EULER) for its three equations in the laser recipe's order Te, Tp, m
+ * (examples/laser-demag.py:35-60), with Te_/Tp_maxAbsError and
+ * _maxRelError from hs_solver_set_implicit_T (defaults 1e-4, 1e-3). */
 HS_API int hs_solver_set_implicit(hs_solver* s, double maxAbsErr, double maxRelErr, double maxIterErr,
                                   int maxIter);
+HS_API int hs_solver_set_implicit_T(hs_solver* s, double maxAbsErr, double maxRelErr);
 /* One RHS evaluation at the current state (no step): k and H_eff to host. */
 HS_API int hs_solver_rhs(hs_solver* s, double* k_host, double* heff_host, uint32_t step, uint32_t attempt,
                          double dt);

@@ -1710,6 +1710,178 @@ int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, 
     return rc;
 }
 
+/* Error of a one-component equation (solve_am01.go:122-132 with COMP = 1). */
+static double t_norm(const double* y, const double* yb, long N, double absErr, double relErr) {
+    double d = or_maxdiff(y, yb, N), mx = or_maxabs(yb, N);
+    double t = pow(d / (absErr + mx * relErr), 2.0);
+    return (1.0 / sqrt(1.0)) * sqrt(t);
+}
+static void sort3(double* v) {
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++)
+            if (v[j] < v[i]) { double x = v[i]; v[i] = v[j]; v[j] = x; }
+}
+
+/* BDFAM12.Step, engine/solve_am01.go:45-300, for the equations of the
+ * laser recipe in their registration order (examples/laser-demag.py:35-60:
+ * temperature/ETM, temperature/PTM, then llbar): e = 0 Te, 1 Tp, 2 m, with
+ * the LLB temperature T = Te.  Every fixed-point update takes the RHS of
+ * all equations at the previous iterate (one UpdateEqRHS per iteration).
+ * As in the reference, the per-equation previous errors er[] and
+ * contractions alp[] are sorted in place after each iteration, so er[i]
+ * pairs with equation i by rank, not by name; the controller's
+ * alpha-restriction applies to the first equation only (the flag is cleared
+ * inside the loop, solve_am01.go:253-258).  absErr, relErr: per equation
+ * (Te, Tp, m).  stats: [0..2] step errors (Te, Tp, m), [3] attempts, [4]
+ * fixed-point iterations, [5] dt of the accepted step. */
+int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, double* Tp, double* t,
+                       double* dtio, const double absErr[3], const double relErr[3], double mindt, double maxdt,
+                       uint32_t step, double* stats) {
+    const long N = s->N, n3 = 3 * N;
+    const int maxTry = 10, maxIter = 5;
+    const double headRoom = 0.8, maxIterErr = 0.1, alpha_ref = 0.6;
+    double* buf = (double*)malloc(sizeof(double) * 15 * N);
+    double *Te0 = buf, *Tp0 = buf + N, *d0e = buf + 2 * N, *d0p = buf + 3 * N, *dEe = buf + 4 * N;
+    double *dEp = buf + 5 * N, *ke = buf + 6 * N, *kp = buf + 7 * N, *y1e = buf + 8 * N, *y1p = buf + 9 * N;
+    double *ybe = buf + 10 * N, *ybp = buf + 11 * N, *work = buf + 12 * N;
+    double *dy0 = s->k1, *dy = s->k2, *dyE = s->k3, *y1 = s->k4, *yb = s->ys;
+    const double* Tsave = s->p.T;
+    const double TMsave = s->p.TMul;
+    s->p.TMul = 1.0;
+    const double t0 = *t;
+    uint32_t a = 0;
+    long iterations = 0;
+    s->p.T = Te;
+    or_llb_rhs(s, m, dy0, step, a, *dtio); /* normalises m = y0 */
+    or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0, d0e, d0p, work);
+    memcpy(s->y0, m, sizeof(double) * n3);
+    memcpy(Te0, Te, sizeof(double) * N);
+    memcpy(Tp0, Tp, sizeof(double) * N);
+    int tr = 0, restrict_step = 0, rc = 1;
+    double salpha[3] = {0, 0, 0}, tErr[3] = {0, 0, 0}, used = 0.0;
+    for (;;) {
+        const double dt = *dtio, h = dt;
+        int badIt = 0;
+        double er[3], alp[3], alpha = 0.0;
+        for (long i = 0; i < N; i++) { Te[i] = Te0[i] + h * d0e[i]; Tp[i] = Tp0[i] + h * d0p[i]; }
+        for (long i = 0; i < n3; i++) m[i] = s->y0[i] + h * dy0[i];
+        iterations++;
+        s->p.T = Te;
+        or_llb_rhs(s, m, dy, step, a, dt);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, ke, kp, work);
+        memcpy(dyE, dy, sizeof(double) * n3);
+        memcpy(dEe, ke, sizeof(double) * N);
+        memcpy(dEp, kp, sizeof(double) * N);
+        for (int e = 0; e < 3; e++) er[e] = maxIterErr;
+        for (int iter = 0;;) {
+            double te;
+            for (long i = 0; i < N; i++) ybe[i] = Te0[i] + h * ke[i];
+            te = t_norm(Te, ybe, N, absErr[0], relErr[0]);
+            alp[0] = te / er[0]; salpha[0] = alp[0]; er[0] = te;
+            memcpy(Te, ybe, sizeof(double) * N);
+            for (long i = 0; i < N; i++) ybp[i] = Tp0[i] + h * kp[i];
+            te = t_norm(Tp, ybp, N, absErr[1], relErr[1]);
+            alp[1] = te / er[1]; salpha[1] = alp[1]; er[1] = te;
+            memcpy(Tp, ybp, sizeof(double) * N);
+            for (long i = 0; i < n3; i++) yb[i] = s->y0[i] + h * dy[i];
+            te = am_norm(m, yb, N, absErr[2], relErr[2]);
+            alp[2] = te / er[2]; salpha[2] = alp[2]; er[2] = te;
+            memcpy(m, yb, sizeof(double) * n3);
+            sort3(er);
+            sort3(alp);
+            alpha = alp[2];
+            iter++;
+            iterations++;
+            if (er[2] <= maxIterErr) break;
+            if (alpha >= 1.0 || iter > maxIter) { badIt = 1; break; }
+            s->p.T = Te;
+            or_llb_rhs(s, m, dy, step, a, dt);
+            or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, ke, kp, work);
+        }
+        if (badIt && tr == maxTry - 1) { rc = -1; break; }
+        if (badIt) {
+            double ha = 0.5 * dt;
+            if (alpha > alpha_ref) ha = dt * pow(alpha_ref / alpha, 0.5);
+            *dtio = ha;
+            restrict_step = 1;
+            a++;
+            continue;
+        }
+        memcpy(y1, m, sizeof(double) * n3);
+        memcpy(y1e, Te, sizeof(double) * N);
+        memcpy(y1p, Tp, sizeof(double) * N);
+        memcpy(dy, dyE, sizeof(double) * n3);
+        memcpy(ke, dEe, sizeof(double) * N);
+        memcpy(kp, dEp, sizeof(double) * N);
+        for (int e = 0; e < 3; e++) er[e] = maxIterErr;
+        for (int iter = 0;;) {
+            double te;
+            for (long i = 0; i < N; i++) ybe[i] = Te0[i] + (0.5 * h) * (ke[i] + d0e[i]);
+            te = t_norm(Te, ybe, N, absErr[0], relErr[0]);
+            alp[0] = te / er[0]; er[0] = te;
+            memcpy(Te, ybe, sizeof(double) * N);
+            for (long i = 0; i < N; i++) ybp[i] = Tp0[i] + (0.5 * h) * (kp[i] + d0p[i]);
+            te = t_norm(Tp, ybp, N, absErr[1], relErr[1]);
+            alp[1] = te / er[1]; er[1] = te;
+            memcpy(Tp, ybp, sizeof(double) * N);
+            for (long i = 0; i < n3; i++) yb[i] = s->y0[i] + (0.5 * h) * (dy[i] + dy0[i]);
+            te = am_norm(m, yb, N, absErr[2], relErr[2]);
+            alp[2] = te / er[2]; er[2] = te;
+            memcpy(m, yb, sizeof(double) * n3);
+            sort3(er);
+            sort3(alp);
+            alpha = alp[2];
+            iter++;
+            iterations++;
+            if (er[2] <= maxIterErr) break;
+            if (alpha >= 1.0 || iter > maxIter) { badIt = 1; break; }
+            s->p.T = Te;
+            or_llb_rhs(s, m, dy, step, a, dt);
+            or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, ke, kp, work);
+        }
+        if (badIt && tr == maxTry - 1) { rc = -1; break; }
+        if (badIt) {
+            double ha = 0.5 * dt;
+            if (alpha > alpha_ref) ha = dt * pow(alpha_ref / alpha, 0.5);
+            *dtio = ha;
+            a++;
+            continue;
+        }
+        tErr[0] = t_norm(Te, y1e, N, absErr[0], relErr[0]);
+        tErr[1] = t_norm(Tp, y1p, N, absErr[1], relErr[1]);
+        tErr[2] = am_norm(m, y1, N, absErr[2], relErr[2]);
+        int bad = 0;
+        double nd[3];
+        for (int e = 0; e < 3; e++) {
+            if (tErr[e] > 1.0) bad = 1;
+            double x = dt * pow(headRoom / tErr[e], 0.5);
+            if (restrict_step) {
+                double ha = dt * pow(alpha_ref / salpha[e], 0.5);
+                x = fmin(x, ha);
+                restrict_step = 0;
+            }
+            if (x < mindt) x = mindt;
+            if (x > maxdt) x = maxdt;
+            nd[e] = x;
+        }
+        sort3(nd);
+        *dtio = nd[0];
+        used = dt;
+        if (!bad || nd[0] == mindt) { *t = t0 + dt; break; }
+        if (tr > maxTry) { rc = -1; break; }
+        tr++;
+        a++;
+    }
+    s->p.T = Tsave;
+    s->p.TMul = TMsave;
+    free(buf);
+    if (stats) {
+        stats[0] = tErr[0]; stats[1] = tErr[1]; stats[2] = tErr[2];
+        stats[3] = tr + 1; stats[4] = (double)iterations; stats[5] = used;
+    }
+    return rc;
+}
+
 /* EnergyUpdater.Update, modules/mod_energy.go:98-116 */
 double or_energy_term(const double* m, const double* H, const double mMul[3], const double HMul[3],
                       const double* msat, double msatMul, double weight, long N) {

@@ -229,6 +229,11 @@ void or_tm_rhs(const or_tm_params* q, const int N[3], const double cell[3], cons
  * the accepted attempt (m, Te, Tp), stats[3] = attempts. */
 int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, double* Tp, double* t,
                        double* dtio, double maxErr, double mindt, double maxdt, uint32_t step, double* stats);
+/* One AM12 step of (Te, Tp, m) (solve_am01.go:45-300) with T = Te; absErr, relErr per
+ * equation in the order Te, Tp, m; stats[6]: errors (Te, Tp, m), attempts, iterations, dt used. */
+int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, double* Tp, double* t,
+                       double* dtio, const double absErr[3], const double relErr[3], double mindt, double maxdt,
+                       uint32_t step, double* stats);
 
 /* energy term: weight*Ms0T0Mul*sum((mMul.m).(HMul.H)*msat) -- mod_energy.go:98-116 */
 double or_energy_term(const double* m, const double* H, const double mMul[3], const double HMul[3],

@@ -80,6 +80,8 @@ def _declare(L):
     L.or_tm_rhs.argtypes = [v, v, v, v, v, v, d, v, v, v]
     L.or_llbtm_step_rk23.argtypes = [v, v, v, v, v, v, v, d, d, d, u32, v]
     L.or_llbtm_step_rk23.restype = i
+    L.or_llbtm_step_am12.argtypes = [v, v, v, v, v, v, v, v, v, d, d, u32, v]
+    L.or_llbtm_step_am12.restype = i
     L.or_llb_heff.argtypes = [v]
     L.or_llb_heff.restype = dp
     L.or_llb_bdemag.argtypes = [v]
@@ -311,6 +313,25 @@ class LLB:
         if rc < 0:
             raise RuntimeError("oracle backward-Euler iterator did not converge")
         return st
+
+    def step_tm_am12(self, q, m, Te, Tp, t, dt, step=0, absErr=(1e-4, 1e-4, 1e-4), relErr=(1e-3, 1e-3, 1e-3),
+                     mindt=1e-38, maxdt=1e38):
+        """One AM12 step of (Te, Tp, m) with T = Te (arrays modified in place);
+        tolerances per equation (Te, Tp, m).  Returns (dt_next, t_next,
+        stats[err_Te, err_Tp, err_m, attempts, iterations, dt_used])."""
+        for a in (m, Te, Tp):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        dtb = np.array([dt], np.float64)
+        tb = np.array([t], np.float64)
+        ab = np.array(absErr, np.float64)
+        rl = np.array(relErr, np.float64)
+        st = np.zeros(6, np.float64)
+        rc = lib().or_llbtm_step_am12(self._h, C.byref(q), ptr(m), ptr(Te), ptr(Tp), tb.ctypes.data, dtb.ctypes.data,
+                                      ab.ctypes.data, rl.ctypes.data, float(mindt), float(maxdt), int(step),
+                                      st.ctypes.data)
+        if rc < 0:
+            raise RuntimeError("oracle AM12 step did not converge")
+        return float(dtb[0]), float(tb[0]), st
 
     def step_tm(self, q, m, Te, Tp, t, dt, step=0, maxErr=1e-5, mindt=1e-38, maxdt=1e38):
         """One adaptive RK23 step of (m, Te, Tp) with T = Te (arrays modified in

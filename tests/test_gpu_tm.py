@@ -85,3 +85,37 @@ def test_coupled_rk23_replays_oracle_steps():
         assert np.abs(teg - Te).max() <= 1e-6, (i, np.abs(teg - Te).max())
         assert np.abs(tpg - Tp).max() <= 1e-6, i
     assert Te.max() > 450.0  # the pump has heated the spot
+
+
+def test_coupled_am12_matches_oracle_step_by_step():
+    """AM12 of (Te, Tp, m) (the reference's laser recipe solver) from the same
+    state and dt each step: the same fixed-point iteration counts, accepted dt
+    within 1e-3, max|dm| <= 1e-5 and |dTe|, |dTp| <= 1e-6 K."""
+    tm, kw, m, _, _ = _problem()
+    T0 = 300.0
+    ncell = N[0] * N[1] * N[2]
+    Te, Tp = np.full(ncell, T0), np.full(ncell, T0)
+    q = po.tm_params(**tm)
+    o = po.LLB(N, CELL, **kw)
+    g = TMSolver(N, CELL, tm=tm, T0=T0, **kw)
+    ab, rl = (1e-3, 1e-3, 1e-4), (1e-5, 1e-5, 1e-3)
+    g.set_implicit(maxAbsErr=ab[2], maxRelErr=rl[2])
+    g.set_implicit_T(maxAbsErr=ab[0], maxRelErr=rl[0])
+    mo, t, dt = m.copy(), 150e-15, 2e-16
+    for i in range(12):
+        m_b, Te_b, Tp_b, t_b, dt_in = mo.copy(), Te.copy(), Tp.copy(), t, dt
+        dt, t, st = o.step_tm_am12(q, mo, Te, Tp, t, dt, i, absErr=ab, relErr=rl, mindt=1e-20, maxdt=5e-14)
+        g.set_m(m_b)
+        g.set_T(Te_b, Tp_b)
+        g.time = t_b
+        dtg, stg = g.run("am12", 1, dt_in, mindt=1e-20, maxdt=5e-14, step0=i)
+        assert stg.iterations == int(st[4]), (i, stg.iterations, st)
+        assert abs(stg.t - st[5]) <= 1e-3 * st[5], (i, stg.t, st)
+        assert abs(dtg - dt) <= 1e-3 * dt, (i, dtg, dt)
+        teg, tpg = g.get_T()
+        assert np.abs(g.get_m() - mo).max() <= 1e-5, i
+        assert np.abs(teg - Te).max() <= 1e-6, (i, np.abs(teg - Te).max())
+        assert np.abs(tpg - Tp).max() <= 1e-6, i
+        # continue both from the oracle's state
+        Te, Tp = Te.copy(), Tp.copy()
+    assert Te.max() > 300.2  # the pump is on
