@@ -260,7 +260,7 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--no-ensemble", action="store_true")
