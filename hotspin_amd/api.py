@@ -220,7 +220,7 @@ class Batch(Solver):
 class TmParams(C.Structure):
     """hs_tm_params of include/hotspin_amd.h."""
     _fields_ = [(n, C.c_double) for n in ("gammaE", "Cl", "G", "kappa", "Q0", "sigma_r", "sigma_t", "t0", "xc",
-                                          "yc", "maxErrT")]
+                                          "yc", "maxErrT", "Td", "nD")] + [("spin_heat", C.c_int)]
 
 
 class TMSolver(Solver):
@@ -233,7 +233,7 @@ class TMSolver(Solver):
         self.params, self._keep = make_params(N, cell, pbc, **kw)
         self.tm = TmParams()
         for k, v in (tm or {}).items():
-            setattr(self.tm, k, float(v))
+            setattr(self.tm, k, int(v) if k == "spin_heat" else float(v))
         self._h = lib().hs_solver_create_tm(C.byref(self.params), C.byref(self.tm), float(T0), stream)
         if not self._h:
             check("hs_solver_create_tm")

@@ -31,6 +31,7 @@ extern "C" int hs_demag_reserve(hs_demag* d, int nsolve);
 namespace {
 
 constexpr int TX = 64, TY = 4;
+constexpr double kPiD = 3.14159265358979323846;
 
 // Per-solve parameters of a batch (solves share the mesh and materials).
 struct SolveP {
@@ -90,6 +91,7 @@ struct Phys {
     float sqa[3];  // sqrt(aMul[c]): the thermal amplitude of a component without an alpha mask
     const float* alpha[3];
     LfConst lf;
+    double sh_w, sh_q;  // spin heating: 3 kB TcMul nMul gammaLL and mu0 Ms0T0Mul gammaLL
 };
 
 // Roles of the fused stage kernel.
@@ -113,6 +115,7 @@ struct StageIO {
     float* hout;      // R_RHS: H_eff
     unsigned long long* errmax;  // max |e| as double bits: kErrLanes lanes per solve
     const float* T;   // two-temperature solves: Te of the stage input (FP32, [solve][N]); overrides the T mask
+    float* qs;        // two-temperature solves with spin heating: the spin term of Qe per cell (out)
     double h;
     uint32_t step, attempt;
     const Clock* clk;  // when set, step/attempt/h of each solve are read from here
@@ -130,6 +133,7 @@ __device__ __forceinline__ StageIO solve_io(const StageIO& io, int sol, long N) 
     if (r.k1) r.k1 += b;
     if (r.k2) r.k2 += b;
     if (r.k3) r.k3 += b;
+    if (r.qs) r.qs += (long)sol * N;
     if (r.mout) r.mout += b;
     if (r.kout) r.kout += b;
     if (r.hout) r.hout += b;
@@ -284,7 +288,7 @@ __device__ __forceinline__ void preload(const Phys& P, const StageIO& io, long I
     q.ms = P.msk ? P.msk[I] : 1.f;
     if (P.demag)
         for (int c = 0; c < 3; c++) q.B[c] = io.B[c * N + I];
-    if (P.longfield) {
+    if (P.longfield || io.qs) {
         q.J = P.J ? P.J[I] : 1.f;
         q.n = P.n ? P.n[I] : 1.f;
         q.Tc = P.Tc ? P.Tc[I] : 1.f;
@@ -400,6 +404,20 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
         H[0] += pre * (s0 * x0);
         H[1] += pre * (s1 * x1);
         H[2] += pre * (s2 * x2);
+    }
+    if (io.qs) {  // spin heating of a two-temperature solve: ws - qs with R = diag(alpha) H_eff
+        double qv = 0.0;
+        if (ms != 0.f && P.local) {  // R is the local damping term: zero without it
+            float R[3];
+            for (int c = 0; c < 3; c++) R[c] = (P.aMul[c] * q.al[c]) * H[c];
+            const double mR = (double)m0[0] * R[0] + (double)m0[1] * R[1] + (double)m0[2] * R[2];
+            const double HR = (double)H[0] * R[0] + (double)H[1] * R[1] + (double)H[2] * R[2];
+            const double S = (double)P.JMul * q.J, Tc = q.Tc, n = q.n;
+            const double ws =
+                (n == 0.0 || S == 0.0 || Tc == 0.0) ? 0.0 : P.sh_w * (Tc * S * S * n / (S * (S + 1.0))) * mR;
+            qv = ws + P.sh_q * HR * (double)ms;  // ws - qs, qs = -mu0 Ms0T0 gamma (H . R) msk
+        }
+        io.qs[I] = (float)qv;
     }
     kk[0] = kk[1] = kk[2] = 0.f;
     if (ms != 0.f) {
@@ -762,7 +780,7 @@ __device__ __forceinline__ void preload2(const Phys& P, const StageIO& io, long 
             v = ld2(io.B, c * N + I);
             q[0].B[c] = v.x, q[1].B[c] = v.y;
         }
-    if (P.longfield) {
+    if (P.longfield || io.qs) {
         v = ld2m(P.J, I);
         q[0].J = v.x, q[1].J = v.y;
         v = ld2m(P.n, I);
@@ -1163,6 +1181,7 @@ __global__ void k_fixup(const Clock* clk, const double* ycur, const float* mcur,
 // ---- two-temperature model ------------------------------------------------
 struct TmP {
     double gammaE, Cl, G, kappa, Q0, ir2, it2, t0, xc, yc;  // ir2 = 1/(2 sigma_r^2), it2 = 1/(2 sigma_t^2)
+    double Td, c9nkB;  // Debye lattice capacity (Td > 0): Cp = c9nkB * debye_cp(Tp, Td), c9nkB = 9 nD kB
     double c2x, c2y, c2z;  // 1/cell^2 of internal axes 0, 1, 2 (Qspat.cu's x, y, z)
     double hx, hy;         // cell sizes of the user x (internal 2) and y (internal 1) axes
 };
@@ -1177,7 +1196,26 @@ struct TmIO {
     const Clock* clk;
     const SolveP* sp;
     double t;                     // laser time of stage-1 (no clock: testing)
+    const float* qs;              // spin term of Qe per cell at the stage input (or null)
 };
+
+// Debye capacity of one cell with n = 1, as cpKern (Cp.cu:14-83): (T/Td)^3 times
+// the 61-interval trapezoid of x^4 e^x/(e^x-1)^2 over [0, Td/T]; below
+// T/Td = 0.01 the T^3 limit.  e^{x_j} by repeated products of e^h (one exp).
+__device__ double debye_cp(double T, double Td) {
+    const double xx = Td / T, ixx = T / Td;
+    if (ixx < 0.01) return (12.0 / 5.0) * (1.0 / 9.0) * (kPiD * kPiD * kPiD * kPiD) * (ixx * ixx * ixx);
+    const double h = xx / 61.0, eh = exp(h);
+    double e = 1.0, inner = 0.0;
+    for (int j = 1; j < 61; j++) {
+        e *= eh;
+        const double x = j * h, em1 = e - 1.0;
+        inner += (x * x * x * x * e) / (em1 * em1);
+    }
+    const double ex = exp(xx), exm1 = ex - 1.0;
+    const double end = (xx * xx * xx * xx * ex) / (exm1 * exm1);
+    return ((0.0 + end + 2.0 * inner) * (0.5 * h)) / (xx * xx * xx);
+}
 
 // Q_e = kappa lap(Te) (Qspat.cu:13-134 with uniform conductance, neighbour
 // order X, Z, Y as there) + G (Tp - Te) (Qinter.cu:13-45) + Q_laser;
@@ -1218,8 +1256,10 @@ __global__ void __launch_bounds__(256) k_tm(Phys P, TmP Q, TmIO io) {
         }
         const double x = (k + 0.5) * Q.hx - Q.xc, y = (j + 0.5) * Q.hy - Q.yc, dt0 = ts - Q.t0;
         const double ql = Q.Q0 * exp(-(x * x + y * y) * Q.ir2) * exp(-(dt0 * dt0) * Q.it2);
-        const double qe = Q.kappa * ddT + Q.G * (Tp - T0) + ql;
-        const double ke = qe / (Q.gammaE * T0), kp = Q.G * (T0 - Tp) / Q.Cl;
+        double qe = Q.kappa * ddT + Q.G * (Tp - T0) + ql;
+        if (io.qs) qe += (double)io.qs[b + I];
+        const double cl = Q.Td > 0.0 ? Q.c9nkB * debye_cp(Tp, Q.Td) : Q.Cl;
+        const double ke = qe / (Q.gammaE * T0), kp = Q.G * (T0 - Tp) / cl;
         const long J = b + I;
         if constexpr (ROLE == R_RHS) {
             io.k1e[J] = ke;
@@ -1346,6 +1386,7 @@ struct hs_solver {
     double imp_abs = 1e-4, imp_rel = 1e-3, imp_iter_err = 1e-5;
     int imp_max_iter = 3;
     double imp_absT = 1e-4, imp_relT = 1e-3;  // Te and Tp (two-temperature solves)
+    float* qs = nullptr;  // spin heating: the spin term of Qe per cell, written by the LLB stages
 };
 
 namespace {
@@ -1571,6 +1612,7 @@ void launch_tm(hs_solver* s, const double* Te, const double* Tp, int par, double
     t.errmax = s->errmax;
     t.clk = s->clk;
     t.sp = s->sp;
+    t.qs = s->qs;
     hipLaunchKernelGGL(k_tm<ROLE>, dim3(hs::grid1d(s->N), s->nsolve), 256, 0, s->st, s->phys, s->tmc, t);
 }
 
@@ -1582,6 +1624,7 @@ void slot_iteration(hs_solver* s, int par) {
     io.y0 = cur.y;
     io.acc = nxt.y;
     io.m = cur.m; io.B = cur.B; io.mout = nxt.m;
+    io.qs = s->qs;  // spin heating: each LLB stage leaves the spin term for the k_tm of its stage
     if (s->tm) io.T = s->Tf[par];
     launch_stage<R_BS_1>(s, io);
     if (s->tm) launch_tm<R_BS_1>(s, s->Te[par], s->Tp[par], par, s->Tea, s->Tpa, s->Tfa);
@@ -1939,7 +1982,7 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
     if (s->slot_host) hipHostFree(s->slot_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
-                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1};
+                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1, s->qs};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
@@ -1986,9 +2029,10 @@ extern "C" int hs_solver_rhs(hs_solver* s, double* k_host, double* h_host, uint3
                              double dt) {
     if (!s) return HS_EINVAL;
     const long NN = s->N * s->nsolve;
-    if (!s->kout) {
-        hipMalloc(&s->kout, sizeof(float) * 3 * NN);
-        hipMalloc(&s->hout, sizeof(float) * 3 * NN);
+    if (!s->kout && (hipMalloc(&s->kout, sizeof(float) * 3 * NN) != hipSuccess ||
+                     hipMalloc(&s->hout, sizeof(float) * 3 * NN) != hipSuccess)) {
+        hs::set_error(HS_ENOMEM, "hs_solver_rhs: out of device memory");
+        return HS_ENOMEM;
     }
     ensure_b0(s);
     StageIO io = base_io(s, dt, step, attempt);
@@ -2059,7 +2103,8 @@ bool ensure_imp(hs_solver* s) {
 
 // LLB RHS of the iterate (FP32 stage input mB, demag first) into k_out; T:
 // the FP32 temperature of a two-temperature solve's iterate
-void imp_rhs(hs_solver* s, float* k_out, uint32_t step, uint32_t attempt, double dt, const float* T) {
+void imp_rhs(hs_solver* s, float* k_out, uint32_t step, uint32_t attempt, double dt, const float* T,
+             float* qs = nullptr) {
     demag_into(s, s->mB, s->Bs);
     StageIO io = base_io(s, dt, step, attempt);
     io.m = s->mB;
@@ -2068,6 +2113,7 @@ void imp_rhs(hs_solver* s, float* k_out, uint32_t step, uint32_t attempt, double
     io.hout = nullptr;
     io.mout = nullptr;
     io.T = T;
+    io.qs = qs;
     launch_stage<R_RHS>(s, io);
 }
 
@@ -2124,13 +2170,14 @@ int imp_fail(hs_solver* s, const char* what, double dt, double* dtio, hs_run_sta
 
 // Explicit-Euler start of every step: dy0 = f(y0) into e (y0 normalised in
 // place with the normaliser), acc = y0 + dt dy0, mB its stage input.
-void implicit_start(hs_solver* s, double dt, uint32_t step, const float* T = nullptr) {
+void implicit_start(hs_solver* s, double dt, uint32_t step, const float* T = nullptr, float* qs = nullptr) {
     ensure_b0(s);
     StageIO io = base_io(s, dt, step, 0);
     io.m = s->mA;
     io.B = s->B0;
     io.mout = s->mB;
     io.T = T;
+    io.qs = qs;
     launch_stage<R_RK12_1>(s, io);
 }
 
@@ -2300,13 +2347,14 @@ struct TmRed {
 // RHS of all three equations at the iterate (time t): m's into km, Te's and Tp's into ke, kp
 void tm_rhs_all(hs_solver* s, float* km, double* ke, double* kp, uint32_t step, uint32_t attempt, double dt,
                 double t) {
-    imp_rhs(s, km, step, attempt, dt, s->Tf[1]);
+    imp_rhs(s, km, step, attempt, dt, s->Tf[1], s->qs);
     TmIO io{};
     io.Te = s->Te[1];
     io.Tp = s->Tp[1];
     io.k1e = ke;
     io.k1p = kp;
     io.t = t;
+    io.qs = s->qs;
     hipLaunchKernelGGL(k_tm<R_RHS>, dim3(hs::grid1d(s->N), 1), 256, 0, s->st, s->phys, s->tmc, io);
 }
 
@@ -2384,7 +2432,7 @@ int run_am12_tm(hs_solver* s, long nsteps, long step0, double* dtio, double mind
         const double t0 = s->tm_time;
         uint32_t a = 0;
         // RHS of the state (time t0): m's into e (and the Euler start of m), Te's and Tp's into k1e, k1p
-        implicit_start(s, dt, step, s->Tf[0]);
+        implicit_start(s, dt, step, s->Tf[0], s->qs);
         {
             TmIO io{};
             io.Te = s->Te[0];
@@ -2392,6 +2440,7 @@ int run_am12_tm(hs_solver* s, long nsteps, long step0, double* dtio, double mind
             io.k1e = s->k1e;
             io.k1p = s->k1p;
             io.t = t0;
+            io.qs = s->qs;
             hipLaunchKernelGGL(k_tm<R_RHS>, dim3(g, 1), 256, 0, s->st, s->phys, s->tmc, io);
         }
         st.rhs_evals++;
@@ -2891,7 +2940,8 @@ extern "C" void hs_ensemble_destroy(hs_ensemble* e) {
 // ---------------------------------------------------------------- two-temperature solves
 extern "C" hs_solver* hs_solver_create_tm(const hs_llb_params* p, const hs_tm_params* tm, double T0,
                                           CUstream stream) {
-    if (!p || !tm || !(tm->gammaE > 0.0) || !(tm->Cl > 0.0) || !(T0 > 0.0) || !(tm->maxErrT > 0.0)) {
+    if (!p || !tm || !(tm->gammaE > 0.0) || !(tm->Cl > 0.0 || (tm->Td > 0.0 && tm->nD > 0.0)) || !(T0 > 0.0) ||
+        !(tm->maxErrT > 0.0)) {
         hs::set_error(HS_EINVAL, "hs_solver_create_tm: bad parameters");
         return nullptr;
     }
@@ -2912,6 +2962,10 @@ extern "C" hs_solver* hs_solver_create_tm(const hs_llb_params* p, const hs_tm_pa
     c.c2y = 1.0 / (p->cell[1] * p->cell[1]);
     c.c2z = 1.0 / (p->cell[2] * p->cell[2]);
     c.hx = p->cell[2], c.hy = p->cell[1];
+    c.Td = tm->Td;
+    c.c9nkB = 9.0 * tm->nD * HS_KB;
+    s->phys.sh_w = 3.0 * HS_KB * p->TcMul * p->nMul * p->gammaLL;
+    s->phys.sh_q = HS_MU0 * p->msat0T0Mul * p->gammaLL;
     const long N = s->N;
     double** d64[] = {&s->Te[0], &s->Te[1], &s->Tp[0], &s->Tp[1], &s->Tea, &s->Tpa, &s->Teb, &s->Tpb,
                       &s->k1e, &s->k1p, &s->k2e, &s->k2p, &s->ee, &s->ep};
@@ -2919,6 +2973,7 @@ extern "C" hs_solver* hs_solver_create_tm(const hs_llb_params* p, const hs_tm_pa
     bool ok = true;
     for (auto q2 : d64) ok = ok && hipMalloc(q2, sizeof(double) * N) == hipSuccess;
     for (auto q2 : d32) ok = ok && hipMalloc(q2, sizeof(float) * N) == hipSuccess;
+    if (tm->spin_heat) ok = ok && hipMalloc(&s->qs, sizeof(float) * N) == hipSuccess;
     if (!ok) {
         hs::set_error(HS_ENOMEM, "hs_solver_create_tm: out of device memory");
         hs_solver_destroy(s);
@@ -2971,12 +3026,23 @@ extern "C" int hs_solver_tm_rhs(hs_solver* s, double t, double* dTe, double* dTp
         hs::set_error(HS_EINVAL, "hs_solver_tm_rhs: not a two-temperature solver");
         return HS_EINVAL;
     }
+    if (s->qs) {  // spin heating: the LLB RHS of the state (step 0, dt 1e-15) leaves the spin term of Qe
+        ensure_b0(s);
+        StageIO sio = base_io(s, 1e-15, 0, 0);
+        sio.m = s->mA;
+        sio.B = s->B0;
+        sio.kout = sio.hout = nullptr;
+        sio.T = s->Tf[0];
+        sio.qs = s->qs;
+        launch_stage<R_RHS>(s, sio);
+    }
     TmIO io{};
     io.Te = s->Te[0];
     io.Tp = s->Tp[0];
     io.k1e = s->k1e;
     io.k1p = s->k1p;
     io.t = t;
+    io.qs = s->qs;
     hipLaunchKernelGGL(k_tm<R_RHS>, dim3(hs::grid1d(s->N), 1), 256, 0, s->st, s->phys, s->tmc, io);
     if (dTe) hipMemcpyAsync(dTe, s->k1e, sizeof(double) * s->N, hipMemcpyDeviceToHost, s->st);
     if (dTp) hipMemcpyAsync(dTp, s->k1p, sizeof(double) * s->N, hipMemcpyDeviceToHost, s->st);

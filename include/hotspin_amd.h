@@ -217,6 +217,16 @@ typedef struct {
     double Q0;      /* laser peak power density, W/m^3 */
     double sigma_r, sigma_t, t0, xc, yc;
     double maxErrT; /* K */
+    /* Td > 0: lattice heat capacity Cp(Tp) = 9 nD kB (Tp/Td)^3 int_0^{Td/Tp} x^4 e^x/(e^x-1)^2 dx
+     * (temperature/Debye, mod_cp.go:75-85 with Cp.cu's 61-interval rule) in place of Cl */
+    double Td, nD;
+    /* 1: Qe also takes the spin system's energy flow minus its dissipation,
+     * ws - qs with ws = 3 kB Tc n gammaLL S/(S+1) (m . R) and
+     * qs = -mu0 Ms0T0 gammaLL (H_eff . R), R = diag(alpha) H_eff the local
+     * damping term, zero when local00nc is off (laser-demag.py:70-88;
+     * mod_molecular-energy-density-rate.go, mod_energy-density-dissipation-rate.go).
+     * The LLB stage forms it from its FP32 field; Cl may be 0 when Td > 0. */
+    int spin_heat;
 } hs_tm_params;
 /* Resident solver with two-temperature coupling (the LLB T and T mask of p
  * are ignored: T = Te per cell).  Te, Tp start uniform at T0. */

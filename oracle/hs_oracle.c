@@ -1591,8 +1591,24 @@ int or_llb_step_bdf_euler(or_llb* s, double* m, double dt, uint32_t step, double
 /* ------------------------------------------------------------------ two-temperature model */
 /* Qe summed in the order Qspat, Qe-p, Qlaser; Qspat = or_qspat with a
  * uniform conductance (Qspat.cu:13-134), Qe-p = or_qinter (Qinter.cu:13-45). */
+/* Debye Cp of one cell with n = 1 (cpKern, Cp.cu:14-83, as or_cp) */
+static double cp_debye(double T, double Td) {
+    double out;
+    or_cp(&out, &T, NULL, NULL, Td, 1);
+    return out;
+}
+
+/* Spin contribution to Qe at the state of the last or_llb_rhs call on m
+ * (H = s->H, R = llbar_local00nc = s->loc): the molecular energy-density
+ * rate ws (mod_molecular-energy-density-rate.go:77-101 with energyFlowKern,
+ * energy_flow.cu:15-48) minus the dissipation function qs
+ * (mod_energy-density-dissipation-rate.go:71-92), laser-demag.py:84-86:
+ *   ws = 3 kB TcMul nMul gammaLL [Tc S^2 n / (S (S+1))] (m . R)
+ *   qs = -mu0 msat0T0Mul gammaLL (H . R) msat0T0 */
+static void or_spin_heat(const or_llb* s, const double* m, double* Qs);
+
 void or_tm_rhs(const or_tm_params* q, const int N[3], const double cell[3], const int pbc[3], const double* Te,
-               const double* Tp, double t, double* dTe, double* dTp, double* work) {
+               const double* Tp, double t, const double* Qs, double* dTe, double* dTp, double* work) {
     long n = (long)N[0] * N[1] * N[2];
     or_qspat(work, Te, NULL, q->kappa, N[0], N[1], N[2], cell[0], cell[1], cell[2], pbc[0], pbc[1], pbc[2]);
     const double ir2 = q->sigma_r > 0 ? 1.0 / (2.0 * q->sigma_r * q->sigma_r) : 0.0;
@@ -1603,11 +1619,36 @@ void or_tm_rhs(const or_tm_params* q, const int N[3], const double cell[3], cons
         const int j = (int)((I / N[2]) % N[1]), k = (int)(I % N[2]);
         const double x = (k + 0.5) * cell[2] - q->xc, y = (j + 0.5) * cell[1] - q->yc;
         const double ql = q->Q0 * exp(-(x * x + y * y) * ir2) * exp(-(d0 * d0) * it2);
-        const double qe = work[I] + q->G * (Tp[I] - Te[I]) + ql;
+        double qe = work[I] + q->G * (Tp[I] - Te[I]) + ql;
+        if (Qs) qe += Qs[I];
+        const double cl = q->Td > 0.0 ? (9.0 * q->nD * OR_KB) * cp_debye(Tp[I], q->Td) : q->Cl;
         dTe[I] = qe / (q->gammaE * Te[I]);
-        dTp[I] = q->G * (Te[I] - Tp[I]) / q->Cl;
+        dTp[I] = q->G * (Te[I] - Tp[I]) / cl;
     }
 }
+
+static void or_spin_heat(const or_llb* s, const double* m, double* Qs) {
+    const or_llb_params* p = &s->p;
+    const long N = s->N;
+    const double cw = 3.0 * OR_KB * p->TcMul * p->nMul * p->gammaLL;
+    const double cq = OR_MU0 * p->msat0T0Mul * p->gammaLL;
+    for (long i = 0; i < N; i++) {
+        const double* R = s->loc;
+        const double* H = s->H;
+        double mR = m[i] * R[i] + m[N + i] * R[N + i] + m[2 * N + i] * R[2 * N + i];
+        double HR = H[i] * R[i] + H[N + i] * R[N + i] + H[2 * N + i] * R[2 * N + i];
+        double S = p->JMul * mask1(p->J, i), Tc = mask1(p->Tc, i), n = mask1(p->n, i);
+        double ws = (n == 0.0 || S == 0.0 || Tc == 0.0) ? 0.0 : cw * (Tc * S * S * n / (S * (S + 1.0))) * mR;
+        double qs = -cq * HR * mask1(p->msat0T0, i);
+        Qs[i] = ws - qs;
+    }
+}
+
+void or_llb_spin_heat(const or_llb* s, const double* m, double* Qs) { or_spin_heat(s, m, Qs); }
+
+/* the spin term of Qe at the state of the or_llb_rhs call just made on X
+ * (or NULL without spin heating); needs a buffer Qsb of N doubles in scope */
+#define SPIN_QS(X) (q->spin_heat ? (or_spin_heat(s, (X), Qsb), (const double*)Qsb) : (const double*)NULL)
 
 static double maxabs_err(const double* k1, const double* k2, const double* k3, const double* k4, double h, long n) {
     double e = 0;
@@ -1623,8 +1664,9 @@ int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, 
     const long N = s->N, n3 = 3 * N;
     const int maxTry = 10;
     const double headRoom = 0.8;
-    double* buf = (double*)malloc(sizeof(double) * 14 * N);
+    double* buf = (double*)malloc(sizeof(double) * 15 * N);
     double *Te0 = buf, *Tp0 = buf + N, *Tes = buf + 2 * N, *Tps = buf + 3 * N, *work = buf + 4 * N;
+    double* Qsb = buf + 14 * N;
     double* ke[4] = {buf + 5 * N, buf + 6 * N, buf + 7 * N, buf + 8 * N};
     double* kp[4] = {buf + 9 * N, buf + 10 * N, buf + 11 * N, buf + 12 * N};
     memcpy(Te0, Te, sizeof(double) * N);
@@ -1643,21 +1685,21 @@ int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, 
         s->p.T = Te0;
         or_llb_rhs(s, m, s->k1, step, (uint32_t)tr, dt); /* normalises y0 */
         if (tr == 0) memcpy(s->y0, m, sizeof(double) * n3);
-        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te0, Tp0, *t + cs[0] * h, ke[0], kp[0], work);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te0, Tp0, *t + cs[0] * h, SPIN_QS(m), ke[0], kp[0], work);
         /* stage 2 at y0 + h/2 k1 */
         axpy_to(s->ys, m, 0.5 * h, s->k1, n3);
         axpy_to(Tes, Te0, 0.5 * h, ke[0], N);
         axpy_to(Tps, Tp0, 0.5 * h, kp[0], N);
         s->p.T = Tes;
         or_llb_rhs(s, s->ys, s->k2, step, (uint32_t)tr, dt);
-        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[1] * h, ke[1], kp[1], work);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[1] * h, SPIN_QS(s->ys), ke[1], kp[1], work);
         /* stage 3 at y0 + 3h/4 k2 */
         axpy_to(s->ys, m, 0.75 * h, s->k2, n3);
         axpy_to(Tes, Te0, 0.75 * h, ke[1], N);
         axpy_to(Tps, Tp0, 0.75 * h, kp[1], N);
         s->p.T = Tes;
         or_llb_rhs(s, s->ys, s->k3, step, (uint32_t)tr, dt);
-        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[2] * h, ke[2], kp[2], work);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[2] * h, SPIN_QS(s->ys), ke[2], kp[2], work);
         /* y1 and stage 4 at (y1, t + h) */
         for (long i = 0; i < n3; i++)
             m[i] = m[i] + h * ((2.0 / 9.0) * s->k1[i] + (1.0 / 3.0) * s->k2[i] + (4.0 / 9.0) * s->k3[i]);
@@ -1667,7 +1709,7 @@ int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, 
         }
         s->p.T = Tes;
         or_llb_rhs(s, m, s->k4, step, (uint32_t)tr, dt); /* normalises y1 */
-        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[3] * h, ke[3], kp[3], work);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Tes, Tps, *t + cs[3] * h, SPIN_QS(m), ke[3], kp[3], work);
         err[0] = maxabs_err(s->k1, s->k2, s->k3, s->k4, h, n3);
         err[1] = maxabs_err(ke[0], ke[1], ke[2], ke[3], h, N);
         err[2] = maxabs_err(kp[0], kp[1], kp[2], kp[3], h, N);
@@ -1740,7 +1782,8 @@ int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, 
     const long N = s->N, n3 = 3 * N;
     const int maxTry = 10, maxIter = 5;
     const double headRoom = 0.8, maxIterErr = 0.1, alpha_ref = 0.6;
-    double* buf = (double*)malloc(sizeof(double) * 15 * N);
+    double* buf = (double*)malloc(sizeof(double) * 16 * N);
+    double* Qsb = buf + 15 * N;
     double *Te0 = buf, *Tp0 = buf + N, *d0e = buf + 2 * N, *d0p = buf + 3 * N, *dEe = buf + 4 * N;
     double *dEp = buf + 5 * N, *ke = buf + 6 * N, *kp = buf + 7 * N, *y1e = buf + 8 * N, *y1p = buf + 9 * N;
     double *ybe = buf + 10 * N, *ybp = buf + 11 * N, *work = buf + 12 * N;
@@ -1753,7 +1796,7 @@ int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, 
     long iterations = 0;
     s->p.T = Te;
     or_llb_rhs(s, m, dy0, step, a, *dtio); /* normalises m = y0 */
-    or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0, d0e, d0p, work);
+    or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0, SPIN_QS(m), d0e, d0p, work);
     memcpy(s->y0, m, sizeof(double) * n3);
     memcpy(Te0, Te, sizeof(double) * N);
     memcpy(Tp0, Tp, sizeof(double) * N);
@@ -1768,7 +1811,7 @@ int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, 
         iterations++;
         s->p.T = Te;
         or_llb_rhs(s, m, dy, step, a, dt);
-        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, ke, kp, work);
+        or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, SPIN_QS(m), ke, kp, work);
         memcpy(dyE, dy, sizeof(double) * n3);
         memcpy(dEe, ke, sizeof(double) * N);
         memcpy(dEp, kp, sizeof(double) * N);
@@ -1796,7 +1839,7 @@ int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, 
             if (alpha >= 1.0 || iter > maxIter) { badIt = 1; break; }
             s->p.T = Te;
             or_llb_rhs(s, m, dy, step, a, dt);
-            or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, ke, kp, work);
+            or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, SPIN_QS(m), ke, kp, work);
         }
         if (badIt && tr == maxTry - 1) { rc = -1; break; }
         if (badIt) {
@@ -1837,7 +1880,7 @@ int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, 
             if (alpha >= 1.0 || iter > maxIter) { badIt = 1; break; }
             s->p.T = Te;
             or_llb_rhs(s, m, dy, step, a, dt);
-            or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, ke, kp, work);
+            or_tm_rhs(q, s->p.N, s->p.cell, s->p.pbc, Te, Tp, t0 + dt, SPIN_QS(m), ke, kp, work);
         }
         if (badIt && tr == maxTry - 1) { rc = -1; break; }
         if (badIt) {

@@ -219,9 +219,14 @@ int or_llb_step_bdf_euler(or_llb* s, double* m, double dt, uint32_t step, double
  * from (xc, yc).  Written for the coupled LLB-r run of configs with a laser. */
 typedef struct {
     double gammaE, Cl, G, kappa, Q0, sigma_r, sigma_t, t0, xc, yc, maxErrT;
+    double Td, nD;  /* Td > 0: Cp(Tp) = 9 nD kB Debye(Tp/Td) (mod_cp.go:75-85, Cp.cu) in place of Cl */
+    int spin_heat;  /* Qe += ws - qs, the spin system's energy flow and dissipation (laser-demag.py:70-88) */
 } or_tm_params;
+/* the spin term of Qe per cell, ws - qs, at the state of the last or_llb_rhs(s, m, ...) call */
+void or_llb_spin_heat(const or_llb* s, const double* m, double* Qs);
+/* Qs: the spin term of Qe per cell (or NULL) */
 void or_tm_rhs(const or_tm_params* q, const int N[3], const double cell[3], const int pbc[3], const double* Te,
-               const double* Tp, double t, double* dTe, double* dTp, double* work);
+               const double* Tp, double t, const double* Qs, double* dTe, double* dTp, double* work);
 /* One adaptive RK23 step of (m, Te, Tp) with the LLB temperature T = Te per
  * cell (the LLB's T mask and TMul are replaced), laser time *t advanced on
  * acceptance; the reference's multi-equation controller (solve_rk12.go:150-190)

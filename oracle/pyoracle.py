@@ -77,7 +77,8 @@ def _declare(L):
     L.or_llb_new.restype = v
     L.or_llb_free.argtypes = [v]
     L.or_llb_rhs.argtypes = [v, v, v, u32, u32, d]
-    L.or_tm_rhs.argtypes = [v, v, v, v, v, v, d, v, v, v]
+    L.or_tm_rhs.argtypes = [v, v, v, v, v, v, d, v, v, v, v]
+    L.or_llb_spin_heat.argtypes = [v, v, v]
     L.or_llbtm_step_rk23.argtypes = [v, v, v, v, v, v, v, d, d, d, u32, v]
     L.or_llbtm_step_rk23.restype = i
     L.or_llbtm_step_am12.argtypes = [v, v, v, v, v, v, v, v, v, d, d, u32, v]
@@ -201,18 +202,18 @@ class LLBParams(C.Structure):
 class TmParams(C.Structure):
     """or_tm_params of oracle/hs_oracle.h (two-temperature model + laser)."""
     _fields_ = [(n, C.c_double) for n in ("gammaE", "Cl", "G", "kappa", "Q0", "sigma_r", "sigma_t", "t0", "xc",
-                                          "yc", "maxErrT")]
+                                          "yc", "maxErrT", "Td", "nD")] + [("spin_heat", C.c_int)]
 
 
 def tm_params(**kw):
     q = TmParams()
     for k, v in kw.items():
-        setattr(q, k, float(v))
+        setattr(q, k, int(v) if k == "spin_heat" else float(v))
     return q
 
 
-def tm_rhs(q, N, cell, pbc, Te, Tp, t):
-    """dTe/dt, dTp/dt of the two-temperature model at time t."""
+def tm_rhs(q, N, cell, pbc, Te, Tp, t, Qs=None):
+    """dTe/dt, dTp/dt of the two-temperature model at time t (Qs: the spin term of Qe per cell)."""
     Te = f64(Te).reshape(-1)
     Tp = f64(Tp).reshape(-1)
     n = Te.size
@@ -220,7 +221,7 @@ def tm_rhs(q, N, cell, pbc, Te, Tp, t):
     Na = (C.c_int * 3)(*N)
     ca = (C.c_double * 3)(*cell)
     pa = (C.c_int * 3)(*pbc)
-    lib().or_tm_rhs(C.byref(q), Na, ca, pa, ptr(Te), ptr(Tp), float(t), ptr(dTe), ptr(dTp), ptr(work))
+    lib().or_tm_rhs(C.byref(q), Na, ca, pa, ptr(Te), ptr(Tp), float(t), ptr(f64(Qs)), ptr(dTe), ptr(dTp), ptr(work))
     return dTe, dTp
 
 
@@ -265,6 +266,14 @@ class LLB:
         k = np.empty_like(m)
         lib().or_llb_rhs(self._h, ptr(m), ptr(k), int(step), int(attempt), float(dt))
         return m, k
+
+    def spin_heat(self, m):
+        """The spin term of Qe (ws - qs) per cell at the state of the last rhs(m) call
+        (pass the m that rhs returned)."""
+        m = f64(m).reshape(3, self.ncell)
+        out = np.empty(self.ncell)
+        lib().or_llb_spin_heat(self._h, ptr(m), ptr(out))
+        return out
 
     def heff(self):
         p = lib().or_llb_heff(self._h)

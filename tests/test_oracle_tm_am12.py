@@ -5,7 +5,14 @@ Cl dTp/dt = G (Te - Tp) conserves E = gamma Te^2 / 2 + Cl Tp and relaxes to
 the temperature where Te = Tp on that energy (an independent LSODA solve of
 the same two-variable ODE pins the trajectory); with diffusion and a laser
 pulse, AM12 at tight tolerances must agree with the adaptive RK23 of the
-same system."""
+same system.
+
+Spin heating (Qs = ws - qs, the laser recipe's molecular-energy-density rate
+minus its dissipation rate) is pinned as the power of the local damping term
+k_loc = gamma alpha H: with the torque off the LLB right-hand side is k_loc
+alone, and Qs = 3 kB Tc n S/(S+1) m.k_loc + mu0 Ms0 H.k_loc.  The Debye
+lattice capacity enters dTp/dt as 9 n kB cp(Tp/Td) with cp the integral that
+the Cp pins check against quadrature."""
 import math
 
 import numpy as np
@@ -16,6 +23,7 @@ from oracle import pyoracle as po
 
 N, CELL = (2, 8, 16), (2e-9, 4e-9, 4e-9)
 GAM, CL, G = 6e3, 3.8e6, 1.8e18
+KB, MU0 = 1.38064881313131313e-23, 4e-7 * math.pi  # the constants of the reference's headers
 
 
 def _llb(longfield=1):
@@ -86,3 +94,42 @@ def test_am12_agrees_with_rk23_on_the_pumped_film():
     assert np.abs(Tea - Ter).max() < 0.05
     assert np.abs(Tpa - Tpr).max() < 0.05
     assert np.abs(ma - mr).max() < 2e-4
+
+
+def test_spin_heat_is_the_power_of_the_local_damping():
+    kw = W.llbr_params(T=300.0, seed=5, Bext=(0.0, 0.01, 0.02))
+    kw.update(torque=0)  # k = gamma alpha H only
+    o = po.LLB(N, CELL, **kw)
+    m, k = o.rhs(_m0(300.0), step=3, attempt=0, dt=1e-15)
+    H = o.heff()
+    Qs = o.spin_heat(m)
+    S = kw["JMul"]
+    ws = 3 * KB * kw["TcMul"] * kw["nMul"] * S / (S + 1) * np.sum(m * k, axis=0)
+    want = ws + MU0 * kw["msat0T0Mul"] * np.sum(H * k, axis=0)
+    assert np.allclose(Qs, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    # both parts matter here: the damping power and the longitudinal relaxation term
+    assert np.abs(ws).max() > 1e-3 * np.abs(want).max()
+    # without the local damping term there is no spin heating
+    kw.update(local00nc=0)
+    o2 = po.LLB(N, CELL, **kw)
+    m2, _ = o2.rhs(_m0(300.0), step=3, attempt=0, dt=1e-15)
+    assert np.all(o2.spin_heat(m2) == 0.0)
+
+
+def test_tm_rhs_debye_capacity_and_spin_heat_source():
+    Td, nD = 470.0, 9.14e28
+    kw = dict(gammaE=GAM, Cl=CL, G=G, kappa=90.0, Q0=1e21, sigma_r=20e-9, sigma_t=1e-13, t0=0.0, xc=32e-9,
+              yc=16e-9, maxErrT=1e-3)
+    rng = np.random.default_rng(4)
+    ncell = N[0] * N[1] * N[2]
+    Te, Tp = 300.0 + 500 * rng.random(ncell), 250.0 + 300 * rng.random(ncell)
+    Qs = 1e21 * rng.standard_normal(ncell)
+    dte0, dtp0 = po.tm_rhs(po.tm_params(**kw), N, CELL, (0, 0, 0), Te, Tp, 2e-14)
+    dte, dtp = po.tm_rhs(po.tm_params(Td=Td, nD=nD, **kw), N, CELL, (0, 0, 0), Te, Tp, 2e-14, Qs)
+    cp = po.cp(Tp, None, None, Td)
+    assert np.allclose(dtp, G * (Te - Tp) / (9 * nD * KB * cp), rtol=1e-13)
+    assert np.allclose(dte, dte0 + Qs / (GAM * Te), rtol=1e-12, atol=1e-12 * np.abs(dte).max())
+    # at Dulong-Petit (T >> Td) the Debye capacity approaches 3 n kB
+    hot = np.full(ncell, 1e5)
+    _, dtp_hot = po.tm_rhs(po.tm_params(Td=Td, nD=nD, **kw), N, CELL, (0, 0, 0), hot + 10.0, hot, 0.0)
+    assert np.allclose(dtp_hot, G * 10.0 / (3 * nD * KB), rtol=1e-3)
