@@ -100,6 +100,20 @@ void add_z2(Variants& v) {
         v.z.push_back({W, 6 * W, &launch_z2_32<L, W>});
 }
 
+// the same with the kernel product in two real halves (fft2.hpp k_zconv2s)
+template <int L, int W>
+void launch_z2s_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
+                   const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
+    hipLaunchKernelGGL((k_zconv2s<float, float, L, W>), g, dim3(6 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
+                       srcMul, tw, lv);
+}
+template <int L, int W>
+void add_z2s(Variants& v) {
+    if constexpr (reg_dft_ok(L) && L > 1 && L <= 64 && (long)L * 6 * W * 4 <= kLdsMax)
+        v.z.push_back({W, 6 * W, &launch_z2s_32<L, W>});
+}
+
 // two-level register FFT columns (fft2.hpp): W columns, W * B threads
 template <int L, int W>
 void add_col2(Variants& v) {
@@ -107,6 +121,23 @@ void add_col2(Variants& v) {
         constexpr int NT = W * split2(L).B;
         if constexpr ((long)(L * W + L) * 8 <= kLdsMax && NT >= 64 && NT <= 1024)
             v.cols.push_back({W, NT, &launch_cols2_32<L, -1, W>, &launch_cols2_32<L, 1, W>});
+    }
+}
+
+// column pass with a split real/imaginary LDS exchange (fft2.hpp k_cols2s)
+template <int L, int S, int W>
+void launch_cols2s_32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int npencil, int nin, int nkeep,
+                      int ncol, const cpx<float>* tw, hipStream_t st, Live lv) {
+    dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
+    hipLaunchKernelGGL((k_cols2s<float, L, S, W>), g, dim3(W * split2(L).B), 0, st, in, out, gi, go, nin, nkeep,
+                       ncol, tw, lv);
+}
+template <int L, int W>
+void add_col2s(Variants& v) {
+    if constexpr (two_level_ok(L) && L > 1) {
+        constexpr int NT = W * split2(L).B;
+        if constexpr ((long)L * W * 4 + (long)L * 8 <= kLdsMax && NT >= 64 && NT <= 1024)
+            v.cols.push_back({W, NT, &launch_cols2s_32<L, -1, W>, &launch_cols2s_32<L, 1, W>});
     }
 }
 
@@ -184,6 +215,9 @@ void register_size() {
     add_col2<L, 16>(v);
     add_col2h<L, 8>(v);
     add_col2h<L, 16>(v);
+    add_col2s<L, 8>(v);
+    add_col2s<L, 16>(v);
+    add_col2s<L, 32>(v);
     add_z<L, 8, 12>(v);
     add_z<L, 16, 12>(v);
     add_z<L, 32, 12>(v);
@@ -192,5 +226,7 @@ void register_size() {
     add_z<L, 8, 6>(v);
     add_z2<L, 16>(v);
     add_z2<L, 32>(v);
+    add_z2s<L, 16>(v);
+    add_z2s<L, 32>(v);
 }
 }  // namespace hsdm

@@ -203,6 +203,74 @@ __global__ void __launch_bounds__(NT) k_cols2(const cpx<T>* in, cpx<T>* out, Geo
     }
 }
 
+// ---- column pass with a split LDS exchange --------------------------------
+// The k_cols2 contract with the phase-1 -> phase-2 exchange done in two real
+// halves (real parts, then imaginary parts) through an L*W array of T: half
+// the LDS of k_cols2, so two blocks of W = 16 columns of length 1024 share a
+// CU and overlap their load, transform and store phases.
+template <typename T, int L, int S, int W, int NT = W * split2(L).B>
+__global__ void __launch_bounds__(NT) k_cols2s(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
+                                               int ncol, const cpx<T>* __restrict__ tw, Live lv) {
+    constexpr int A = split2(L).A, B = split2(L).B;
+    static_assert(A * B == L && NT == W * B && A <= B, "two-level split");
+    __shared__ T lds[L * W];
+    __shared__ cpx<T> tws[L];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int a = lb / gridDim.x;
+    if (all_dead(lv, a, a)) return;
+    const int col0 = (lb % gridDim.x) * W;
+    const long bi = geo_base(gi, a) + col0;
+    const long bo = geo_base(go, a) + col0;
+    for (int e = tid; e < L; e += NT) tws[e] = tw[e];
+    const int p = tid % W, n2 = tid / W;
+    const bool pin = col0 + p < ncol;
+    cpx<T> v[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) {
+        const int t = B * n1 + n2;
+        cpx<T> x = {T(0), T(0)};
+        if (pin && t < nin) x = in[bi + (long)t * gi.ts + p];
+        v[n1] = x;
+    }
+    __syncthreads();  // twiddles visible
+    dft_reg<T, A, S>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < A; k1++) {
+        if (n2 * k1) {
+            cpx<T> w = tws[n2 * k1];
+            if (S > 0) w.y = -w.y;
+            v[k1] = cmul(v[k1], w);
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) lds[(k1 * B + n2) * W + p] = v[k1].x;
+    __syncthreads();
+    const bool ph2 = tid < W * A;
+    const int q1 = tid / W;
+    cpx<T> u[B];
+    if (ph2) {
+#pragma unroll
+        for (int m = 0; m < B; m++) u[m].x = lds[(q1 * B + m) * W + p];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) lds[(k1 * B + n2) * W + p] = v[k1].y;
+    __syncthreads();
+    if (ph2) {
+#pragma unroll
+        for (int m = 0; m < B; m++) u[m].y = lds[(q1 * B + m) * W + p];
+        dft_reg<T, B, S>(u);
+        if (pin) {
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) {
+                const int t = q1 + A * k2;
+                if (t < nkeep) out[bo + (long)t * go.ts + p] = u[k2];
+            }
+        }
+    }
+}
+
 // ---- half-length column pass (F2 / I2 of a zero-padded axis) -------------
 // L = 2H with the data on one side of the padding: the forward transform of
 // an input with rows t < nin <= H, or the inverse transform whose output rows
@@ -549,5 +617,87 @@ __global__ void __launch_bounds__(6 * W) k_zconv2(cpx<T>* X, const TK* __restric
             for (int t = 0; t < L; t++)
                 if (t < N0) base[t * tstride] = v[t];
         }
+    }
+}
+
+// The axis-0 pass with the kernel product done in two real halves: K is real,
+// so the real and the imaginary parts of the spectrum go through the 3x3
+// product separately, each through an L x 6W array of T (half the LDS of
+// k_zconv2, more blocks per CU).
+template <typename T, typename TK, int L, int W>
+__device__ __forceinline__ void zconv_half(T* lds, const TK* __restrict__ kern, int jp, int col0, int H2, int K1,
+                                           int kp, T srcMul) {
+    constexpr int WP = 3 * W, W2 = 2 * WP, NT = 6 * W;
+    const long kplane = (long)(L / 2 + 1) * K1 * kp;
+    for (int e = threadIdx.x; e < L * W; e += NT) {
+        const int t = e / W, p = e % W;
+        const int col = col0 + p;
+        if (col >= H2) continue;
+        const int ii = (t > L / 2) ? L - t : t;
+        const T si = (t > L / 2) ? T(-1) : T(1);
+        const long K = ((long)ii * K1 + jp) * kp + col;
+        const T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
+        const T kyz0 = T(kern[3 * kplane + K]);
+        const T kxz = si * T(kern[4 * kplane + K]);
+        const T kxy0 = si * T(kern[5 * kplane + K]);
+#pragma unroll
+        for (int sm = 0; sm < 2; sm++) {  // mirror pencil: YZ and XY change sign
+            const T sj = sm ? T(-1) : T(1);
+            const T kyz = sj * kyz0, kxy = sj * kxy0;
+            T* a = lds + t * W2 + sm * WP + p;
+            const T mx = a[0] * srcMul, my = a[W] * srcMul, mz = a[2 * W] * srcMul;
+            a[0] = kxx * mx + kxy * my + kxz * mz;
+            a[W] = kxy * mx + kyy * my + kyz * mz;
+            a[2 * W] = kxz * mx + kyz * my + kzz * mz;
+        }
+    }
+}
+template <typename T, typename TK, int L, int W>
+__global__ void __launch_bounds__(6 * W) k_zconv2s(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+                                                   int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
+                                                   Live lv) {
+    constexpr int WP = 3 * W, W2 = 2 * WP;
+    static_assert(reg_dft_ok(L), "register DFT length");
+    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
+    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
+    __shared__ T lds[L * W2];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int jp = lb / gridDim.x;
+    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;
+    const int col0 = (lb % gridDim.x) * W;
+    const int p = tid % W, c = (tid / W) % 3, sm = tid / WP;
+    const int j = sm ? jm : jp;
+    const bool live = j >= 0 && col0 + p < H2;
+    const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
+    cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p;
+    cpx<T> v[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        cpx<T> x = {T(0), T(0)};
+        if (live && t < N0) x = base[t * tstride];
+        v[t] = x;
+    }
+    dft_reg<T, L, -1>(v);
+#pragma unroll
+    for (int t = 0; t < L; t++) lds[t * W2 + tid] = v[t].x;
+    __syncthreads();
+    zconv_half<T, TK, L, W>(lds, kern, jp, col0, H2, K1, kp, srcMul);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < L; t++) v[t].x = lds[t * W2 + tid];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < L; t++) lds[t * W2 + tid] = v[t].y;
+    __syncthreads();
+    zconv_half<T, TK, L, W>(lds, kern, jp, col0, H2, K1, kp, srcMul);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < L; t++) v[t].y = lds[t * W2 + tid];
+    dft_reg<T, L, +1>(v);
+    if (live) {
+#pragma unroll
+        for (int t = 0; t < L; t++)
+            if (t < N0) base[t * tstride] = v[t];
     }
 }
