@@ -357,6 +357,19 @@ def main():
                      "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "algorithmic_bytes": dom_bytes},
         "kernels": per,
     }
+    # energy terms of the final state (micromag/energy): one FP64 pass reading the FP64
+    # state (24 B/cell) and the FP32 demag field (12 B/cell) after its own demag evaluation
+    s.energy()
+    t0 = time.perf_counter()
+    ne = 10
+    for _ in range(ne):
+        E = s.energy()
+    e_ms = (time.perf_counter() - t0) / ne * 1e3
+    out["energy"] = {"E_J": dict(zip(["zeeman", "exchange", "demag", "longfield", "total"], map(float, E))),
+                     "ms_per_call": e_ms, "algorithmic_bytes": 36 * ncell,
+                     "note": "wall time per hs_solver_energy call incl. the 40-byte copy back; the demag "
+                             "field of the state is cached after the first call"}
+    log("bench: energy %.4f ms per call, E_total %.4e J" % (e_ms, E[4]))
     s.close()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_steps, 5)

@@ -114,6 +114,14 @@ class Solver:
         check("hs_solver_rhs")
         return k, h
 
+    def energy(self):
+        """[E_zeeman, E_ex, E_demag, E_lf, E_total] of the current state in J
+        (micromag/energy, modules/mod_energy.go:25-61)."""
+        E = np.empty(5, np.float64)
+        lib().hs_solver_energy(self._h, E.ctypes.data)
+        check("hs_solver_energy")
+        return E
+
     def run_graph(self, nsteps, dt, step0=None):
         """Fixed-step RK4 from a captured hipGraph (same arithmetic as run("rk4"))."""
         s0 = self.step_count if step0 is None else int(step0)
@@ -206,6 +214,13 @@ class Batch(Solver):
         lib().hs_solver_rhs(self._h, k.ctypes.data, h.ctypes.data, int(step), int(attempt), float(dt))
         check("hs_solver_rhs")
         return k, h
+
+    def energy(self):
+        """(nsolve, 5): [E_zeeman, E_ex, E_demag, E_lf, E_total] of each solve in J."""
+        E = np.empty((self.nsolve, 5), np.float64)
+        lib().hs_solver_energy(self._h, E.ctypes.data)
+        check("hs_solver_energy")
+        return E
 
     def run_graph(self, nsteps, dt, step0=None):
         """Fixed-step RK4 of every solve from a captured hipGraph."""

@@ -44,6 +44,7 @@ struct SolveP {
     uint64_t seed;
     uint32_t stream;
     float sthC;  // sqrt(thC): H_therm amplitude = sqrt(T / msk) sqrt(thC) / sqrt(max(cutoff_dt, dt))
+    double bextT[3];  // B_ext in T (FP64, for the Zeeman energy)
 };
 
 // Material constants of the longitudinal field when J, n, Tc and T are
@@ -1387,6 +1388,7 @@ struct hs_solver {
     int imp_max_iter = 3;
     double imp_absT = 1e-4, imp_relT = 1e-3;  // Te and Tp (two-temperature solves)
     float* qs = nullptr;  // spin heating: the spin term of Qe per cell, written by the LLB stages
+    double* epart = nullptr;  // energy: block partials, then the per-solve terms
 };
 
 namespace {
@@ -1545,6 +1547,127 @@ void ensure_b0(hs_solver* s) {
         demag_into(s, s->mA, s->B0);
         s->b0_valid = true;
     }
+}
+
+// ---- energy terms ------------------------------------------------------------
+// LoadEnergy (modules/mod_energy.go:25-61) and EnergyUpdater.Update (:98-116):
+// E_term = weight * Ms0T0Mul * sum_cells (m . F) msat0T0 with F = B_ext
+// (weight -V), H_ex (-V mu0 / 2), B (-V / 2) and H_lf (-V mu0).  One FP64
+// pass over the state: m from the FP64 state, H_ex (exchange6.cu:11-127,
+// neighbour order X, Z, Y) and H_lf (long_field.cu:13-72) in FP64, B from
+// the FP32 demag of the state.  Block partials, then a fixed-order sum per
+// solve (deterministic).
+constexpr int kEnergyTerms = 4;  // zeeman, exchange, demag, longitudinal
+struct EnergyC {
+    double ms0T0, ex[3], JMul, nMul, TcMul;
+};
+
+__device__ double block_sum256(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    return (threadIdx.x == 0) ? ((red[0] + red[1]) + (red[2] + red[3])) : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_energy(Phys P, EnergyC C, const double* __restrict__ y,
+                                                const float* __restrict__ B, const float* __restrict__ Tf,
+                                                double* __restrict__ part, long N) {
+    __shared__ double red[4];
+    const int sol = blockIdx.y;
+    const long I = (long)blockIdx.x * 256 + threadIdx.x;
+    const SolveP& sp = P.sp[sol];
+    double acc[kEnergyTerms] = {0.0, 0.0, 0.0, 0.0};
+    if (I < N) {
+        const double* ys = y + (long)sol * 3 * N;
+        // m of a cell, unit length in the LLG limit (the state is normalised lazily)
+        auto load_m = [&](long a, double v[3]) {
+            for (int d = 0; d < 3; d++) v[d] = ys[d * N + a];
+            if (P.normalize) {
+                const double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                const double sc = (l == 0.0) ? 0.0 : 1.0 / l;
+                for (int d = 0; d < 3; d++) v[d] *= sc;
+            }
+        };
+        double m[3];
+        load_m(I, m);
+        const double ms = P.msk ? (double)P.msk[I] : 1.0;
+        if (ms != 0.0) {
+            if (P.zeeman) acc[0] = (m[0] * sp.bextT[0] + m[1] * sp.bextT[1] + m[2] * sp.bextT[2]) * ms;
+            if (P.exchange) {
+                const long N12 = (long)P.N1 * P.N2;
+                const int ci[3] = {(int)(I / N12), (int)((I / P.N2) % P.N1), (int)(I % P.N2)};
+                const int nn[3] = {P.N0, P.N1, P.N2}, ww[3] = {P.w0, P.w1, P.w2};
+                const long st[3] = {N12, (long)P.N2, 1L};
+                const double l0 = P.lex ? (double)P.lex[I] : 1.0, l02 = l0 * l0;
+                double H[3] = {0.0, 0.0, 0.0};
+                const int order[3] = {0, 2, 1};
+                for (int q = 0; q < 3; q++) {
+                    const int ax = order[q], n = nn[ax], c = ci[ax];
+                    int lo = c - 1;
+                    lo = (lo < 0 && ww[ax]) ? n + lo : lo;
+                    lo = max(lo, 0);
+                    int hi = c + 1;
+                    hi = (hi == n && ww[ax]) ? hi - n : hi;
+                    hi = min(hi, n - 1);
+                    const long a1 = I + (long)(lo - c) * st[ax], a2 = I + (long)(hi - c) * st[ax];
+                    const double l1 = P.lex ? (double)P.lex[a1] : 1.0, l2 = P.lex ? (double)P.lex[a2] : 1.0;
+                    const double p1 = avg_geom_zero(l02, l1 * l1), p2 = avg_geom_zero(l02, l2 * l2);
+                    double v1[3], v2[3];
+                    load_m(a1, v1);
+                    load_m(a2, v2);
+                    for (int d = 0; d < 3; d++) H[d] += C.ex[ax] * ((p1 * v1[d] + p2 * v2[d]) - (p1 * m[d] + p2 * m[d]));
+                }
+                acc[1] = C.ms0T0 * (m[0] * H[0] + m[1] * H[1] + m[2] * H[2]) * ms;
+            }
+            if (P.demag) {
+                const float* Bs = B + (long)sol * 3 * N;
+                acc[2] = (m[0] * (double)Bs[I] + m[1] * (double)Bs[N + I] + m[2] * (double)Bs[2 * N + I]) * ms;
+            }
+            if (P.longfield) {
+                const double Ms0T0 = C.ms0T0 * ms;
+                const double S = C.JMul * (P.J ? (double)P.J[I] : 1.0);
+                const double n = C.nMul * (P.n ? (double)P.n[I] : 1.0);
+                const double Tc = C.TcMul * (P.Tc ? (double)P.Tc[I] : 1.0);
+                const double Ts = (double)sp.TMul * (Tf ? (double)Tf[(long)sol * N + I] : (P.T ? (double)P.T[I] : 1.0));
+                const double am = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+                if (!(Ms0T0 == 0.0 || n == 0.0 || am <= HS_ZERO)) {
+                    const double preS = (S > HS_INFSPIN) ? 1.0 : S / (S + 1.0);
+                    const double b = 3.0 * preS * Tc / Ts;
+                    double W, dW;
+                    brillouin_w_dw<double>(S, am * b, W, dW);
+                    const double mult = n * (HS_KB / HS_MU0) * Ts * (W - am) / (Ms0T0 * dW) / am;
+                    acc[3] = mult * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) * ms;
+                }
+            }
+        }
+    }
+    for (int t = 0; t < kEnergyTerms; t++) {
+        const double v = block_sum256(acc[t], red);
+        if (threadIdx.x == 0) part[((long)sol * gridDim.x + blockIdx.x) * kEnergyTerms + t] = v;
+    }
+}
+
+// Per solve: the block partials summed in a fixed order, times Ms0T0Mul and
+// the term weights; out[sol * 5 + t], t = 4 the sum of the four.
+__global__ void __launch_bounds__(256) k_energy_sum(const double* __restrict__ part, int nb, double ms0T0, double V,
+                                                    double* __restrict__ out) {
+    __shared__ double red[4];
+    const int sol = blockIdx.x;
+    const double w[kEnergyTerms] = {-V, -0.5 * V * HS_MU0, -0.5 * V, -V * HS_MU0};
+    double tot = 0.0;
+    for (int t = 0; t < kEnergyTerms; t++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) v += part[((long)sol * nb + b) * kEnergyTerms + t];
+        v = block_sum256(v, red);
+        if (threadIdx.x == 0) {
+            const double e = v * ms0T0 * w[t];
+            out[(long)sol * 5 + t] = e;
+            tot += e;
+        }
+    }
+    if (threadIdx.x == 0) out[(long)sol * 5 + 4] = tot;
 }
 
 // One fixed-size RK4 step of every solve (4 demag evaluations, 4 stage kernels).
@@ -1897,7 +2020,10 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     for (int q = 0; q < nsolve; q++) {
         double T = TMul ? TMul[q] : p->TMul;
         solve_constants(p, T, hsp[q]);
-        for (int c = 0; c < 3; c++) hsp[q].bext[c] = (float)(1.0 / HS_MU0 * (Bext ? Bext[3 * q + c] : p->Bext[c]));
+        for (int c = 0; c < 3; c++) {
+            hsp[q].bextT[c] = Bext ? Bext[3 * q + c] : p->Bext[c];
+            hsp[q].bext[c] = (float)(1.0 / HS_MU0 * hsp[q].bextT[c]);
+        }
         hsp[q].seed = seed ? seed[q] : p->seed;
         hsp[q].stream = p->stream + (uint32_t)(seed ? 0 : q);
     }
@@ -1982,7 +2108,7 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
     if (s->slot_host) hipHostFree(s->slot_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
-                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1, s->qs};
+                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1, s->qs, s->epart};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
@@ -2022,6 +2148,37 @@ extern "C" int hs_solver_get_m(hs_solver* s, double* m) {
     hipMemcpyAsync(m, s->y0, sizeof(double) * 3 * s->N * s->nsolve, hipMemcpyDeviceToHost, s->st);
     hipStreamSynchronize(s->st);
     HS_LAUNCH_CHECK("hs_solver_get_m");
+    return hs_last_error();
+}
+
+extern "C" int hs_solver_energy(hs_solver* s, double* E) {
+    if (!s || !E) {
+        hs::set_error(HS_EINVAL, "hs_solver_energy: null argument");
+        return HS_EINVAL;
+    }
+    const int nb = (int)hs::div_up(s->N, 256);
+    if (!s->epart &&
+        hipMalloc(&s->epart, sizeof(double) * ((size_t)nb * s->nsolve * kEnergyTerms + 5 * s->nsolve)) != hipSuccess) {
+        s->epart = nullptr;
+        hs::set_error(HS_ENOMEM, "hs_solver_energy: out of device memory");
+        return HS_ENOMEM;
+    }
+    if (s->phys.demag) ensure_b0(s);
+    const hs_llb_params& p = s->p;
+    EnergyC c;
+    c.ms0T0 = p.msat0T0Mul;
+    for (int d = 0; d < 3; d++) c.ex[d] = p.lexMul * p.lexMul / (p.cell[d] * p.cell[d]);
+    c.JMul = p.JMul;
+    c.nMul = p.nMul;
+    c.TcMul = p.TcMul;
+    double* out = s->epart + (size_t)nb * s->nsolve * kEnergyTerms;
+    hipLaunchKernelGGL(k_energy, dim3(nb, s->nsolve), 256, 0, s->st, s->phys, c, s->y0, s->B0,
+                       s->tm ? s->Tf[0] : nullptr, s->epart, s->N);
+    hipLaunchKernelGGL(k_energy_sum, dim3(s->nsolve), 256, 0, s->st, s->epart, nb, p.msat0T0Mul,
+                       p.cell[0] * p.cell[1] * p.cell[2], out);
+    hipMemcpyAsync(E, out, sizeof(double) * 5 * s->nsolve, hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    HS_LAUNCH_CHECK("hs_solver_energy");
     return hs_last_error();
 }
 
@@ -2745,7 +2902,10 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
     const hs_llb_params* p = &s->p;
     SolveP q = s->sp_host[slot];
     solve_constants(p, TMul, q);
-    for (int k = 0; k < 3; k++) q.bext[k] = (float)(1.0 / HS_MU0 * (Bext ? Bext[k] : p->Bext[k]));
+    for (int k = 0; k < 3; k++) {
+        q.bextT[k] = Bext ? Bext[k] : p->Bext[k];
+        q.bext[k] = (float)(1.0 / HS_MU0 * q.bextT[k]);
+    }
     q.seed = seed;
     q.stream = stream;
     s->sp_host[slot] = q;

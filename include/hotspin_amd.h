@@ -181,6 +181,14 @@ HS_API int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, doub
 HS_API int hs_solver_set_implicit(hs_solver* s, double maxAbsErr, double maxRelErr, double maxIterErr,
                                   int maxIter);
 HS_API int hs_solver_set_implicit_T(hs_solver* s, double maxAbsErr, double maxRelErr);
+/* Energy terms of the current state, per solve: E[5 q + t] with t = 0 Zeeman
+ * (B_ext, weight -V), 1 exchange (H_ex, -V mu0 / 2), 2 demag (B, -V / 2),
+ * 3 longitudinal field (H_lf, -V mu0), 4 their sum, each
+ * weight * Ms0T0Mul * sum_cells (m . F) msat0T0 (LoadEnergy and
+ * EnergyUpdater.Update, modules/mod_energy.go:25-61,98-116; a term whose
+ * field is not loaded is 0).  FP64 m, H_ex and H_lf; B from the FP32 demag
+ * of the state; deterministic FP64 sums.  E: 5 * nsolve doubles (host). */
+HS_API int hs_solver_energy(hs_solver* s, double* E);
 /* One RHS evaluation at the current state (no step): k and H_eff to host. */
 HS_API int hs_solver_rhs(hs_solver* s, double* k_host, double* heff_host, uint32_t step, uint32_t attempt,
                          double dt);

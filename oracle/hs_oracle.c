@@ -1925,6 +1925,33 @@ int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, 
     return rc;
 }
 
+/* LoadEnergy, modules/mod_energy.go:25-61: the terms of the fields this
+ * solver loads, each weight * Ms0T0Mul * sum (m . F) msat0T0
+ * (EnergyUpdater.Update, :98-116): E[0] Zeeman (uniform B_ext, weight -V),
+ * E[1] exchange (H_ex, -V mu0 / 2), E[2] demag (B, -V / 2), E[3]
+ * longitudinal field (H_lf, -V mu0), E[4] their sum.  The fields are those
+ * of the last or_llb_rhs call; m is the (normalised) m that call used. */
+void or_llb_energy(const or_llb* s, const double* m, double E[5]) {
+    const or_llb_params* p = &s->p;
+    const long N = s->N;
+    const double V = p->cell[0] * p->cell[1] * p->cell[2];
+    const double one[3] = {1.0, 1.0, 1.0};
+    for (int t = 0; t < 5; t++) E[t] = 0.0;
+    if (p->zeeman) {
+        double sum = 0.0;
+        for (long i = 0; i < N; i++) {
+            double w = m[i] * p->Bext[0] + m[N + i] * p->Bext[1] + m[2 * N + i] * p->Bext[2];
+            if (p->msat0T0) w = w * p->msat0T0[i];
+            sum += w;
+        }
+        E[0] = sum * p->msat0T0Mul * (-V);
+    }
+    if (p->exchange) E[1] = or_energy_term(m, s->Hex, one, one, p->msat0T0, p->msat0T0Mul, -0.5 * V * OR_MU0, N);
+    if (p->demag) E[2] = or_energy_term(m, s->B, one, one, p->msat0T0, p->msat0T0Mul, -0.5 * V, N);
+    if (p->longfield) E[3] = or_energy_term(m, s->Hlf, one, one, p->msat0T0, p->msat0T0Mul, -V * OR_MU0, N);
+    E[4] = E[0] + E[1] + E[2] + E[3];
+}
+
 /* EnergyUpdater.Update, modules/mod_energy.go:98-116 */
 double or_energy_term(const double* m, const double* H, const double mMul[3], const double HMul[3],
                       const double* msat, double msatMul, double weight, long N) {

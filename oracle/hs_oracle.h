@@ -243,6 +243,7 @@ int or_llbtm_step_am12(or_llb* s, const or_tm_params* q, double* m, double* Te, 
 /* energy term: weight*Ms0T0Mul*sum((mMul.m).(HMul.H)*msat) -- mod_energy.go:98-116 */
 double or_energy_term(const double* m, const double* H, const double mMul[3], const double HMul[3],
                       const double* msat, double msatMul, double weight, long N);
+void or_llb_energy(const or_llb* s, const double* m, double E[5]);
 
 void or_set_threads(int n);
 int or_get_threads(void);

@@ -79,6 +79,7 @@ def _declare(L):
     L.or_llb_rhs.argtypes = [v, v, v, u32, u32, d]
     L.or_tm_rhs.argtypes = [v, v, v, v, v, v, d, v, v, v, v]
     L.or_llb_spin_heat.argtypes = [v, v, v]
+    L.or_llb_energy.argtypes = [v, v, v]
     L.or_llbtm_step_rk23.argtypes = [v, v, v, v, v, v, v, d, d, d, u32, v]
     L.or_llbtm_step_rk23.restype = i
     L.or_llbtm_step_am12.argtypes = [v, v, v, v, v, v, v, v, v, d, d, u32, v]
@@ -266,6 +267,14 @@ class LLB:
         k = np.empty_like(m)
         lib().or_llb_rhs(self._h, ptr(m), ptr(k), int(step), int(attempt), float(dt))
         return m, k
+
+    def energy(self, m):
+        """[E_zeeman, E_ex, E_demag, E_lf, E_total] at the state of the last rhs(m)
+        call (pass the m that rhs returned)."""
+        m = f64(m).reshape(3, self.ncell)
+        out = np.empty(5)
+        lib().or_llb_energy(self._h, ptr(m), ptr(out))
+        return out
 
     def spin_heat(self, m):
         """The spin term of Qe (ws - qs) per cell at the state of the last rhs(m) call
