@@ -141,6 +141,24 @@ void add_col2s(Variants& v) {
     }
 }
 
+// the same with a bank-conflict-free padded exchange (fft2.hpp k_cols2p)
+template <int L, int S, int W>
+void launch_cols2p_32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int npencil, int nin, int nkeep,
+                      int ncol, const cpx<float>* tw, hipStream_t st, Live lv) {
+    dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
+    hipLaunchKernelGGL((k_cols2p<float, L, S, W>), g, dim3(W * split2(L).B), 0, st, in, out, gi, go, nin, nkeep,
+                       ncol, tw, lv);
+}
+template <int L, int W>
+void add_col2p(Variants& v) {
+    if constexpr (two_level_ok(L) && L > 1) {
+        constexpr int NT = W * split2(L).B;
+        if constexpr ((long)split2(L).A * (split2(L).B * W + W) * 4 + (long)L * 8 <= kLdsMax && NT >= 64 &&
+                      NT <= 1024)
+            v.cols.push_back({W, NT, &launch_cols2p_32<L, -1, W>, &launch_cols2p_32<L, 1, W>});
+    }
+}
+
 // half-length column pass (fft2.hpp k_cols2h) for a zero-padded axis: the
 // forward pass needs nin <= L/2 and the inverse nkeep <= L/2; otherwise (a
 // periodic axis, no padding) the launcher runs the length-L k_cols2
@@ -218,6 +236,8 @@ void register_size() {
     add_col2s<L, 8>(v);
     add_col2s<L, 16>(v);
     add_col2s<L, 32>(v);
+    add_col2p<L, 8>(v);
+    add_col2p<L, 16>(v);
     add_z<L, 8, 12>(v);
     add_z<L, 16, 12>(v);
     add_z<L, 32, 12>(v);

@@ -271,6 +271,73 @@ __global__ void __launch_bounds__(NT) k_cols2s(const cpx<T>* in, cpx<T>* out, Ge
     }
 }
 
+// The split-exchange column pass with the phase-2 reads free of LDS bank
+// conflicts: each k1 row of the exchange array is padded by W floats, so the
+// 32 lanes of a half-wave (32 / W values of k1, W columns each) fall on 32
+// distinct banks (three W = 8 blocks per CU instead of four).
+template <typename T, int L, int S, int W, int NT = W * split2(L).B>
+__global__ void __launch_bounds__(NT) k_cols2p(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
+                                               int ncol, const cpx<T>* __restrict__ tw, Live lv) {
+    constexpr int A = split2(L).A, B = split2(L).B, RS = B * W + W;  // padded k1 row stride
+    static_assert(A * B == L && NT == W * B && A <= B, "two-level split");
+    __shared__ T lds[A * RS];
+    __shared__ cpx<T> tws[L];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int a = lb / gridDim.x;
+    if (all_dead(lv, a, a)) return;
+    const int col0 = (lb % gridDim.x) * W;
+    const long bi = geo_base(gi, a) + col0;
+    const long bo = geo_base(go, a) + col0;
+    for (int e = tid; e < L; e += NT) tws[e] = tw[e];
+    const int p = tid % W, n2 = tid / W;
+    const bool pin = col0 + p < ncol;
+    cpx<T> v[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) {
+        const int t = B * n1 + n2;
+        cpx<T> x = {T(0), T(0)};
+        if (pin && t < nin) x = in[bi + (long)t * gi.ts + p];
+        v[n1] = x;
+    }
+    __syncthreads();  // twiddles visible
+    dft_reg<T, A, S>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < A; k1++) {
+        if (n2 * k1) {
+            cpx<T> w = tws[n2 * k1];
+            if (S > 0) w.y = -w.y;
+            v[k1] = cmul(v[k1], w);
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) lds[k1 * RS + n2 * W + p] = v[k1].x;
+    __syncthreads();
+    const bool ph2 = tid < W * A;
+    const int q1 = tid / W;
+    cpx<T> u[B];
+    if (ph2) {
+#pragma unroll
+        for (int m = 0; m < B; m++) u[m].x = lds[q1 * RS + m * W + p];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) lds[k1 * RS + n2 * W + p] = v[k1].y;
+    __syncthreads();
+    if (ph2) {
+#pragma unroll
+        for (int m = 0; m < B; m++) u[m].y = lds[q1 * RS + m * W + p];
+        dft_reg<T, B, S>(u);
+        if (pin) {
+#pragma unroll
+            for (int k2 = 0; k2 < B; k2++) {
+                const int t = q1 + A * k2;
+                if (t < nkeep) out[bo + (long)t * go.ts + p] = u[k2];
+            }
+        }
+    }
+}
+
 // ---- half-length column pass (F2 / I2 of a zero-padded axis) -------------
 // L = 2H with the data on one side of the padding: the forward transform of
 // an input with rows t < nin <= H, or the inverse transform whose output rows
