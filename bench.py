@@ -357,6 +357,18 @@ def main():
                      "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "algorithmic_bytes": dom_bytes},
         "kernels": per,
     }
+    # the same steps with the state handed over through host buffers (hs_solver_set_m /
+    # hs_solver_get_m, 100 MB each way over PCIe): reported beside `value`, never as it
+    mh = m.copy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.set_m(mh)
+    _, stp = s.run("rk23", args.steps, dt, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12)
+    mh = s.get_m()
+    el_p = time.perf_counter() - t0
+    out["pcie_inclusive"] = {"value": ncell * stp.steps / el_p, "unit": "cell-steps/s", "steps": stp.steps,
+                             "note": "set_m + %d RK23 steps + get_m, host FP64 state copied in and out" % stp.steps}
+    log("bench: with host hand-over %.3e cell-steps/s" % (ncell * stp.steps / el_p))
     # energy terms of the final state (micromag/energy): one FP64 pass reading the FP64
     # state (24 B/cell) and the FP32 demag field (12 B/cell) after its own demag evaluation
     s.energy()
