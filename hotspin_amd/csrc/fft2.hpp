@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(NT) k_r2c_rows2(const T* __restrict__ x, const
     constexpr int A = split2(M).A, B = split2(M).B, BP = B + 1;
     static_assert(A * B == M && NT == W * B && A <= B, "two-level split");
     __shared__ cpx<T> lds[rows2_lds<M, W>()];
-    __shared__ cpx<T> tws[M];
+    __shared__ cpx<T> tws[M + 1];  // twM for phase 1, then twP[0..M] for the split
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
@@ -486,7 +486,8 @@ __global__ void __launch_bounds__(NT) k_r2c_rows2(const T* __restrict__ x, const
 #pragma unroll
         for (int k1 = 0; k1 < A; k1++) lds[(p * A + k1) * BP + nb] = (nb * k1) ? cmul(v[k1], tws[nb * k1]) : v[k1];
     }
-    __syncthreads();
+    __syncthreads();  // phase 1 done with twM: the table takes twP for the split
+    for (int e = tid; e <= M; e += NT) tws[e] = twP[e];
     cpx<T> u[B];
     const int k1 = tid % A, p2 = tid / A;
     if (tid < W * A) {  // phase 2: thread (k1, row) gathers Y[k1][n2]
@@ -509,7 +510,7 @@ __global__ void __launch_bounds__(NT) k_r2c_rows2(const T* __restrict__ x, const
         cpx<T> zm = lds[p * M + ((M - k) % M)];
         cpx<T> E = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
         cpx<T> O = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
-        out[r * H2p + k] = cadd(E, cmul(twP[k], O));
+        out[r * H2p + k] = cadd(E, cmul(tws[k], O));
     }
 }
 
