@@ -195,6 +195,41 @@ def laser_leg(chunk=50):
             "note": "peak temperatures sampled every %d steps (the host reads Te, Tp between runs)" % chunk}
 
 
+
+def laser_recipe_leg(nsteps=200):
+    """The same film and pump with the couplings and solver of the reference's
+    laser recipe (examples/laser-demag.py:30-101): Debye lattice capacity
+    (Td = 470 K, nD = the spin density), spin heating Qs = ws - qs into Qe, no
+    thermal field (the recipe loads none: with one, qs grows as 1/dt), and
+    AM12 of (Te, Tp, m) with absolute and relative bounds 1e-5 for all three.
+    The demag field stays on.  A fixed number of steps from t = 0."""
+    import torch
+    from hotspin_amd.api import TMSolver
+    w, kw, tm, m0 = W.laser()
+    tm.update(Td=470.0, nD=kw["nMul"], spin_heat=1)
+    kw["thermal"] = 0
+    ncell = int(np.prod(w["N"]))
+    s = TMSolver(w["N"], w["cell"], tm=tm, T0=w["T0"], **kw)
+    s.set_m(m0)
+    s.set_implicit(maxAbsErr=1e-5, maxRelErr=1e-5)
+    s.set_implicit_T(maxAbsErr=1e-5, maxRelErr=1e-5)
+    dt, _ = s.run("am12", 5, 1e-16, mindt=1e-20, maxdt=5e-14)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dt, st = s.run("am12", nsteps, dt, mindt=1e-20, maxdt=5e-14)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    te, tp = s.get_T()
+    t_end = s.time
+    s.close()
+    log("laser recipe: %d AM12 steps (%d fixed-point iterations) in %.2f s, t = %.3g s, peak Te %.1f K" % (
+        st.steps, st.iterations, el, t_end, float(te.max())))
+    return {"workload": "laser recipe: LLB-r + 2TM 256x256x8 cells 4x4x2nm, %d AM12 steps from t = 0 (bounds "
+                        "1e-5), Debye Cp, spin heating, no thermal field" % nsteps,
+            "cells": ncell, "steps": st.steps, "iterations": st.iterations, "rejected": st.rejected,
+            "cell_steps_per_s": ncell * st.steps / el, "wall_s": el, "t_end_s": t_end,
+            "peak_Te_K": float(te.max()), "peak_Tp_K": float(tp.max())}
+
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "llbr_512x512x16_rk23_hbm_traffic.md")
 
 
@@ -389,6 +424,7 @@ def main():
         out["serving"] = serving_leg(args.serve_requests)
     if not args.no_laser:
         out["laser"] = laser_leg()
+        out["laser_recipe"] = laser_recipe_leg()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log("bench: cpu baseline (%d threads)..." % threads)
