@@ -498,7 +498,7 @@ void tune32(hs_demag* d, hipStream_t st) {
         if (t < best) { best = t; d->z32 = v; pick[2] = (int)i; }
     }
     // One sweep of refinement inside the whole five-pass convolution: each
-    // pass's variants again, the other passes fixed, so a pass is judged with
+    // pass's variants again, the other passes fixed, so a pass is timed with
     // the caches and neighbours it has in a step (isolated timings can rank
     // close variants the other way).
     {
