@@ -1009,6 +1009,21 @@ __global__ void __launch_bounds__(256) k_imp(const double* __restrict__ y0, doub
 // red[3] = 1 on a non-finite value; fout (optional) takes the FP32 copy of
 // y_new (the LLB stage's temperature).
 constexpr int kImp1Vals = 4;
+constexpr int kImpAll = kImpVals + 2 * kImp1Vals;  // m, Te, Tp maxima of one fixed-point update
+
+// Maxima q < nq, each over its kImpLanes lanes (value q, lane l at
+// src[(q kImpLanes + l) kImpStride]), into dst[q] for one small readback;
+// the same max as the host's std::max over the lanes.
+__global__ void k_imp_fold(const unsigned long long* __restrict__ src, int nq, double* __restrict__ dst) {
+    const int q = threadIdx.x;
+    if (q >= nq) return;
+    double v = 0.0;
+    for (int l = 0; l < kImpLanes; l++) {
+        const double x = __longlong_as_double((long long)src[((long)q * kImpLanes + l) * kImpStride]);
+        v = (v < x) ? x : v;
+    }
+    dst[q] = v;
+}
 __global__ void __launch_bounds__(256) k_imp1(const double* __restrict__ y0, double* __restrict__ acc,
                                               const double* __restrict__ k, const double* __restrict__ dy0,
                                               const double* __restrict__ y1, float* __restrict__ fout, double c,
@@ -1389,6 +1404,7 @@ struct hs_solver {
     double imp_absT = 1e-4, imp_relT = 1e-3;  // Te and Tp (two-temperature solves)
     float* qs = nullptr;  // spin heating: the spin term of Qe per cell, written by the LLB stages
     double* epart = nullptr;  // energy: block partials, then the per-solve terms
+    double* imp_host = nullptr;  // pinned: the folded maxima of a fixed-point update
 };
 
 namespace {
@@ -2106,6 +2122,7 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
         if (s->slot_graph[v]) hipGraphExecDestroy(s->slot_graph[v]);
     if (s->slot_ev) hipEventDestroy(s->slot_ev);
     if (s->slot_host) hipHostFree(s->slot_host);
+    if (s->imp_host) hipHostFree(s->imp_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
                   s->errmax, s->sp, s->clk, s->live, s->imp, s->y1, s->qs, s->epart};
@@ -2250,12 +2267,15 @@ struct ImpRed {
 
 bool ensure_imp(hs_solver* s) {
     // region 0: the m maxima (kImpVals), regions 1, 2: Te, Tp (kImp1Vals each)
-    if (!s->imp && hipMalloc(&s->imp, sizeof(unsigned long long) * (kImpVals + 2 * kImp1Vals) * kImpLanes *
-                                          kImpStride) != hipSuccess)
+    // followed by kImpAll folded doubles
+    if (!s->imp && hipMalloc(&s->imp, sizeof(unsigned long long) * (kImpAll * kImpLanes * kImpStride + kImpAll)) !=
+                       hipSuccess)
         s->imp = nullptr;
+    if (!s->imp_host && hipHostMalloc((void**)&s->imp_host, sizeof(double) * kImpAll) != hipSuccess)
+        s->imp_host = nullptr;
     if (!s->y1 && hipMalloc(&s->y1, sizeof(double) * 3 * s->N * s->nsolve) != hipSuccess) s->y1 = nullptr;
-    if (!s->imp || !s->y1) hs::set_error(HS_ENOMEM, "implicit driver: out of device memory");
-    return s->imp && s->y1;
+    if (!s->imp || !s->y1 || !s->imp_host) hs::set_error(HS_ENOMEM, "implicit driver: out of memory");
+    return s->imp && s->y1 && s->imp_host;
 }
 
 // LLB RHS of the iterate (FP32 stage input mB, demag first) into k_out; T:
@@ -2288,18 +2308,11 @@ ImpRed imp_update(hs_solver* s, const float* k_in, float* k_out, const float* dy
     hipMemsetAsync(s->imp, 0, sizeof(unsigned long long) * nred, s->st);
     hipLaunchKernelGGL(k_imp, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->y0, s->acc, k_in, dy0,
                        y1, s->mB, c, norm_in ? s->p.normalize : 0, s->p.normalize, s->imp, s->N);
-    std::vector<unsigned long long> b(nred);
-    hipMemcpyAsync(b.data(), s->imp, sizeof(unsigned long long) * nred, hipMemcpyDeviceToHost, s->st);
+    double* fold = (double*)(s->imp + (long)kImpAll * kImpLanes * kImpStride);
+    hipLaunchKernelGGL(k_imp_fold, 1, 64, 0, s->st, s->imp, kImpVals, fold);
+    hipMemcpyAsync(s->imp_host, fold, sizeof(double) * kImpVals, hipMemcpyDeviceToHost, s->st);
     hipStreamSynchronize(s->st);
-    double v[kImpVals];
-    for (int q = 0; q < kImpVals; q++) {
-        v[q] = 0.0;
-        for (int l = 0; l < kImpLanes; l++) {
-            double x;
-            memcpy(&x, &b[((long)q * kImpLanes + l) * kImpStride], sizeof(x));
-            v[q] = std::max(v[q], x);
-        }
-    }
+    const double* v = s->imp_host;
     ImpRed r;
     for (int p = 0; p < 3; p++) {
         r.d[p] = v[p];
@@ -2533,18 +2546,11 @@ TmRed tm_updates(hs_solver* s, const float* km, const double* ke, const double* 
                        with_y1 ? s->Tea : nullptr, s->Tf[1], c, rT, s->N);
     hipLaunchKernelGGL(k_imp1, g, hs::kBlock, 0, s->st, s->Tp[0], s->Tp[1], kp, trap ? s->k1p : nullptr,
                        with_y1 ? s->Tpa : nullptr, (float*)nullptr, c, rP, s->N);
-    std::vector<unsigned long long> b(nred);
-    hipMemcpyAsync(b.data(), s->imp, sizeof(unsigned long long) * nred, hipMemcpyDeviceToHost, s->st);
+    double* fold = (double*)(s->imp + (long)kImpAll * region);
+    hipLaunchKernelGGL(k_imp_fold, 1, 64, 0, s->st, s->imp, kImpAll, fold);
+    hipMemcpyAsync(s->imp_host, fold, sizeof(double) * kImpAll, hipMemcpyDeviceToHost, s->st);
     hipStreamSynchronize(s->st);
-    auto mx = [&](long q) {
-        double v = 0.0;
-        for (int l = 0; l < kImpLanes; l++) {
-            double x;
-            memcpy(&x, &b[q * region + (long)l * kImpStride], sizeof(x));
-            v = std::max(v, x);
-        }
-        return v;
-    };
+    auto mx = [&](long q) { return s->imp_host[q]; };
     TmRed r{};
     for (int p = 0; p < 3; p++) {
         r.m.d[p] = mx(p);
