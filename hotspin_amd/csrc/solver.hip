@@ -1013,13 +1013,16 @@ constexpr int kImpAll = kImpVals + 2 * kImp1Vals;  // m, Te, Tp maxima of one fi
 
 // Maxima q < nq, each over its kImpLanes lanes (value q, lane l at
 // src[(q kImpLanes + l) kImpStride]), into dst[q] for one small readback;
-// the same max as the host's std::max over the lanes.
-__global__ void k_imp_fold(const unsigned long long* __restrict__ src, int nq, double* __restrict__ dst) {
+// the same max as the host's std::max over the lanes.  Each lane is left at
+// zero for the next update's atomics.
+__global__ void k_imp_fold(unsigned long long* __restrict__ src, int nq, double* __restrict__ dst) {
     const int q = threadIdx.x;
     if (q >= nq) return;
     double v = 0.0;
     for (int l = 0; l < kImpLanes; l++) {
-        const double x = __longlong_as_double((long long)src[((long)q * kImpLanes + l) * kImpStride]);
+        unsigned long long* a = src + ((long)q * kImpLanes + l) * kImpStride;
+        const double x = __longlong_as_double((long long)*a);
+        *a = 0ull;
         v = (v < x) ? x : v;
     }
     dst[q] = v;
@@ -1405,6 +1408,7 @@ struct hs_solver {
     float* qs = nullptr;  // spin heating: the spin term of Qe per cell, written by the LLB stages
     double* epart = nullptr;  // energy: block partials, then the per-solve terms
     double* imp_host = nullptr;  // pinned: the folded maxima of a fixed-point update
+    bool imp_zeroed = false;     // imp lanes cleared once; k_imp_fold clears them after each read
 };
 
 namespace {
@@ -2273,6 +2277,10 @@ bool ensure_imp(hs_solver* s) {
         s->imp = nullptr;
     if (!s->imp_host && hipHostMalloc((void**)&s->imp_host, sizeof(double) * kImpAll) != hipSuccess)
         s->imp_host = nullptr;
+    if (s->imp && !s->imp_zeroed) {  // the lanes start at zero; each fold leaves them so
+        hipMemsetAsync(s->imp, 0, sizeof(unsigned long long) * kImpAll * kImpLanes * kImpStride, s->st);
+        s->imp_zeroed = true;
+    }
     if (!s->y1 && hipMalloc(&s->y1, sizeof(double) * 3 * s->N * s->nsolve) != hipSuccess) s->y1 = nullptr;
     if (!s->imp || !s->y1 || !s->imp_host) hs::set_error(HS_ENOMEM, "implicit driver: out of memory");
     return s->imp && s->y1 && s->imp_host;
@@ -2304,8 +2312,6 @@ ImpRed imp_update(hs_solver* s, const float* k_in, float* k_out, const float* dy
         imp_rhs(s, k_out, step, attempt, dt, nullptr);
         k_in = k_out;
     }
-    const long nred = (long)kImpVals * kImpLanes * kImpStride;
-    hipMemsetAsync(s->imp, 0, sizeof(unsigned long long) * nred, s->st);
     hipLaunchKernelGGL(k_imp, dim3(hs::grid1d(s->N), s->nsolve), hs::kBlock, 0, s->st, s->y0, s->acc, k_in, dy0,
                        y1, s->mB, c, norm_in ? s->p.normalize : 0, s->p.normalize, s->imp, s->N);
     double* fold = (double*)(s->imp + (long)kImpAll * kImpLanes * kImpStride);
@@ -2533,12 +2539,10 @@ void tm_rhs_all(hs_solver* s, float* km, double* ke, double* kp, uint32_t step, 
 TmRed tm_updates(hs_solver* s, const float* km, const double* ke, const double* kp, bool trap, bool norm_in,
                  bool with_y1, double h) {
     const long region = (long)kImpLanes * kImpStride;
-    const long nred = (kImpVals + 2 * kImp1Vals) * region;
     unsigned long long* rT = s->imp + kImpVals * region;
     unsigned long long* rP = rT + kImp1Vals * region;
     const double c = trap ? 0.5 * h : h;
     const unsigned g = hs::grid1d(s->N);
-    hipMemsetAsync(s->imp, 0, sizeof(unsigned long long) * nred, s->st);
     hipLaunchKernelGGL(k_imp, dim3(g, 1), hs::kBlock, 0, s->st, s->y0, s->acc, km, trap ? s->e : nullptr,
                        with_y1 ? s->y1 : nullptr, s->mB, c, norm_in ? s->p.normalize : 0, s->p.normalize, s->imp,
                        s->N);
