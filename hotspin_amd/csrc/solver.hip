@@ -2177,6 +2177,7 @@ extern "C" int hs_solver_energy(hs_solver* s, double* E) {
         hs::set_error(HS_EINVAL, "hs_solver_energy: null argument");
         return HS_EINVAL;
     }
+    if (!whole_batch_ok(s, "hs_solver_energy: handle is in slot mode")) return HS_EINVAL;
     const int nb = (int)hs::div_up(s->N, 256);
     if (!s->epart &&
         hipMalloc(&s->epart, sizeof(double) * ((size_t)nb * s->nsolve * kEnergyTerms + 5 * s->nsolve)) != hipSuccess) {

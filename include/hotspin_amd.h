@@ -187,7 +187,9 @@ HS_API int hs_solver_set_implicit_T(hs_solver* s, double maxAbsErr, double maxRe
  * weight * Ms0T0Mul * sum_cells (m . F) msat0T0 (LoadEnergy and
  * EnergyUpdater.Update, modules/mod_energy.go:25-61,98-116; a term whose
  * field is not loaded is 0).  FP64 m, H_ex and H_lf; B from the FP32 demag
- * of the state; deterministic FP64 sums.  E: 5 * nsolve doubles (host). */
+ * of the state; deterministic FP64 sums.  E: 5 * nsolve doubles (host).
+ * Two-temperature solves use T = Te (its FP32 copy, as the stages do); a
+ * handle in slot mode is refused (HS_EINVAL). */
 HS_API int hs_solver_energy(hs_solver* s, double* E);
 /* One RHS evaluation at the current state (no step): k and H_eff to host. */
 HS_API int hs_solver_rhs(hs_solver* s, double* k_host, double* heff_host, uint32_t step, uint32_t attempt,

@@ -122,3 +122,16 @@ def test_ensemble_graph_equals_batch_graphs():
     for a, b in zip(*runs):
         assert np.isfinite(a).all()
         assert np.array_equal(a, b)
+
+
+def test_energy_refuses_a_handle_in_slot_mode():
+    """The state of a slot batch lives in its slot buffers, not in the whole-batch
+    state that hs_solver_energy reads: the call is an error (HS_EINVAL), not a
+    read of stale memory."""
+    from hotspin_amd._lib import lib
+    g = SlotBatch(N, CELL, 2, **_kw(thermal=False))
+    _admit(g, 0, _req(0))
+    E = np.zeros(10)
+    rc = lib().hs_solver_energy(g._h, E.ctypes.data)
+    assert rc == 10001  # HS_EINVAL
+    lib().hs_clear_error()

@@ -165,3 +165,21 @@ def test_tm_rhs_with_debye_capacity_and_spin_heat():
         assert np.all(np.abs(dte - ote) <= bar), np.max(np.abs(dte - ote) / bar)
         # the spin term is well above that bar: leaving it out would fail
         assert np.abs(Qs / (gam * Te)).max() > 1e3 * bar.max()
+
+
+def test_energy_of_a_two_temperature_state():
+    """hs_solver_energy with T = Te per cell (the stages' FP32 copy of Te): the
+    oracle evaluated with T = Te rounded to FP32, terms to the bars of
+    tests/test_gpu_energy.py."""
+    tm, kw, m, Te, Tp = _problem()
+    g = TMSolver(N, CELL, tm=tm, T0=300.0, **kw)
+    g.set_m(m)
+    g.set_T(Te, Tp)
+    kwo = dict(kw)
+    kwo.update(TMul=1.0, T=Te.astype(np.float32).astype(np.float64))
+    o = po.LLB(N, CELL, **kwo)
+    mo, _ = o.rhs(m)
+    Eo = o.energy(mo)
+    Eg = g.energy()
+    for t, tol in ((0, 1e-12), (1, 1e-10), (2, 1e-5), (3, 1e-9)):
+        assert abs(Eg[t] - Eo[t]) <= tol * abs(Eo[t]), (t, Eg, Eo)
