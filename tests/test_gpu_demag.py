@@ -129,3 +129,13 @@ def test_every_fp32_pass_variant_matches_fp64(monkeypatch, N, pbc):
             err = np.abs(host(Bf) - B64).max() / np.abs(B64).max()
             assert err <= 2e-6, (kind, i, err)
             d.close()
+
+
+@pytest.mark.parametrize("N", [(11, 8, 8), (7, 7, 14), (4, 8, 1)])
+def test_unsupported_fft_length_is_an_error(N):
+    """Padded lengths must be 2^a 3^b 5^c with an even x length: 11 or 7 cells
+    (padded 22, 14) and a one-cell x extent (P2 = 1) are refused with
+    HS_ESIZE at plan creation, before anything runs."""
+    from hotspin_amd._lib import HotspinError
+    with pytest.raises(HotspinError, match="hs error 10002"):  # HS_ESIZE
+        Demag(N, (2e-9, 2e-9, 2e-9))
