@@ -456,11 +456,11 @@ void tune32(hs_demag* d, hipStream_t st) {
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    // best of two rounds of 3 launches after a warm-up launch
-    auto timeit = [&](auto&& launch) {
+    // best of `rounds` rounds of 3 launches after a warm-up launch
+    auto timeit = [&](auto&& launch, int rounds = 2) {
         launch();
         float best_ms = 1e30f;
-        for (int round = 0; round < 2; round++) {
+        for (int round = 0; round < rounds; round++) {
             hipEventRecord(e0, st);
             for (int r = 0; r < 3; r++) launch();
             hipEventRecord(e1, st);
@@ -500,17 +500,19 @@ void tune32(hs_demag* d, hipStream_t st) {
     // One sweep of refinement inside the whole five-pass convolution: each
     // pass's variants again, the other passes fixed, so a pass is timed with
     // the caches and neighbours it has in a step (isolated timings can rank
-    // close variants the other way).
+    // close variants the other way).  Best of three rounds, and a variant
+    // replaces the current one only when it is more than 1 % faster (closer
+    // than that they are equivalent, and timing noise would decide).
     {
         auto pipe = [&] { run_field32(d, m, nullptr, 1.0, B, st); };
-        float cur = timeit(pipe);
+        float cur = timeit(pipe, 3);
         auto sweep = [&](auto& slot, auto& list, int which) {
             for (size_t i = 0; i < list.size(); i++) {
                 if ((int)i == pick[which]) continue;
                 auto keep = slot;
                 slot = list[i];
-                const float t = timeit(pipe);
-                if (t < cur) {
+                const float t = timeit(pipe, 3);
+                if (t < 0.99f * cur) {
                     cur = t;
                     pick[which] = (int)i;
                 } else {
