@@ -604,7 +604,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     d->K[2] = d->H2;
     d->ncell = (long)N[0] * N[1] * N[2];
     if (!sizes_ok(d)) {
-        hs::set_error(HS_ESIZE, "hs_demag_create: padded size needs even P2 and 2^a 3^b 5^c FFT lengths");
+        hs::set_error(HS_ESIZE, "hs_demag_create: padded lengths must be 2^k, 3*2^k or 5*2^k (even along x)");
         delete d;
         return nullptr;
     }

@@ -182,7 +182,7 @@ __device__ __forceinline__ void fft_stages(cpx<T>* lds, const cpx<T>* __restrict
 // __syncthreads); returns with the result visible to all threads.
 template <typename T, int L, int W, int NT, int S>
 __device__ __forceinline__ void fft_lds(cpx<T>* lds, const cpx<T>* __restrict__ tw, int tid) {
-    static_assert(radices_of(L).n >= 0, "FFT length must be 2^a 3^b 5^c");
+    static_assert(radices_of(L).n >= 0, "FFT length must be a product of 2, 3 and 5");
     if constexpr (L > 1) fft_stages<T, L, W, NT, S, 0, 1>(lds, tw, tid);
 }
 

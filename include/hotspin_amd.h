@@ -30,7 +30,7 @@ extern "C" {
 /* 0 = no error since the last hs_clear_error(); otherwise a hipError_t code
  * or one of the HS_E* codes below.  Sticky: the first error is kept. */
 #define HS_EINVAL 10001 /* invalid argument (sizes, NULL where not allowed) */
-#define HS_ESIZE 10002  /* FFT length not supported (padded lengths 2^a 3^b 5^c, even along x) */
+#define HS_ESIZE 10002  /* FFT length not supported (padded lengths 2^k, 3*2^k or 5*2^k, even along x) */
 #define HS_ECONV 10003  /* adaptive step did not converge within 10 retries */
 #define HS_ENOMEM 10004
 HS_API int hs_last_error(void);

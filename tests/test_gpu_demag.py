@@ -133,7 +133,7 @@ def test_every_fp32_pass_variant_matches_fp64(monkeypatch, N, pbc):
 
 @pytest.mark.parametrize("N", [(11, 8, 8), (7, 7, 14), (4, 8, 1)])
 def test_unsupported_fft_length_is_an_error(N):
-    """Padded lengths must be 2^a 3^b 5^c with an even x length: 11 or 7 cells
+    """Padded lengths must be 2^k, 3 2^k or 5 2^k, even along x: 11 or 7 cells
     (padded 22, 14) and a one-cell x extent (P2 = 1) are refused with
     HS_ESIZE at plan creation, before anything runs."""
     from hotspin_amd._lib import HotspinError
