@@ -19,6 +19,7 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <limits>
 #include <vector>
 
 #include "hs_common.hpp"
@@ -1409,6 +1410,8 @@ struct hs_solver {
     double* epart = nullptr;  // energy: block partials, then the per-solve terms
     double* imp_host = nullptr;  // pinned: the folded maxima of a fixed-point update
     bool imp_zeroed = false;     // imp lanes cleared once; k_imp_fold clears them after each read
+    double* err_dev = nullptr;   // lockstep error estimate, folded on the device
+    double* err_host = nullptr;  // pinned copy of it
 };
 
 namespace {
@@ -1544,19 +1547,41 @@ void demag_into(hs_solver* s, const float* m, float* B, const int* live = nullpt
         hs_demag_field_f32_internal(s->demag, m, s->phys.msk, s->p.msat0T0Mul, B, s->st, s->nsolve, live);
 }
 
-// max over the batch of the per-solve error maxima (lockstep stepping)
+// Max of the error lanes em[q], q = 0, kErrStride, ... < n, in one block of
+// 256 threads into *out: each thread keeps v = (v < x) ? x : v over its lanes
+// (a NaN lane leaves v unchanged, as a sequential std::max from 0 does), then
+// a tree of the same comparison over finite values.
+__global__ void __launch_bounds__(256) k_err_fold(const unsigned long long* __restrict__ em, long n,
+                                                  double* __restrict__ out) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    double v = 0.0;
+    for (long q = (long)t * kErrStride; q < n; q += 256L * kErrStride) {
+        const double x = __longlong_as_double((long long)em[q]);
+        v = (v < x) ? x : v;
+    }
+    red[t] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] = (red[t] < red[t + h]) ? red[t + h] : red[t];
+        __syncthreads();
+    }
+    if (t == 0) *out = red[0];
+}
+
+// max over the batch of the per-solve error maxima (lockstep stepping): folded
+// on the device, one double back into pinned memory
 double read_err(hs_solver* s) {
     const long n = (long)s->nsolve * kErrPerSolve;
-    std::vector<unsigned long long> b(n);
-    hipMemcpyAsync(b.data(), s->errmax, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s->st);
-    hipStreamSynchronize(s->st);
-    double m = 0.0;
-    for (long q = 0; q < n; q += kErrStride) {
-        double v;
-        memcpy(&v, &b[q], sizeof(v));
-        m = std::max(m, v);
+    if ((!s->err_dev && hipMalloc(&s->err_dev, sizeof(double)) != hipSuccess) ||
+        (!s->err_host && hipHostMalloc((void**)&s->err_host, sizeof(double)) != hipSuccess)) {
+        hs::set_error(HS_ENOMEM, "error estimate: out of memory");
+        return std::numeric_limits<double>::quiet_NaN();
     }
-    return m;
+    hipLaunchKernelGGL(k_err_fold, 1, 256, 0, s->st, s->errmax, n, s->err_dev);
+    hipMemcpyAsync(s->err_host, s->err_dev, sizeof(double), hipMemcpyDeviceToHost, s->st);
+    hipStreamSynchronize(s->st);
+    return *s->err_host;
 }
 void zero_err(hs_solver* s) {
     hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * s->nsolve * kErrPerSolve, s->st);
@@ -2127,9 +2152,10 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
     if (s->slot_ev) hipEventDestroy(s->slot_ev);
     if (s->slot_host) hipHostFree(s->slot_host);
     if (s->imp_host) hipHostFree(s->imp_host);
+    if (s->err_host) hipHostFree(s->err_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
-                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1, s->qs, s->epart};
+                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1, s->qs, s->epart, s->err_dev};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
