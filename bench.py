@@ -404,6 +404,21 @@ def main():
     out["pcie_inclusive"] = {"value": ncell * stp.steps / el_p, "unit": "cell-steps/s", "steps": stp.steps,
                              "note": "set_m + %d RK23 steps + get_m, host FP64 state copied in and out" % stp.steps}
     log("bench: with host hand-over %.3e cell-steps/s" % (ncell * stp.steps / el_p))
+    # the reference's own adaptive solver (solver/rk12, solve_rk12.go) on the same workload:
+    # host-driven Heun-Euler with the reference's controller and retries
+    s.set_m(m)
+    dt12, _ = s.run("rk12", 3, 1e-15, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dt12, st12 = s.run("rk12", args.steps, dt12, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12)
+    torch.cuda.synchronize()
+    el12 = time.perf_counter() - t0
+    out["rk12"] = {"value": ncell * st12.steps / el12, "unit": "cell-steps/s", "steps": st12.steps,
+                   "rejected": st12.rejected, "ms_per_step": el12 / max(st12.steps, 1) * 1e3, "final_dt": dt12,
+                   "note": "reference semantics (solve_rk12.go: 2 RHS per attempt, host-side controller); "
+                           "its first-order error estimate holds dt far below RK23's, so compare simulated "
+                           "time per second (final_dt), not cell-steps/s"}
+    log("bench: rk12 %d steps in %.3f s (%d rejected), dt=%.3e" % (st12.steps, el12, st12.rejected, dt12))
     # energy terms of the final state (micromag/energy): one FP64 pass reading the FP64
     # state (24 B/cell) and the FP32 demag field (12 B/cell) after its own demag evaluation
     s.energy()
