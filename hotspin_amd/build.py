@@ -3,9 +3,10 @@
 Every HIP source under hotspin_amd/csrc is compiled for --offload-arch=gfx950
 only, with packed-FP32 instructions disabled, and linked into
 hotspin_amd/libhotspin_amd.so.  Objects are cached under hotspin_amd/_build
-and rebuilt when a source or header is newer.
+and rebuilt when a source or header is newer; build(force=True) (the
+repository's build entry point) recompiles every source.
 
-Usage: python -m hotspin_amd.build [-v] [-j N]
+Usage: python -m hotspin_amd.build [-v] [-j N] [--force]
 """
 import glob
 import os
@@ -30,10 +31,10 @@ def _headers():
     return glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
 
 
-def _compile(src, verbose):
+def _compile(src, verbose, force=False):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     newest = max([os.path.getmtime(src)] + [os.path.getmtime(h) for h in _headers()])
-    if os.path.exists(obj) and os.path.getmtime(obj) >= newest:
+    if not force and os.path.exists(obj) and os.path.getmtime(obj) >= newest:
         return obj
     lang = ["-x", "hip"] if src.endswith(".hip") else []
     cmd = [HIPCC] + COMMON + lang + ["-c", src, "-o", obj + ".tmp"]
@@ -44,11 +45,11 @@ def _compile(src, verbose):
     return obj
 
 
-def build(verbose=False, jobs=8):
+def build(verbose=False, jobs=8, force=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+        objs = list(ex.map(lambda s: _compile(s, verbose, force), srcs))
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", OUT + ".tmp"] + objs + ["-fopenmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -61,4 +62,4 @@ if __name__ == "__main__":
     j = 8
     if "-j" in sys.argv:
         j = int(sys.argv[sys.argv.index("-j") + 1])
-    print(build(verbose="-v" in sys.argv, jobs=j))
+    print(build(verbose="-v" in sys.argv, jobs=j, force="--force" in sys.argv))
