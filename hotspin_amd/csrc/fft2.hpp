@@ -203,6 +203,7 @@ __global__ void __launch_bounds__(NT) k_cols2(const cpx<T>* in, cpx<T>* out, Geo
     }
 }
 
+
 // ---- column pass with a split LDS exchange --------------------------------
 // The k_cols2 contract with the phase-1 -> phase-2 exchange done in two real
 // halves (real parts, then imaginary parts) through an L*W array of T: half
@@ -767,5 +768,89 @@ __global__ void __launch_bounds__(6 * W) k_zconv2s(cpx<T>* X, const TK* __restri
 #pragma unroll
         for (int t = 0; t < L; t++)
             if (t < N0) base[t * tstride] = v[t];
+    }
+}
+
+// k_zconv2 with non-temporal pencil loads and stores: the spectrum is read
+// once and written once per convolution, so its lines need not stay in L2
+// (the kernel octant, reused by the mirror pencil and the t, L - t pair,
+// keeps the normal policy).
+template <typename T, typename TK, int L, int W>
+__global__ void __launch_bounds__(6 * W) k_zconv2n(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+                                                   int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
+                                                   Live lv) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    constexpr int WP = 3 * W, W2 = 2 * WP, NT = 6 * W;
+    static_assert(reg_dft_ok(L), "register DFT length");
+    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
+    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
+    __shared__ cpx<T> lds[L * W2];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int jp = lb / gridDim.x;
+    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;
+    const int col0 = (lb % gridDim.x) * W;
+    const int p = tid % W, c = (tid / W) % 3, sm = tid / WP;
+    const int j = sm ? jm : jp;
+    const bool live = j >= 0 && col0 + p < H2;
+    const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
+    cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p;
+    {
+        cpx<T> v[L];
+#pragma unroll
+        for (int t = 0; t < L; t++) {
+            cpx<T> x = {T(0), T(0)};
+            if (live && t < N0) {
+                const v2 q = __builtin_nontemporal_load((const v2*)(base + t * tstride));
+                x = {q.x, q.y};
+            }
+            v[t] = x;
+        }
+        dft_reg<T, L, -1>(v);
+#pragma unroll
+        for (int t = 0; t < L; t++) lds[t * W2 + tid] = v[t];
+    }
+    __syncthreads();
+    const long kplane = (long)(L / 2 + 1) * K1 * kp;
+    for (int e = tid; e < L * W; e += NT) {
+        const int t = e / W, pp = e % W;
+        const int col = col0 + pp;
+        if (col >= H2) continue;
+        const int ii = (t > L / 2) ? L - t : t;
+        const T si = (t > L / 2) ? T(-1) : T(1);
+        const long K = ((long)ii * K1 + jp) * kp + col;
+        const T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
+        const T kyz0 = T(kern[3 * kplane + K]);
+        const T kxz = si * T(kern[4 * kplane + K]);
+        const T kxy0 = si * T(kern[5 * kplane + K]);
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const T sj = s2 ? T(-1) : T(1);
+            const T kyz = sj * kyz0, kxy = sj * kxy0;
+            cpx<T>* a = lds + t * W2 + s2 * WP + pp;
+            cpx<T> mx = cscale(a[0], srcMul);
+            cpx<T> my = cscale(a[W], srcMul);
+            cpx<T> mz = cscale(a[2 * W], srcMul);
+            a[0] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
+            a[W] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
+            a[2 * W] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
+        }
+    }
+    __syncthreads();
+    {
+        cpx<T> v[L];
+#pragma unroll
+        for (int t = 0; t < L; t++) v[t] = lds[t * W2 + tid];
+        dft_reg<T, L, +1>(v);
+        if (live) {
+#pragma unroll
+            for (int t = 0; t < L; t++)
+                if (t < N0) {
+                    v2 q;
+                    q.x = v[t].x;
+                    q.y = v[t].y;
+                    __builtin_nontemporal_store(q, (v2*)(base + t * tstride));
+                }
+        }
     }
 }
