@@ -570,14 +570,35 @@ __device__ __forceinline__ bool has_m(const StageIO& io) {
     return !(ROLE == R_RHS || ROLE == R_BS_4) && io.mout;
 }
 
+// Stored slopes and error partials of an RK stage are read back only one or
+// two demag evaluations later: non-temporal stores keep them from displacing
+// the lines the passes in between reuse (R_RHS outputs go to the host).
+template <int ROLE>
+__device__ __forceinline__ void st_slope(float* a, float v) {
+    if constexpr (ROLE == R_RHS) *a = v;
+    else __builtin_nontemporal_store(v, a);
+}
+template <int ROLE>
+__device__ __forceinline__ void st_slope2(float* a, float v0, float v1) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    if constexpr (ROLE == R_RHS) {
+        *(float2*)a = make_float2(v0, v1);
+    } else {
+        f2 q;
+        q.x = v0;
+        q.y = v1;
+        __builtin_nontemporal_store(q, (f2*)a);
+    }
+}
+
 // Scalar stores of one cell's outputs.
 template <int ROLE>
 __device__ __forceinline__ void store_out(const Phys& P, const StageIO& io, long I, long N, const Out& o,
                                           const Pre& q) {
     if (float* d = k_dst<ROLE>(io))
-        for (int c = 0; c < 3; c++) d[c * N + I] = o.k[c];
+        for (int c = 0; c < 3; c++) st_slope<ROLE>(d + c * N + I, o.k[c]);
     if (float* d = h_dst<ROLE>(io))
-        for (int c = 0; c < 3; c++) d[c * N + I] = o.h[c];
+        for (int c = 0; c < 3; c++) st_slope<ROLE>(d + c * N + I, o.h[c]);
     if (has_m<ROLE>(io))
         for (int c = 0; c < 3; c++) io.mout[c * N + I] = o.m[c];
     if (double* d = y_dst<ROLE>(P, io))
@@ -598,9 +619,9 @@ template <int ROLE>
 __device__ __forceinline__ void store_out2(const Phys& P, const StageIO& io, long I, long N, const Out& a,
                                            const Out& b, const Pre& qa, const Pre& qb) {
     if (float* d = k_dst<ROLE>(io))
-        for (int c = 0; c < 3; c++) *(float2*)(d + c * N + I) = make_float2(a.k[c], b.k[c]);
+        for (int c = 0; c < 3; c++) st_slope2<ROLE>(d + c * N + I, a.k[c], b.k[c]);
     if (float* d = h_dst<ROLE>(io))
-        for (int c = 0; c < 3; c++) *(float2*)(d + c * N + I) = make_float2(a.h[c], b.h[c]);
+        for (int c = 0; c < 3; c++) st_slope2<ROLE>(d + c * N + I, a.h[c], b.h[c]);
     if (has_m<ROLE>(io))
         for (int c = 0; c < 3; c++) *(float2*)(io.mout + c * N + I) = make_float2(a.m[c], b.m[c]);
     if (double* d = y_dst<ROLE>(P, io))
