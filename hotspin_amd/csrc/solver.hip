@@ -601,8 +601,13 @@ __device__ __forceinline__ void store_out(const Phys& P, const StageIO& io, long
         for (int c = 0; c < 3; c++) st_slope<ROLE>(d + c * N + I, o.h[c]);
     if (has_m<ROLE>(io))
         for (int c = 0; c < 3; c++) io.mout[c * N + I] = o.m[c];
-    if (double* d = y_dst<ROLE>(P, io))
-        for (int c = 0; c < 3; c++) d[c * N + I] = o.y[c];
+    if (double* d = y_dst<ROLE>(P, io)) {
+        if constexpr (ROLE == R_BS_3) {  // y_{n+1}: read back at the next step, after another demag
+            for (int c = 0; c < 3; c++) __builtin_nontemporal_store(o.y[c], d + c * N + I);
+        } else {
+            for (int c = 0; c < 3; c++) d[c * N + I] = o.y[c];
+        }
+    }
     if constexpr (ROLE == R_RK12_1 || ROLE == R_EULER) {  // lazily normalised y0 written back
         if (P.normalize && ROLE == R_RK12_1) {
             double y[3] = {q.y[0], q.y[1], q.y[2]};
@@ -624,8 +629,19 @@ __device__ __forceinline__ void store_out2(const Phys& P, const StageIO& io, lon
         for (int c = 0; c < 3; c++) st_slope2<ROLE>(d + c * N + I, a.h[c], b.h[c]);
     if (has_m<ROLE>(io))
         for (int c = 0; c < 3; c++) *(float2*)(io.mout + c * N + I) = make_float2(a.m[c], b.m[c]);
-    if (double* d = y_dst<ROLE>(P, io))
-        for (int c = 0; c < 3; c++) *(double2*)(d + c * N + I) = make_double2(a.y[c], b.y[c]);
+    if (double* d = y_dst<ROLE>(P, io)) {
+        if constexpr (ROLE == R_BS_3) {  // y_{n+1}: read back at the next step, after another demag
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            for (int c = 0; c < 3; c++) {
+                d2 q;
+                q.x = a.y[c];
+                q.y = b.y[c];
+                __builtin_nontemporal_store(q, (d2*)(d + c * N + I));
+            }
+        } else {
+            for (int c = 0; c < 3; c++) *(double2*)(d + c * N + I) = make_double2(a.y[c], b.y[c]);
+        }
+    }
     if constexpr (ROLE == R_RK12_1) {
         if (P.normalize) {
             double ya[3] = {qa.y[0], qa.y[1], qa.y[2]}, yb[3] = {qb.y[0], qb.y[1], qb.y[2]};
