@@ -46,6 +46,14 @@ extern "C" const char* hs_version(void) { return "hotspin_amd 0.1 gfx950"; }
         }                                                 \
     } while (0)
 
+// true when every pointer is non-null: the arrays a kernel always
+// dereferences (masks, which may be NULL = uniform, are not passed here)
+static inline bool nn() { return true; }
+template <class P, class... R>
+static inline bool nn(P p, R... r) {
+    return p != nullptr && nn(r...);
+}
+
 // ------------------------------------------------------------------ stencils
 // exchange6Kern, src/libhotspin/exchange6.cu:11-127 (neighbour order X, Z, Y)
 __global__ void __launch_bounds__(256) k_exchange6(double* __restrict__ hx, double* __restrict__ hy,
@@ -101,7 +109,7 @@ extern "C" void exchange6Async(double* hx, double* hy, double* hz, double* mx, d
                                double* msat0T0, double* lex, int N0, int N1, int N2, int p0, int p1, int p2,
                                double msat0T0Mul, double c0, double c1, double c2, CUstream stream) {
     (void)msat0T0;  // accepted but unused, as in exchange6.cu:134-147
-    ARGCHECK(N0 > 0 && N1 > 0 && N2 > 0 && hx && mx, "exchange6Async");
+    ARGCHECK(N0 > 0 && N1 > 0 && N2 > 0 && nn(hx, hy, hz, mx, my, mz), "exchange6Async");
     dim3 blk(64, 4), grd(hs::div_up(N2, 64), hs::div_up(N1, 4), N0);
     hipLaunchKernelGGL(k_exchange6, grd, blk, 0, hs::S(stream), hx, hy, hz, mx, my, mz, lex, N0, N1, N2, p0, p1,
                        p2, msat0T0Mul, c0, c1, c2);
@@ -176,7 +184,7 @@ __global__ void k_torque(double* tx, double* ty, double* tz, const double* Mx, c
 extern "C" void llbar_torque_async(double* tx, double* ty, double* tz, double* Mx, double* My, double* Mz,
                                    double* hx, double* hy, double* hz, double* msat0T0, CUstream stream,
                                    int N) {
-    ARGCHECK(N >= 0, "llbar_torque_async");
+    ARGCHECK(N >= 0 && (N == 0 || nn(tx, ty, tz, Mx, My, Mz, hx, hy, hz)), "llbar_torque_async");
     if (N == 0) return;
     hipLaunchKernelGGL(k_torque, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), tx, ty, tz, Mx, My, Mz, hx, hy, hz,
                        msat0T0, N);
@@ -203,7 +211,7 @@ __global__ void k_local00nc(double* tx, double* ty, double* tz, const double* hx
 extern "C" void llbar_local00nc_async(double* tx, double* ty, double* tz, double* hx, double* hy, double* hz,
                                       double* msat0T0, double* lxx, double* lyy, double* lzz, const double mxx,
                                       const double myy, const double mzz, CUstream stream, int N) {
-    ARGCHECK(N >= 0, "llbar_local00nc_async");
+    ARGCHECK(N >= 0 && (N == 0 || nn(tx, ty, tz, hx, hy, hz)), "llbar_local00nc_async");
     if (N == 0) return;
     hipLaunchKernelGGL(k_local00nc, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), tx, ty, tz, hx, hy, hz, msat0T0,
                        lxx, lyy, lzz, mxx, myy, mzz, N);
@@ -234,7 +242,7 @@ extern "C" void llbar_local02nc_async(double* tx, double* ty, double* tz, double
                                       double* hx, double* hy, double* hz, double* msat0T0, double* uxx,
                                       double* uyy, double* uzz, const double mxx, const double myy,
                                       const double mzz, CUstream stream, int N) {
-    ARGCHECK(N >= 0, "llbar_local02nc_async");
+    ARGCHECK(N >= 0 && (N == 0 || nn(tx, ty, tz, mx, my, mz, hx, hy, hz)), "llbar_local02nc_async");
     if (N == 0) return;
     hipLaunchKernelGGL(k_local02nc, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), tx, ty, tz, mx, my, mz, hx, hy,
                        hz, msat0T0, uxx, uyy, uzz, mxx, myy, mzz, N);
@@ -265,7 +273,7 @@ extern "C" void llbar_local02c_async(double* tx, double* ty, double* tz, double*
                                      double* hx, double* hy, double* hz, double* msat0T0, double* uxx,
                                      double* uyy, double* uzz, const double mxx, const double myy,
                                      const double mzz, CUstream stream, int N) {
-    ARGCHECK(N >= 0, "llbar_local02c_async");
+    ARGCHECK(N >= 0 && (N == 0 || nn(tx, ty, tz, mx, my, mz, hx, hy, hz)), "llbar_local02c_async");
     if (N == 0) return;
     hipLaunchKernelGGL(k_local02c, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), tx, ty, tz, mx, my, mz, hx, hy,
                        hz, msat0T0, uxx, uyy, uzz, mxx, myy, mzz, N);
@@ -302,7 +310,7 @@ __global__ void k_long_field(double* hx, double* hy, double* hz, const double* m
 extern "C" void long_field_async(double* hx, double* hy, double* hz, double* mx, double* my, double* mz,
                                  double* msat0T0, double* S, double* n, double* Tc, double* Ts, double msat0T0Mul,
                                  double SMul, double nMul, double TcMul, double TsMul, int N, CUstream stream) {
-    ARGCHECK(N >= 0, "long_field_async");
+    ARGCHECK(N >= 0 && (N == 0 || nn(hx, hy, hz, mx, my, mz)), "long_field_async");
     if (N == 0) return;
     hipLaunchKernelGGL(k_long_field, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), hx, hy, hz, mx, my, mz,
                        msat0T0, S, n, Tc, Ts, msat0T0Mul, SMul, nMul, TcMul, TsMul, N);
@@ -329,7 +337,7 @@ __global__ void k_temp_scale(double* hx, double* hy, double* hz, const double* u
 extern "C" void temperature_scaleAnizNoise(double* hx, double* hy, double* hz, double* uxx, double* uyy,
                                            double* uzz, double* T, double* msk, double mxx, double myy,
                                            double mzz, double K, CUstream stream, int N) {
-    ARGCHECK(N >= 0, "temperature_scaleAnizNoise");
+    ARGCHECK(N >= 0 && (N == 0 || nn(hx, hy, hz)), "temperature_scaleAnizNoise");
     if (N == 0) return;
     hipLaunchKernelGGL(k_temp_scale, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), hx, hy, hz, uxx, uyy, uzz, T,
                        msk, mxx, myy, mzz, K, N);
@@ -397,7 +405,7 @@ __global__ void k_brillouin(double* msat0, const double* msMsk, const double* T,
 extern "C" void brillouinAsync(double* msat0, double* msat0T0, double* T, double* Tc, double* S,
                                const double msat0Mul, const double msat0T0Mul, const double TcMul,
                                const double SMul, int N, CUstream stream) {
-    ARGCHECK(N >= 0 && T, "brillouinAsync");
+    ARGCHECK(N >= 0 && (N == 0 || nn(msat0, T)), "brillouinAsync");
     if (N == 0) return;
     hipLaunchKernelGGL(k_brillouin, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), msat0, msat0T0, T, Tc, S,
                        msat0Mul, msat0T0Mul, TcMul, SMul, N);
@@ -413,7 +421,7 @@ __global__ void k_normalize(double* mx, double* my, double* mz, int N) {
     mx[i] = a * n; my[i] = b * n; mz[i] = c * n;
 }
 extern "C" void normalizeAsync(double* mx, double* my, double* mz, CUstream stream, int N) {
-    ARGCHECK(N >= 0, "normalizeAsync");
+    ARGCHECK(N >= 0 && (N == 0 || nn(mx, my, mz)), "normalizeAsync");
     if (N == 0) return;
     hipLaunchKernelGGL(k_normalize, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), mx, my, mz, N);
     HS_LAUNCH_CHECK("normalizeAsync");
@@ -426,7 +434,7 @@ __global__ void k_qinter(double* Qi, const double* Ti, const double* Tj, const d
     Qi[i] = g * (Tj[i] - Ti[i]);
 }
 extern "C" void QinterAsync(double* Qi, double* Ti, double* Tj, double* Gij, double GijMul, int N, CUstream s) {
-    ARGCHECK(N >= 0, "QinterAsync");
+    ARGCHECK(N >= 0 && (N == 0 || nn(Qi, Ti, Tj)), "QinterAsync");
     if (N == 0) return;
     hipLaunchKernelGGL(k_qinter, hs::grid1d(N), hs::kBlock, 0, hs::S(s), Qi, Ti, Tj, Gij, GijMul, N);
     HS_LAUNCH_CHECK("QinterAsync");
@@ -502,53 +510,54 @@ __global__ void k_dotsign(double* d, const double* ax, const double* ay, const d
     d[i] = copysign(p, s);
 }
 
-#define L1(k, ...)                                                                   \
+#define L1(req, k, ...)                                                              \
     do {                                                                             \
         if (N <= 0) return;                                                          \
+        ARGCHECK(req, #k ": NULL operand");                                          \
         hipLaunchKernelGGL(k, hs::grid1d(N), hs::kBlock, 0, hs::S(stream), __VA_ARGS__); \
         HS_LAUNCH_CHECK(#k);                                                         \
     } while (0)
 
-extern "C" void addAsync(double* dst, double* a, double* b, CUstream stream, int N) { L1(k_add, dst, a, b, N); }
+extern "C" void addAsync(double* dst, double* a, double* b, CUstream stream, int N) { L1(nn(dst, a, b), k_add, dst, a, b, N); }
 extern "C" void linearCombination2Async(double* dst, double* a, double ma, double* b, double mb, CUstream stream,
                                         int N) {
-    L1(k_lc2, dst, a, ma, b, mb, N);
+    L1(nn(dst, a, b), k_lc2, dst, a, ma, b, mb, N);
 }
 extern "C" void linearCombination3Async(double* dst, double* a, double ma, double* b, double mb, double* c,
                                         double mc, CUstream stream, int N) {
-    L1(k_lc3, dst, a, ma, b, mb, c, mc, N);
+    L1(nn(dst, a, b, c), k_lc3, dst, a, ma, b, mb, c, mc, N);
 }
 extern "C" void addMaddAsync(double* dst, double* a, double* b, double* c, double m, CUstream stream, int N) {
-    L1(k_addmadd, dst, a, b, c, m, N);
+    L1(nn(dst, a, b, c), k_addmadd, dst, a, b, c, m, N);
 }
-extern "C" void madd1Async(double* a, double* b, double mb, CUstream stream, int N) { L1(k_madd1, a, b, mb, N); }
+extern "C" void madd1Async(double* a, double* b, double mb, CUstream stream, int N) { L1(nn(a), k_madd1, a, b, mb, N); }
 extern "C" void madd2Async(double* a, double* b, double mb, double* c, double mc, CUstream stream, int N) {
-    L1(k_madd2, a, b, mb, c, mc, N);
+    L1(nn(a), k_madd2, a, b, mb, c, mc, N);
 }
 extern "C" void maddAsync(double* dst, double* a, double* b, double mb, CUstream stream, int N) {
-    L1(k_madd, dst, a, b, mb, N);
+    L1(nn(dst, a), k_madd, dst, a, b, mb, N);
 }
 extern "C" void vecMaddAsync(double* dx, double* dy, double* dz, double* ax, double* ay, double* az, double* bx,
                              double* by, double* bz, double mx, double my, double mz, CUstream stream, int N) {
-    L1(k_vecmadd, dx, dy, dz, ax, ay, az, bx, by, bz, mx, my, mz, N);
+    L1(nn(dx, dy, dz, ax, ay, az) && (!bx || nn(by, bz)), k_vecmadd, dx, dy, dz, ax, ay, az, bx, by, bz, mx, my, mz, N);
 }
 extern "C" void cmaddAsync(double* dst, double a, double b, double* kern, double* src, CUstream stream, int N) {
-    L1(k_cmadd, dst, a, b, kern, src, N);
+    L1(nn(dst, kern, src), k_cmadd, dst, a, b, kern, src, N);
 }
-extern "C" void mulAsync(double* dst, double* a, double* b, CUstream stream, int N) { L1(k_mul, dst, a, b, N); }
-extern "C" void divAsync(double* dst, double* a, double* b, CUstream stream, int N) { L1(k_div, dst, a, b, N); }
+extern "C" void mulAsync(double* dst, double* a, double* b, CUstream stream, int N) { L1(nn(dst, a, b), k_mul, dst, a, b, N); }
+extern "C" void divAsync(double* dst, double* a, double* b, CUstream stream, int N) { L1(nn(dst, a), k_div, dst, a, b, N); }
 extern "C" void dotMaskAsync(double* dst, double* ax, double* ay, double* az, double* bx, double* by, double* bz,
                              double axm, double aym, double azm, double bxm, double bym, double bzm, CUstream stream,
                              int N) {
-    L1(k_dotmask, dst, ax, ay, az, bx, by, bz, axm, aym, azm, bxm, bym, bzm, N);
+    L1(nn(dst), k_dotmask, dst, ax, ay, az, bx, by, bz, axm, aym, azm, bxm, bym, bzm, N);
 }
 extern "C" void dotAsync(double* dst, double* ax, double* ay, double* az, double* bx, double* by, double* bz,
                          CUstream stream, int N) {
-    L1(k_dot, dst, ax, ay, az, bx, by, bz, N);
+    L1(nn(dst, ax, ay, az, bx, by, bz), k_dot, dst, ax, ay, az, bx, by, bz, N);
 }
 extern "C" void dotSignAsync(double* dst, double* ax, double* ay, double* az, double* bx, double* by, double* bz,
                              double* cx, double* cy, double* cz, CUstream stream, int N) {
-    L1(k_dotsign, dst, ax, ay, az, bx, by, bz, cx, cy, cz, N);
+    L1(nn(dst, ax, ay, az, bx, by, bz, cx, cy, cz), k_dotsign, dst, ax, ay, az, bx, by, bz, cx, cy, cz, N);
 }
 
 // tensSYMMVecMulKern, mul.cu:31-90 (in place allowed: dst may alias src)
@@ -584,7 +593,7 @@ __global__ void k_tens(double* dX, double* dY, double* dZ, const double* sX, con
 extern "C" void tensSYMMVecMul(double* dstX, double* dstY, double* dstZ, double* srcX, double* srcY, double* srcZ,
                                double* kXX, double* kYY, double* kZZ, double* kYZ, double* kXZ, double* kXY,
                                const double srcMul, const int Nx, const int Ny, const int Nz, CUstream stream) {
-    ARGCHECK(Nx > 0 && Ny > 0 && Nz > 0, "tensSYMMVecMul");
+    ARGCHECK(Nx > 0 && Ny > 0 && Nz > 0 && nn(dstX, dstY, dstZ, srcX, srcY, srcZ), "tensSYMMVecMul");
     long n = (long)Nx * Ny * Nz;
     hipLaunchKernelGGL(k_tens, hs::grid1d(n), hs::kBlock, 0, hs::S(stream), dstX, dstY, dstZ, srcX, srcY, srcZ, kXX,
                        kYY, kZZ, kYZ, kXZ, kXY, srcMul, Nx, Ny, Nz);
@@ -606,7 +615,8 @@ __global__ void k_copy3d(double* dst, int D1, int D2, long dstComp, const double
 }
 extern "C" void copyPad3DAsync(double* dst, int D0, int D1, int D2, double* src, int S0, int S1, int S2, int Ncomp,
                                CUstream stream) {
-    ARGCHECK(S0 <= D0 && S1 <= D1 && S2 <= D2 && S0 > 0 && S1 > 0 && S2 > 0 && Ncomp > 0, "copyPad3DAsync");
+    ARGCHECK(S0 <= D0 && S1 <= D1 && S2 <= D2 && S0 > 0 && S1 > 0 && S2 > 0 && Ncomp > 0 && nn(dst, src),
+             "copyPad3DAsync");
     long n = (long)S0 * S1 * S2;
     dim3 g(hs::grid1d(n), Ncomp);
     hipLaunchKernelGGL(k_copy3d, g, hs::kBlock, 0, hs::S(stream), dst, D1, D2, (long)D0 * D1 * D2, src, S1, S2, n,
@@ -615,7 +625,8 @@ extern "C" void copyPad3DAsync(double* dst, int D0, int D1, int D2, double* src,
 }
 extern "C" void copyUnPad3DAsync(double* dst, int D0, int D1, int D2, double* src, int S0, int S1, int S2,
                                  int Ncomp, CUstream stream) {
-    ARGCHECK(D0 <= S0 && D1 <= S1 && D2 <= S2 && D0 > 0 && D1 > 0 && D2 > 0 && Ncomp > 0, "copyUnPad3DAsync");
+    ARGCHECK(D0 <= S0 && D1 <= S1 && D2 <= S2 && D0 > 0 && D1 > 0 && D2 > 0 && Ncomp > 0 && nn(dst, src),
+             "copyUnPad3DAsync");
     long n = (long)D0 * D1 * D2;
     dim3 g(hs::grid1d(n), Ncomp);
     hipLaunchKernelGGL(k_copy3d, g, hs::kBlock, 0, hs::S(stream), dst, D1, D2, n, src, S1, S2,
@@ -623,7 +634,7 @@ extern "C" void copyUnPad3DAsync(double* dst, int D0, int D1, int D2, double* sr
     HS_LAUNCH_CHECK("copyUnPad3DAsync");
 }
 __global__ void k_zero(double* a, int N) { LANE_LOOP(N) a[i] = 0.0; }
-extern "C" void zeroArrayAsync(double* A, int N, CUstream stream) { L1(k_zero, A, N); }
+extern "C" void zeroArrayAsync(double* A, int N, CUstream stream) { L1(nn(A), k_zero, A, N); }
 
 // ------------------------------------------------------------------ partial reductions, reduce.cu
 // Each of the caller's `blocks` blocks folds a grid-stride slice (wave64
@@ -676,7 +687,11 @@ __global__ void __launch_bounds__(256) k_partial(const double* a, const double* 
 template <int OP>
 static void launch_partial(const char* name, const double* a, const double* b, const double* c, const double* d,
                            const double* e, const double* f, double* out, int blocks, int N, CUstream stream) {
-    ARGCHECK(blocks > 0 && N >= 0 && out, name);
+    const int nops = (OP == R_NORM3DIFF) ? 6 : (OP == R_NORM3) ? 3 : (OP == R_SDOT || OP == R_MAXDIFF || OP == R_MAXSUM) ? 2 : 1;
+    const double* ops[6] = {a, b, c, d, e, f};
+    bool ok = blocks > 0 && N >= 0 && out;
+    for (int i = 0; i < nops && N > 0; i++) ok = ok && ops[i];
+    ARGCHECK(ok, name);
     hipLaunchKernelGGL(k_partial<OP>, blocks, 256, 0, hs::S(stream), a, b, c, d, e, f, out, (long)N);
     HS_LAUNCH_CHECK(name);
 }

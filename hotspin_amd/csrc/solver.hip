@@ -124,6 +124,13 @@ struct StageIO {
 };
 
 // Offset the buffers of `io` to solve `sol` (layout [solve][3][N]).
+// |e| for the error maxima, with NaN mapped to +inf: fmax drops a NaN
+// operand, so without this a diverged state would report a finite error.
+__device__ __forceinline__ double err_abs(double e) {
+    const double a = fabs(e);
+    return a == a ? a : __builtin_huge_val();
+}
+
 __device__ __forceinline__ StageIO solve_io(const StageIO& io, int sol, long N) {
     StageIO r = io;
     const long b = (long)sol * 3 * N;
@@ -509,7 +516,7 @@ __device__ __forceinline__ void stage_update(const Phys& P, const StageIO& io, c
     } else if constexpr (ROLE == R_BS_4) {
         for (int c = 0; c < 3; c++) {
             double ev = (double)q.ka[c] - (h / 8.0) * (double)kk[c];
-            errloc = fmax(errloc, fabs(ev));
+            errloc = fmax(errloc, err_abs(ev));
         }
         for (int c = 0; c < 3; c++) acc[c] = q.y[c];
         nrm(acc);
@@ -528,7 +535,7 @@ __device__ __forceinline__ void stage_update(const Phys& P, const StageIO& io, c
             float k1 = q.ka[c];
             acc[c] += (0.5 * h) * (double)kk[c] + (-0.5 * h) * (double)k1;
             o.y[c] = acc[c];
-            errloc = fmax(errloc, fabs((double)kk[c] - (double)k1));
+            errloc = fmax(errloc, err_abs((double)kk[c] - (double)k1));
         }
         set_m(acc);
     } else if constexpr (ROLE == R_EULER) {
@@ -1171,12 +1178,23 @@ __global__ void k_control(Clock* clk, unsigned long long* errmax, int* live, int
     }
     const double err = errs[0];
     c.err = err;
-    if (!finite) {  // the state has diverged: stop this slot
-        c.state = HS_SLOT_FAILED;
-        c.home = par;
-        c.retry = 0;
+    if (!finite) {
+        // a non-finite estimate (the stage values overflowed or went NaN) is
+        // never accepted: retry at the smallest factor while retries and dt
+        // remain, otherwise the slot has diverged
+        const double nd = controller(c.h, errs[0] + errs[1] + errs[2], 1.0, true, 0.5, 1.0 / 3.0, c.mindt, c.maxdt);
+        if (c.attempt >= (uint32_t)kMaxTry + 1 || !(nd < c.h)) {
+            c.state = HS_SLOT_FAILED;
+            c.home = par;
+            c.retry = 0;
+        } else {
+            c.h = nd;
+            c.attempt += 1;
+            c.rejected += 1;
+            c.retry = 1;
+        }
         clk[q] = c;
-        live[q] = 0;
+        live[q] = c.state == HS_SLOT_RUNNING ? 1 : 0;
         return;
     }
     bool bad = false;
@@ -1341,8 +1359,8 @@ __global__ void __launch_bounds__(256) k_tm(Phys P, TmP Q, TmIO io) {
             io.ee[J] = h * ((-5.0 / 72.0) * k1e + (1.0 / 12.0) * k2e + (1.0 / 9.0) * ke);
             io.ep[J] = h * ((-5.0 / 72.0) * k1p + (1.0 / 12.0) * k2p + (1.0 / 9.0) * kp);
         } else if constexpr (ROLE == R_BS_4) {
-            el = fabs(io.ee[J] - (h / 8.0) * ke);
-            pl = fabs(io.ep[J] - (h / 8.0) * kp);
+            el = err_abs(io.ee[J] - (h / 8.0) * ke);
+            pl = err_abs(io.ep[J] - (h / 8.0) * kp);
         }
     }
     if constexpr (ROLE == R_BS_4) {
@@ -2067,8 +2085,9 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     s->p = *p;
     s->nsolve = nsolve;
     s->st = hs::S(stream);
-    if (!s->st) {  // own a real stream so a step can be captured into a graph
-        hipStreamCreate(&s->st);
+    if (!s->st) {  // own a real stream so a step can be captured into a graph; non-blocking, so
+                   // that no copy on the legacy null stream waits for it (or it for them)
+        hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
         s->own_stream = true;
     }
     s->N = (long)p->N[0] * p->N[1] * p->N[2];
@@ -2904,9 +2923,19 @@ extern "C" int hs_solver_run(hs_solver* s, int method, long nsteps, long step0, 
                 }
                 err = read_err(s) * dt;
                 if (!std::isfinite(err)) {
-                    hs::set_error(HS_ECONV, "hs_solver_run: RK12 error estimate is not finite");
-                    if (stats) *stats = st;
-                    return HS_ECONV;
+                    // never accepted: retry at the smallest factor while retries
+                    // remain and dt can still shrink
+                    const double nd = controller(dt, err, maxErr, true, 0.5, 1.0 / 3.0, mindt, maxdt);
+                    if (tr > maxTry || !(nd < dt)) {
+                        hs::set_error(HS_ECONV, "hs_solver_run: RK12 error estimate is not finite");
+                        *dtio = dt;
+                        if (stats) *stats = st;
+                        return HS_ECONV;
+                    }
+                    dt = nd;
+                    tr++;
+                    st.rejected++;
+                    continue;
                 }
                 bool bad = err > maxErr;
                 double nd = controller(dt, err, maxErr, bad, 0.5, 1.0 / 3.0, mindt, maxdt);
@@ -2954,20 +2983,19 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
     }
     if (!s->slot_mode) {
         // first admission: every slot idle, pinned record buffer, graphs captured lazily
-        hipStreamSynchronize(s->st);
-        std::vector<Clock> hc(s->nsolve);
-        memset(hc.data(), 0, sizeof(Clock) * s->nsolve);
-        hipMemcpy(s->clk, hc.data(), sizeof(Clock) * s->nsolve, hipMemcpyHostToDevice);
-        hipMemset(s->errmax, 0, sizeof(unsigned long long) * s->nsolve * kErrPerSolve);
         if (!ensure_devctl(s)) return HS_ENOMEM;
-        hipMemset(s->live, 0, sizeof(int) * s->nsolve);
+        hipMemsetAsync(s->clk, 0, sizeof(Clock) * s->nsolve, s->st);
+        hipMemsetAsync(s->errmax, 0, sizeof(unsigned long long) * s->nsolve * kErrPerSolve, s->st);
+        hipMemsetAsync(s->live, 0, sizeof(int) * s->nsolve, s->st);
         memset(s->slot_host, 0, sizeof(hs_slot) * s->nsolve);
         s->slot_mode = true;
         s->slot_iter = 0;
     }
+    // every copy below is ordered on the solver's own stream; waiting for it
+    // alone also makes slot_host (copied back at the end of each
+    // hs_slot_advance) the device's current slot records
     hipStreamSynchronize(s->st);
-    Clock c;
-    hipMemcpy(&c, s->clk + slot, sizeof(Clock), hipMemcpyDeviceToHost);
+    Clock c = s->slot_host[slot];
     if (c.state == HS_SLOT_RUNNING) {
         hs::set_error(HS_EINVAL, "hs_slot_admit: slot is running");
         return HS_EINVAL;
@@ -2983,11 +3011,11 @@ extern "C" int hs_slot_admit(hs_solver* s, int slot, const double* m, double TMu
     q.seed = seed;
     q.stream = stream;
     s->sp_host[slot] = q;
-    hipMemcpy(s->sp + slot, &q, sizeof(SolveP), hipMemcpyHostToDevice);
+    hipMemcpyAsync(s->sp + slot, &s->sp_host[slot], sizeof(SolveP), hipMemcpyHostToDevice, s->st);
     // state into the set that the next iteration reads, its FP32 copy and field
     const Set cur = slot_set(s, (int)(s->slot_iter & 1));
     const long off = (long)slot * 3 * s->N;
-    hipMemcpy(cur.y + off, m, sizeof(double) * 3 * s->N, hipMemcpyHostToDevice);
+    hipMemcpyAsync(cur.y + off, m, sizeof(double) * 3 * s->N, hipMemcpyHostToDevice, s->st);
     hipLaunchKernelGGL(k_set_state, dim3(hs::grid1d(s->N), 1), hs::kBlock, 0, s->st, cur.y + off, cur.m + off,
                        s->p.normalize, s->N);
     if (s->phys.demag)
@@ -3048,11 +3076,12 @@ extern "C" int hs_slot_get_m(hs_solver* s, int slot, double* m) {
         hs::set_error(HS_EINVAL, "hs_slot_get_m: bad argument");
         return HS_EINVAL;
     }
-    hipStreamSynchronize(s->st);
-    Clock c;
-    hipMemcpy(&c, s->clk + slot, sizeof(Clock), hipMemcpyDeviceToHost);
+    hipStreamSynchronize(s->st);  // slot_host is then the device's current record
+    const Clock c = s->slot_host[slot];
     const int set = c.state == HS_SLOT_RUNNING ? (int)(s->slot_iter & 1) : (int)c.home;
-    hipMemcpy(m, slot_set(s, set).y + (long)slot * 3 * s->N, sizeof(double) * 3 * s->N, hipMemcpyDeviceToHost);
+    hipMemcpyAsync(m, slot_set(s, set).y + (long)slot * 3 * s->N, sizeof(double) * 3 * s->N, hipMemcpyDeviceToHost,
+                   s->st);
+    hipStreamSynchronize(s->st);
     HS_LAUNCH_CHECK("hs_slot_get_m");
     return hs_last_error();
 }
@@ -3062,13 +3091,11 @@ extern "C" int hs_slot_release(hs_solver* s, int slot) {
         hs::set_error(HS_EINVAL, "hs_slot_release: bad argument");
         return HS_EINVAL;
     }
+    // ordered after the iterations already enqueued on the solver's stream
+    hipMemsetAsync(s->clk + slot, 0, sizeof(Clock), s->st);
+    hipMemsetAsync(s->live + slot, 0, sizeof(int), s->st);
     hipStreamSynchronize(s->st);
-    Clock c;
-    memset(&c, 0, sizeof(c));
-    hipMemcpy(s->clk + slot, &c, sizeof(Clock), hipMemcpyHostToDevice);
-    const int lv = 0;
-    hipMemcpy(s->live + slot, &lv, sizeof(int), hipMemcpyHostToDevice);
-    s->slot_host[slot] = c;
+    memset(&s->slot_host[slot], 0, sizeof(Clock));
     HS_LAUNCH_CHECK("hs_slot_release");
     return hs_last_error();
 }
@@ -3115,7 +3142,7 @@ extern "C" hs_ensemble* hs_ensemble_create(hs_solver* const* batches, int nbatch
             }
         e->b.push_back(batches[i]);
     }
-    hipStreamCreate(&e->origin);
+    hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking);
     e->ev.resize(nbatch + 1);
     for (auto& v : e->ev) hipEventCreateWithFlags(&v, hipEventDisableTiming);
     return e;

@@ -1210,6 +1210,12 @@ void or_thermal_normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t 
 }
 
 /* ------------------------------------------------------------------ LLB-r problem */
+/* Test hook: a factor on the Brown prefactor 2 kB T / (mu0 V gamma dt Ms0)
+ * of or_llb_rhs, 1 except in tests that plant a wrong amplitude on purpose
+ * to show that the fluctuation-dissipation checks catch it. */
+static double g_thermal_prefactor_scale = 1.0;
+void or_debug_set_thermal_prefactor_scale(double f) { g_thermal_prefactor_scale = f; }
+
 struct or_llb {
     or_llb_params p;
     long N;
@@ -1289,7 +1295,7 @@ void or_llb_rhs(or_llb* s, double* m, double* k, uint32_t step, uint32_t attempt
         double KB2TMul = OR_KB * 2.0 * p->TMul;
         double deltat = fmax(p->cutoff_dt, dt);
         double denom = OR_MU0 * V * p->gammaLL * deltat * p->msat0T0Mul;
-        double K = KB2TMul / denom;
+        double K = KB2TMul / denom * g_thermal_prefactor_scale;
         or_temperature_scale_noise(e, e + N, e + 2 * N, p->alpha[0], p->alpha[1], p->alpha[2],
                                    p->T, p->msat0T0, p->alphaMul[0], p->alphaMul[1], p->alphaMul[2], K, n);
         double* b[3] = {e, e + N, e + 2 * N};
@@ -1355,6 +1361,13 @@ int or_llb_step_rk4(or_llb* s, double* m, double dt, uint32_t step) {
     return 1;
 }
 
+/* |e| for the adaptive error estimates, NaN mapped to +inf so that a
+ * diverged attempt can never look converged.  A non-finite estimate is never
+ * accepted (the reference would take it when dt reaches mindt): the attempt
+ * is retried at the smallest factor while retries remain and dt can still
+ * shrink, otherwise the step fails. */
+static inline double err_abs(double e) { double a = fabs(e); return a == a ? a : INFINITY; }
+
 /* RK12Solver.Step, engine/solve_rk12.go:91-214 (single equation). */
 int or_llb_step_rk12(or_llb* s, double* m, double* dtio, double maxErr,
                      double mindt, double maxdt, uint32_t step, double* stats) {
@@ -1372,7 +1385,12 @@ int or_llb_step_rk12(or_llb* s, double* m, double* dtio, double maxErr,
         or_llb_rhs(s, m, s->k2, step, (uint32_t)tr, dt);
         double h = dt;
         or_madd2(m, s->k2, 0.5 * h, s->k1, -0.5 * h, (int)n3);
-        err = or_maxdiff(s->k2, s->k1, n3) * h;
+        err = 0;
+        for (long i = 0; i < n3; i++) {
+            double d = err_abs(s->k2[i] - s->k1[i]);
+            if (d > err) err = d;
+        }
+        err *= h;
         int bad = err > maxErr;
         double factor = bad ? pow(maxErr / err, 1. / 3.) : sqrt(maxErr / err);
         factor *= headRoom;
@@ -1382,6 +1400,11 @@ int or_llb_step_rk12(or_llb* s, double* m, double* dtio, double maxErr,
         if (newDt < mindt) newDt = mindt;
         if (newDt > maxdt) newDt = maxdt;
         *dtio = newDt;
+        if (!isfinite(err)) {
+            if (tr > maxTry || !(newDt < dt)) return -1;
+            tr++;
+            continue;
+        }
         if (!bad || newDt == mindt) break;
         if (tr > maxTry) return -1;
         tr++;
@@ -1419,7 +1442,7 @@ int or_llb_step_rk23(or_llb* s, double* m, double* dtio, double maxErr,
         for (long i = 0; i < n3; i++) {
             double e = h * ((-5.0 / 72.0) * s->k1[i] + (1.0 / 12.0) * s->k2[i] +
                             (1.0 / 9.0) * s->k3[i] - (1.0 / 8.0) * s->k4[i]);
-            if (fabs(e) > err) err = fabs(e);
+            if (err_abs(e) > err) err = err_abs(e);
         }
         int bad = err > maxErr;
         double factor = headRoom * pow(maxErr / err, 1. / 3.);
@@ -1429,6 +1452,11 @@ int or_llb_step_rk23(or_llb* s, double* m, double* dtio, double maxErr,
         if (newDt < mindt) newDt = mindt;
         if (newDt > maxdt) newDt = maxdt;
         *dtio = newDt;
+        if (!isfinite(err)) {
+            if (tr > maxTry || !(newDt < dt)) return -1;
+            tr++;
+            continue;
+        }
         if (!bad || newDt == mindt) break;
         if (tr > maxTry) return -1;
         tr++;
@@ -1654,7 +1682,7 @@ static double maxabs_err(const double* k1, const double* k2, const double* k3, c
     double e = 0;
     for (long i = 0; i < n; i++) {
         double v = h * ((-5.0 / 72.0) * k1[i] + (1.0 / 12.0) * k2[i] + (1.0 / 9.0) * k3[i] - (1.0 / 8.0) * k4[i]);
-        if (fabs(v) > e) e = fabs(v);
+        if (err_abs(v) > e) e = err_abs(v);
     }
     return e;
 }
@@ -1728,7 +1756,12 @@ int or_llbtm_step_rk23(or_llb* s, const or_tm_params* q, double* m, double* Te, 
         if (newDt < mindt) newDt = mindt;
         if (newDt > maxdt) newDt = maxdt;
         *dtio = newDt;
-        if (!bad || newDt == mindt) {
+        const int finite = isfinite(err[0]) && isfinite(err[1]) && isfinite(err[2]);
+        if (!finite && (tr > maxTry || !(newDt < dt))) {
+            rc = -1;
+            break;
+        }
+        if (finite && (!bad || newDt == mindt)) {
             memcpy(Te, Tes, sizeof(double) * N);
             memcpy(Tp, Tps, sizeof(double) * N);
             *t += h;

@@ -193,6 +193,8 @@ void or_llb_free(or_llb* s);
 /* one RHS evaluation: normalises m in place when the normalizer is loaded,
  * writes k (3N).  H_eff is kept and can be read with or_llb_heff. */
 void or_llb_rhs(or_llb* s, double* m, double* k, uint32_t step, uint32_t attempt, double dt);
+/* test hook: scales the Brown prefactor of or_llb_rhs (1 = the reference's formula) */
+void or_debug_set_thermal_prefactor_scale(double f);
 const double* or_llb_heff(const or_llb* s);
 const double* or_llb_bdemag(const or_llb* s);
 /* steppers: return accepted-step count (1) or <0 on failure */

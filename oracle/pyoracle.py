@@ -77,6 +77,7 @@ def _declare(L):
     L.or_llb_new.restype = v
     L.or_llb_free.argtypes = [v]
     L.or_llb_rhs.argtypes = [v, v, v, u32, u32, d]
+    L.or_debug_set_thermal_prefactor_scale.argtypes = [d]
     L.or_tm_rhs.argtypes = [v, v, v, v, v, v, d, v, v, v, v]
     L.or_llb_spin_heat.argtypes = [v, v, v]
     L.or_llb_energy.argtypes = [v, v, v]
@@ -118,6 +119,12 @@ def f64(a):
 
 def set_threads(n):
     lib().or_set_threads(int(n))
+
+
+def set_thermal_prefactor_scale(f):
+    """Test hook: multiply the Brown prefactor of the LLB RHS by f (1 = the
+    reference's formula); used only to plant a wrong amplitude on purpose."""
+    lib().or_debug_set_thermal_prefactor_scale(float(f))
 
 
 def get_threads():

@@ -1,0 +1,93 @@
+"""Fluctuation-dissipation pin of the GPU's Brown thermal field: the HIP
+solver (FP32 fields, in-kernel Philox4x32-10 normals, FP64 state) driven over
+32768 independent macrospin cells reaches the stationary state that the
+reference's SDE predicts in closed form (tests/fdt_util.py has the
+derivation; the oracle passes the same checks in test_oracle_fdt.py, where a
+planted factor-2 prefactor fails them):
+
+- LLG limit: <cos theta> = L(Ms0 V B / (kB T (1 + lambda^2))) at 150, 300
+  and 600 K; non-cubic cells with cutoff_dt = 2 dt;
+- LLB-r: <cos theta> = L(xi(|m|)) at 900 and 1000 K in a 1.3 T field; at
+  B = 0 the mean |m| (= the Brillouin m_e(T) up to O(lambda^2)) within 3
+  standard errors and Var(|m|) within 8 % of the stationary radial density.
+"""
+import numpy as np
+import pytest
+
+from hotspin_amd._lib import check
+from hotspin_amd.api import Solver
+from oracle import pyoracle as po
+from tests import fdt_util as F
+from tests.gpu_util import require_gpu
+
+pytestmark = pytest.mark.gpu
+N = (8, 64, 64)
+NCELL = 8 * 64 * 64
+CUBE = (2e-9, 2e-9, 2e-9)
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    require_gpu()
+    yield
+    check("fdt")
+
+
+def _sample(kw, cell, dt, nequil, nsample, m0):
+    s = Solver(N, cell, **kw)
+    m = np.zeros((3, NCELL))
+    m[0] = m0
+    s.set_m(m)
+    print("fdt: N=%s T=%g B=%g dt=%g normalize=%d" % (N, kw["TMul"], kw["Bext"][0], dt, kw["normalize"]), flush=True)
+
+    def advance(i0, n):
+        s.run("rk4", n, dt, step0=i0)
+
+    out = F.run(advance, s.get_m, NCELL, nequil, nsample)
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("T,cell,cutoff", [(150.0, CUBE, 0.0), (300.0, CUBE, 0.0), (600.0, CUBE, 0.0),
+                                           (600.0, (2e-9, 3e-9, 1.5e-9), 1e-13)])
+def test_llg_limit_langevin(T, cell, cutoff):
+    dt, B, lam = 5e-14, 1.3, 0.1
+    s = _sample(F.params(T, B, lam, cutoff_dt=cutoff), cell, dt, 4000, 4000, 1.0)
+    got, se = s.cos()
+    want = F.langevin(F.xi_transverse(T, B, cell, lam, 1.0, cutoff, dt))
+    print("fdt: <cos> %.5f +- %.5f want %.5f" % (got, se, want))
+    assert abs(got - want) <= 3 * se, (got, want, se)
+
+
+@pytest.mark.parametrize("T", [900.0, 1000.0])
+def test_llb_transverse_langevin(T):
+    B, lam = 1.3, 0.1
+    s = _sample(F.params(T, B, lam, normalize=0, longfield=1), CUBE, 2.5e-14, 8000, 8000, F.m_e_half(T))
+    got, se = s.cos()
+    r, _ = s.radius()
+    want = F.langevin(F.xi_transverse(T, B, CUBE, lam, r))
+    print("fdt: <cos> %.5f +- %.5f want %.5f (|m| %.5f)" % (got, se, want, r))
+    assert abs(got - want) <= 3 * se, (got, want, se)
+
+
+def _h_lf(T):
+    def f(r):
+        kw = dict(F.params(T, 0.0, longfield=1, normalize=0), thermal=0, zeeman=0, torque=0, local00nc=0)
+        o = po.LLB((1, 1, r.size), CUBE, **kw)
+        m = np.zeros((3, r.size))
+        m[0] = r
+        o.rhs(m)
+        return o.heff()[0]
+    return f
+
+
+@pytest.mark.parametrize("T", [900.0, 1000.0])
+def test_llb_radial_equilibrium(T):
+    lam = 0.1
+    s = _sample(F.params(T, 0.0, lam, normalize=0, longfield=1), CUBE, 2.5e-14, 4000, 4000, F.m_e_half(T))
+    r, se = s.radius()
+    var = s.radial_var()
+    want_r, want_var, me = F.radial_prediction(T, lam, CUBE, _h_lf(T))
+    print("fdt: <|m|> %.6f +- %.6f want %.6f (m_e %.6f); var %.4e want %.4e" % (r, se, want_r, me, var, want_var))
+    assert abs(r - want_r) <= 3 * se, (r, want_r, se)
+    assert abs(var / want_var - 1) <= 0.08, (var, want_var)
