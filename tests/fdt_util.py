@@ -18,6 +18,13 @@ the noise fixed over a step, which converges to that interpretation):
       xi = Ms0 V B |m| / (kB T (|m|^2 + lambda^2)),
   and <cos theta> = L(xi) = coth xi - 1/xi (Langevin).  In the LLG limit
   (normaliser, |m| = 1) this is Brown's result at temperature T (1 + lambda^2).
+  With a free |m| (LLB-r) the field also pushes r by gamma lambda B cos(theta),
+  with the radial noise lambda^2 kB T instead of (r^2 + lambda^2) kB T: the two
+  zero-flux conditions disagree, the stationary state carries a circulating
+  flux, and L(xi(<r>)) is exact only where the field barely moves r.  Near
+  Tc (large longitudinal susceptibility) a 1.3 T field biases <cos theta> by
+  about +1 % (oracle and GPU alike, 1000 K); 0.3 T keeps the bias below the
+  statistical error of 32768 cells.
 * Radial (LLB-r, B = 0): only the additive damping noise gamma lambda H_th
   moves |m| = r, so P(r) ~ r^2 exp(-E(r) / (lambda^2 kB T)) with
   E(r) = -mu0 Ms0 V int H_lf dr: <r> is the MFA equilibrium m_e(T) up to

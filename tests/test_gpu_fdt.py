@@ -7,10 +7,13 @@ planted factor-2 prefactor fails them):
 
 - LLG limit: <cos theta> = L(Ms0 V B / (kB T (1 + lambda^2))) at 150, 300
   and 600 K; non-cubic cells with cutoff_dt = 2 dt;
-- LLB-r: <cos theta> = L(xi(|m|)) at 900 and 1000 K in a 1.3 T field; at
+- LLB-r: <cos theta> = L(xi(|m|)) at 900 and 1000 K in a 0.3 T field; at
   B = 0 the mean |m| (= the Brillouin m_e(T) up to O(lambda^2)) within 3
-  standard errors and Var(|m|) within 8 % of the stationary radial density.
+  standard errors and Var(|m|) within 3 % of the stationary radial density,
+  both extrapolated to dt -> 0 from runs at dt and dt / 2.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -33,7 +36,7 @@ def _gpu():
     check("fdt")
 
 
-def _sample(kw, cell, dt, nequil, nsample, m0):
+def _sample(kw, cell, dt, nequil, nsample, m0, every=10):
     s = Solver(N, cell, **kw)
     m = np.zeros((3, NCELL))
     m[0] = m0
@@ -43,7 +46,7 @@ def _sample(kw, cell, dt, nequil, nsample, m0):
     def advance(i0, n):
         s.run("rk4", n, dt, step0=i0)
 
-    out = F.run(advance, s.get_m, NCELL, nequil, nsample)
+    out = F.run(advance, s.get_m, NCELL, nequil, nsample, every)
     s.close()
     return out
 
@@ -61,7 +64,8 @@ def test_llg_limit_langevin(T, cell, cutoff):
 
 @pytest.mark.parametrize("T", [900.0, 1000.0])
 def test_llb_transverse_langevin(T):
-    B, lam = 1.3, 0.1
+    # 0.3 T: the closed form holds where the field barely moves |m| (see tests/fdt_util.py)
+    B, lam = 0.3, 0.1
     s = _sample(F.params(T, B, lam, normalize=0, longfield=1), CUBE, 2.5e-14, 8000, 8000, F.m_e_half(T))
     got, se = s.cos()
     r, _ = s.radius()
@@ -83,11 +87,22 @@ def _h_lf(T):
 
 @pytest.mark.parametrize("T", [900.0, 1000.0])
 def test_llb_radial_equilibrium(T):
+    """The step keeps the noise fixed over dt, a scheme of weak order 1: with
+    32768 cells the standard error of <|m|> (~2e-6) resolves its O(dt) bias
+    (+1.1e-5 at dt = 2.5e-14 s and +5e-6 at half that, oracle and GPU alike
+    at 900 K).  The mean and variance are therefore compared in the dt -> 0
+    limit, 2 x(dt/2) - x(dt), over the same simulated time."""
     lam = 0.1
-    s = _sample(F.params(T, 0.0, lam, normalize=0, longfield=1), CUBE, 2.5e-14, 4000, 4000, F.m_e_half(T))
-    r, se = s.radius()
-    var = s.radial_var()
     want_r, want_var, me = F.radial_prediction(T, lam, CUBE, _h_lf(T))
-    print("fdt: <|m|> %.6f +- %.6f want %.6f (m_e %.6f); var %.4e want %.4e" % (r, se, want_r, me, var, want_var))
-    assert abs(r - want_r) <= 3 * se, (r, want_r, se)
-    assert abs(var / want_var - 1) <= 0.08, (var, want_var)
+    runs = []
+    for dt, n, every in ((2.5e-14, 4000, 10), (1.25e-14, 8000, 20)):
+        s = _sample(F.params(T, 0.0, lam, normalize=0, longfield=1), CUBE, dt, n, n, F.m_e_half(T), every)
+        r, se = s.radius()
+        runs.append((r, se, s.radial_var()))
+        print("fdt: dt %g <|m|> %.7f +- %.7f var %.4e" % (dt, r, se, runs[-1][2]))
+    (r1, se1, v1), (r2, se2, v2) = runs
+    r0, se0, v0 = 2 * r2 - r1, math.sqrt(4 * se2 * se2 + se1 * se1), 2 * v2 - v1
+    print("fdt: dt->0 <|m|> %.7f +- %.7f want %.7f (m_e %.7f); var %.4e want %.4e" % (r0, se0, want_r, me, v0,
+                                                                                      want_var))
+    assert abs(r0 - want_r) <= 3 * se0, (r0, want_r, se0)
+    assert abs(v0 / want_var - 1) <= 0.03, (v0, want_var)
