@@ -11,10 +11,16 @@ resident in HBM.
 Two more legs report batched vs plain per-solve throughput:
 - "ensemble": 512 stochastic solves on four mesh shapes stepped together
   (fixed RK4, one graph launch per step) vs the same solves run one at a
-  time through the plain solver;
-- "serving": Poisson request stream served by the continuous-batching loop
-  (p50/p99 latency, cell-steps/s) vs the plain solver per request.
-A "laser" leg runs the laser-heated LLB-r + two-temperature film (5 ps).
+  time through the plain solver (a captured graph per solve; eager beside it);
+- "serving": the 2000-request Poisson stream served by the
+  continuous-batching loop (p50/p99 latency split into queue wait and
+  service time, cell-steps/s) vs the plain solver per request.
+A "laser" leg runs the laser-heated LLB-r + two-temperature film (5 ps) and
+"laser_recipe" the reference's AM12 recipe on the same film.
+
+Every leg reports its error against the FP64 oracle on the same inputs
+("error_vs_oracle", "max_abs_dm_vs_oracle", "max_abs_err_vs_oracle"), and
+"cpu_baseline" times the oracle on the host cores (and on one thread).
 
 Prints one JSON line (see the keys below); progress goes to stderr.
 
@@ -66,11 +72,16 @@ def stage_bytes_rk23(ncell, normalize=False):
     return [x * ncell for x in b]
 
 
-def ensemble_leg(steps, warmup, plain_steps=10):
+def ensemble_leg(steps, warmup, plain_steps=10, check_steps=20):
     """512 solves on four shapes: one ensemble graph launch per RK4 step vs
-    the plain solver stepping each solve alone (eager, host loop)."""
+    the plain solver stepping each solve alone.  The plain path is timed as
+    a captured graph per solve (hs_solver_run_graph, the same launch savings
+    as the batch) and, beside it, eagerly.  Before the timed run the first
+    solve of each shape is checked against the FP64 oracle after
+    `check_steps` steps taken inside the mixed ensemble."""
     import torch
     from hotspin_amd.api import Batch, Ensemble, Solver
+    from oracle import pyoracle as po
     mat = W.MATERIAL_WIDE_T
     cfg = W.ensemble(512, seed=0)
     dt = 1e-15
@@ -82,65 +93,95 @@ def ensemble_leg(steps, warmup, plain_steps=10):
             continue
         kw = W.llbr_params(T=300.0, seed=0, material=mat)
         b = Batch(shape, W.ENSEMBLE_CELL, len(qs), TMuls=[c["T"] for c in qs], seeds=[c["seed"] for c in qs], **kw)
-        b.set_m(np.stack([W.random_domains(shape, 8, W.equilibrium_m(c["T"], mat["Tc"], mat["J"]),
-                                           np.random.default_rng(c["seed"])) for c in qs]))
-        batches.append((shape, qs, b))
+        m0 = np.stack([W.random_domains(shape, 8, W.equilibrium_m(c["T"], mat["Tc"], mat["J"]),
+                                        np.random.default_rng(c["seed"])) for c in qs])
+        b.set_m(m0)
+        batches.append((shape, qs, b, m0))
         counts.append(len(qs))
         cells += len(qs) * int(np.prod(shape))
-    e = Ensemble([b for _, _, b in batches])
+    e = Ensemble([b for _, _, b, _ in batches])
     log("ensemble: %d solves, %d cells, set-up %.1f s" % (sum(counts), cells, time.time() - t0))
-    e.run_graph(warmup, dt)
+    # FP64 oracle check of one solve per shape, taken from the mixed ensemble
+    e.run_graph(check_steps, dt, step0=0)
+    errs = {}
+    for shape, qs, b, m0 in batches:
+        c = qs[0]
+        o = po.LLB(shape, W.ENSEMBLE_CELL, **W.llbr_params(T=c["T"], seed=c["seed"], material=mat))
+        mo = m0[0].copy()
+        for i in range(check_steps):
+            o.step("rk4", mo, dt, i)
+        errs["x".join(str(n) for n in shape[::-1])] = float(np.abs(b.get_m()[0] - mo).max())
+    log("ensemble: max|dm| vs FP64 oracle after %d steps: %s" % (check_steps, errs))
+    for _, _, b, m0 in batches:
+        b.set_m(m0)
+    e.run_graph(warmup, dt, step0=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e.run_graph(steps, dt)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     batched = cells * steps / el
-    bad = sum(int(not np.isfinite(b.get_m()).all(axis=(1, 2)).all()) for _, _, b in batches)
+    bad = sum(int(not np.isfinite(b.get_m()).all(axis=(1, 2)).all()) for _, _, b, _ in batches)
     log("ensemble: %d RK4 steps in %.3f s: %.3e cell-steps/s" % (steps, el, batched))
-    # plain path: every solve alone, eager launches, host loop per step
-    plain_time = 0.0
-    for shape, qs, _ in batches:
+    # plain path: every solve alone, one captured graph per solve (and eager launches)
+    plain_time = eager_time = 0.0
+    for shape, qs, _, m0 in batches:
         c = qs[0]
         kw = W.llbr_params(T=c["T"], seed=c["seed"], material=mat)
         s = Solver(shape, W.ENSEMBLE_CELL, **kw)
-        s.set_m(W.random_domains(shape, 8, W.equilibrium_m(c["T"], mat["Tc"], mat["J"]),
-                                 np.random.default_rng(c["seed"])))
+        s.set_m(m0[0])
+        s.run_graph(2, dt, step0=0)
         s.run("rk4", 2, dt)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        s.run("rk4", plain_steps, dt)
+        s.run_graph(plain_steps, dt)
         torch.cuda.synchronize()
         per_step = (time.perf_counter() - t0) / plain_steps
+        t0 = time.perf_counter()
+        s.run("rk4", plain_steps, dt)
+        torch.cuda.synchronize()
+        per_eager = (time.perf_counter() - t0) / plain_steps
         plain_time += len(qs) * per_step * steps
-        log("ensemble: plain %s %.3f ms/step" % (shape, per_step * 1e3))
+        eager_time += len(qs) * per_eager * steps
+        log("ensemble: plain %s %.3f ms/step graph, %.3f ms/step eager" % (shape, per_step * 1e3, per_eager * 1e3))
         s.close()
     plain = cells * steps / plain_time
+    eager = cells * steps / eager_time
     e.close()
-    for _, _, b in batches:
+    for _, _, b, _ in batches:
         b.close()
     return {"workload": "512 LLB-r solves, seeds 0..511, meshes {32x32x32, 64x64x4, 128x128x2, 96x48x8} "
                         "3 nm, T~U(100,1000) K, J=7/2, RK4 dt=1e-15", "solves": sum(counts),
             "solves_per_shape": counts, "cells": cells, "steps": steps, "batched_cell_steps_per_s": batched,
-            "plain_cell_steps_per_s": plain, "ratio": batched / plain, "nonfinite_solves": bad,
-            "plain_sample": "%d eager RK4 steps of one solve per shape, scaled by the shape counts" % plain_steps}
+            "plain_cell_steps_per_s": plain, "ratio": batched / plain, "plain_eager_cell_steps_per_s": eager,
+            "ratio_vs_eager": batched / eager, "nonfinite_solves": bad,
+            "max_abs_dm_vs_oracle": errs,
+            "oracle_check": "first solve of each shape after %d RK4 steps inside the mixed ensemble vs the FP64 "
+                            "oracle (same seed, T, noise counters)" % check_steps,
+            "plain_sample": "%d RK4 steps of one solve per shape as a captured graph per solve (eager beside it), "
+                            "scaled by the shape counts" % plain_steps}
 
 
-def serving_leg(nreq, plain_steps=20):
-    """Continuous batching of a Poisson request stream vs the plain solver."""
+def serving_leg(nreq, plain_steps=20, check_steps=5):
+    """Continuous batching of a Poisson request stream vs the plain solver.
+    The first request of each shape class is also solved alone for
+    `check_steps` adaptive RK23 steps and compared with the FP64 oracle (a
+    request's result in the mix equals its solve alone bit for bit)."""
     import torch
-    from hotspin_amd.api import Solver
+    from hotspin_amd.api import SlotBatch, Solver
     from hotspin_amd.serve import Server
+    from oracle import pyoracle as po
     reqs = W.requests(n=nreq, rate=200.0, seed=0)
     srv = Server(keep_results=False)
     for r in reqs:  # the client sends the initial state with the request
         r["m0"] = srv._initial_state(r)
     srv.run(reqs, log=log)
     summ = srv.summary()
-    log("serving: %d requests in %.2f s: %.3e cell-steps/s, p50 %.2f s p99 %.2f s" % (
-        summ["requests"], summ["wall_s"], summ["cell_steps_per_s"], summ["p50_latency_s"], summ["p99_latency_s"]))
+    log("serving: %d requests in %.2f s: %.3e cell-steps/s, p50 %.2f s p99 %.2f s (service p50 %.2f s p99 %.2f s)" % (
+        summ["requests"], summ["wall_s"], summ["cell_steps_per_s"], summ["p50_latency_s"], summ["p99_latency_s"],
+        summ["p50_service_s"], summ["p99_service_s"]))
     # plain: each request alone through hs_solver_run RK23 (host-side control); one sample request per class
-    rate = {}
+    rate, errs = {}, {}
     for cls, shape in enumerate(W.SERVE_SHAPES):
         rs = [r for r in reqs if r["cls"] == cls]
         if not rs:
@@ -148,7 +189,7 @@ def serving_leg(nreq, plain_steps=20):
         r = rs[0]
         kw = W.llbr_params(T=r["T"], seed=r["seed"], material=W.MATERIAL_WIDE_T, Bext=r["Bext"])
         s = Solver(shape, W.SERVE_CELL, **kw)
-        s.set_m(srv._initial_state(r))
+        s.set_m(r["m0"])
         dt, _ = s.run("rk23", 3, 1e-15, maxErr=srv.tol, mindt=srv.mindt, maxdt=srv.maxdt)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -156,20 +197,44 @@ def serving_leg(nreq, plain_steps=20):
         torch.cuda.synchronize()
         rate[cls] = (time.perf_counter() - t0) / max(st.steps, 1)
         s.close()
+        # FP64 oracle against the request solved alone in a slot
+        b = SlotBatch(shape, W.SERVE_CELL, 1, **srv.kw)
+        b.admit(0, r["m0"], r["T"], r["seed"], check_steps, srv.dt0, maxErr=srv.tol, mindt=srv.mindt,
+                maxdt=srv.maxdt, Bext=r["Bext"])
+        b.run_to_end(chunk=check_steps)
+        mg = b.get_m(0)
+        b.close()
+        o = po.LLB(shape, W.SERVE_CELL, **kw)
+        mo, dto = r["m0"].copy(), srv.dt0
+        for i in range(check_steps):
+            dto, _ = o.step("rk23", mo, dto, i, maxErr=srv.tol, mindt=srv.mindt, maxdt=srv.maxdt)
+        errs["x".join(str(n) for n in shape[::-1])] = float(np.abs(mg - mo).max())
+    log("serving: max|dm| vs FP64 oracle after %d RK23 steps: %s" % (check_steps, errs))
     srv.close()
     plain_time = sum(r["steps"] * rate[r["cls"]] for r in reqs)
     plain = summ["cell_steps"] / plain_time
     summ.update({"workload": "%d of 2000 Poisson requests at 200 req/s, cells log-uniform 16^3..256x256x4, "
                              "T~U(0,1.2Tc) (>=0.1Tc), |B_ext|<=100 mT, 200-5000 RK23 steps tol 1e-5, J=7/2" % nreq,
                  "plain_cell_steps_per_s": plain, "ratio": summ["cell_steps_per_s"] / plain,
+                 "max_abs_dm_vs_oracle": errs,
+                 "oracle_check": "first request of each shape class solved alone in a slot for %d adaptive RK23 "
+                                 "steps vs the FP64 oracle" % check_steps,
+                 "latency_note": "latency = done - arrival; the stream offers ~3e11 cell-steps in 10 s, more than "
+                                 "the device serves in that time, so latency is dominated by the queue wait; "
+                                 "service = done - admission",
                  "plain_sample": "%d RK23 steps of one request per shape class through hs_solver_run" % plain_steps})
     return summ
 
 
-def laser_leg(chunk=50):
-    """Laser-heated LLB-r + two-temperature model, 256x256x8, 5 ps of RK23."""
+def laser_leg(chunk=50, fine_chunk=5, check_steps=3):
+    """Laser-heated LLB-r + two-temperature model, 256x256x8, 5 ps of RK23.
+    Te and Tp are read back every `fine_chunk` steps until 1 ps (the pulse
+    peaks at 0.3 ps) and every `chunk` steps after it.  Before the timed run
+    the GPU's first `check_steps` coupled RK23 steps are compared with the
+    FP64 oracle's."""
     import torch
     from hotspin_amd.api import TMSolver
+    from oracle import pyoracle as po
     w, kw, tm, m0 = W.laser()
     ncell = int(np.prod(w["N"]))
     t0 = time.time()
@@ -177,10 +242,25 @@ def laser_leg(chunk=50):
     s.set_m(m0)
     torch.cuda.synchronize()
     setup = time.time() - t0
+    # FP64 oracle check of the first steps (each side takes its own adaptive steps)
+    o = po.LLB(w["N"], w["cell"], **kw)
+    q = po.tm_params(**tm)
+    mo, Te, Tp, to, dto = m0.copy(), np.full(ncell, w["T0"]), np.full(ncell, w["T0"]), 0.0, 1e-15
+    for i in range(check_steps):
+        dto, to, _ = o.step_tm(q, mo, Te, Tp, to, dto, i, maxErr=w["tol"], mindt=1e-20, maxdt=5e-14)
+    s.run("rk23", check_steps, 1e-15, maxErr=w["tol"], mindt=1e-20, maxdt=5e-14, step0=0)
+    teg, tpg = s.get_T()
+    err = {"m": float(np.abs(s.get_m() - mo).max()), "Te_K": float(np.abs(teg - Te).max()),
+           "Tp_K": float(np.abs(tpg - Tp).max()), "t_rel": abs(s.time / to - 1)}
+    log("laser: vs FP64 oracle after %d steps: %s" % (check_steps, err))
+    s.set_m(m0)
+    s.set_T(np.full(ncell, w["T0"]), np.full(ncell, w["T0"]))
+    s.time = 0.0
     dt, steps, rejected, peak_te, peak_tp = 1e-15, 0, 0, 0.0, 0.0
     t0 = time.perf_counter()
     while s.time < w["span"]:
-        dt, st = s.run("rk23", chunk, dt, maxErr=w["tol"], mindt=1e-20, maxdt=5e-14)
+        n = fine_chunk if s.time < 1e-12 else chunk
+        dt, st = s.run("rk23", n, dt, maxErr=w["tol"], mindt=1e-20, maxdt=5e-14, step0=steps)
         steps += st.steps
         rejected += st.rejected
         te, tp = s.get_T()
@@ -192,8 +272,10 @@ def laser_leg(chunk=50):
     return {"workload": w["name"], "cells": ncell, "steps": steps, "rejected": rejected,
             "cell_steps_per_s": ncell * steps / el, "wall_s": el, "setup_s": setup,
             "peak_Te_K": peak_te, "target_peak_Te_K": w["Tmax"], "peak_Tp_K": peak_tp,
-            "note": "peak temperatures sampled every %d steps (the host reads Te, Tp between runs)" % chunk}
-
+            "max_abs_err_vs_oracle": err,
+            "oracle_check": "first %d adaptive coupled RK23 steps of (m, Te, Tp) vs the FP64 oracle" % check_steps,
+            "note": "wall time includes reading Te, Tp back every %d steps to 1 ps and every %d after" % (
+                fine_chunk, chunk)}
 
 
 def laser_recipe_leg(nsteps=200):
@@ -253,8 +335,20 @@ def measured_traffic(kernel):
     return table.get(kernel)
 
 
-def cpu_baseline(w, kw, m, threads, nsteps=1):
-    """FP64 oracle on the host cores: `nsteps` RK23 steps of the same workload."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, kw, m, threads, nsteps=5, single_steps=1):
+    """FP64 oracle on the host cores: `nsteps` RK23 steps of the same workload
+    on `threads` OpenMP threads, then `single_steps` more on one thread.
+    Returns (record, final oracle state after the first nsteps, dt history)."""
     from oracle import pyoracle as po
     po.set_threads(threads)
     t0 = time.time()
@@ -266,10 +360,26 @@ def cpu_baseline(w, kw, m, threads, nsteps=1):
     for i in range(nsteps):
         dt, _ = o.step("rk23", mo, dt, i, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12)
     el = time.time() - t0
+    m_end, dt_end = mo.copy(), dt
+    po.set_threads(1)
+    t1 = time.time()
+    for i in range(nsteps, nsteps + single_steps):
+        dt, _ = o.step("rk23", mo, dt, i, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12)
+    el1 = time.time() - t1
+    po.set_threads(threads)
     ncell = int(np.prod(w["N"]))
-    return {"value": ncell * nsteps / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d RK23 step(s) of the same 512x512x16 workload on the FP64 oracle (%.1f s; "
-                      "kernel set-up %.1f s excluded)" % (nsteps, el, setup)}
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    rec = {"value": ncell * nsteps / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+           "sample": "%d RK23 step(s) of the same 512x512x16 workload on the FP64 oracle (%.1f s; "
+                     "kernel set-up %.1f s excluded)" % (nsteps, el, setup),
+           "single_thread": {"value": ncell * single_steps / el1, "unit": "cell-steps/s", "cores": 1,
+                             "sample": "%d further RK23 step(s) on one thread (%.1f s)" % (single_steps, el1)},
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": affinity,
+           "threads_note": "threads = the CPUs this process may use, capped at 16 (the host share of one GPU)"}
+    return rec, m_end, dt_end
 
 
 def copy_calibration():
@@ -297,7 +407,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--ens-steps", type=int, default=1000)
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-laser", action="store_true")
@@ -432,6 +542,21 @@ def main():
                      "note": "wall time per hs_solver_energy call incl. the 40-byte copy back; the demag "
                              "field of the state is cached after the first call"}
     log("bench: energy %.4f ms per call, E_total %.4e J" % (e_ms, E[4]))
+    if not args.no_cpu:
+        try:
+            threads = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            threads = min(16, os.cpu_count() or 1)
+        log("bench: cpu baseline (%d threads, then 1 thread)..." % threads)
+        out["cpu_baseline"], m_or, dt_or = cpu_baseline(w, kw, m, threads, args.cpu_steps)
+        # the same steps on the GPU from the same FP64 state, against the oracle's result
+        s.set_m(m)
+        dtg, stg = s.run("rk23", args.cpu_steps, 1e-15, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12, step0=0)
+        out["error_vs_oracle"] = {"max_abs_dm": float(np.abs(s.get_m() - m_or).max()), "steps": stg.steps,
+                                  "dt_rel": abs(dtg / dt_or - 1), "bar": 1e-5,
+                                  "note": "%d adaptive RK23 steps of the 512x512x16 workload from the same FP64 "
+                                          "state, GPU (FP32 fields) vs the FP64 oracle" % stg.steps}
+        log("bench: vs FP64 oracle after %d steps: max|dm| %.3e" % (stg.steps, out["error_vs_oracle"]["max_abs_dm"]))
     s.close()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_steps, 5)
@@ -440,10 +565,6 @@ def main():
     if not args.no_laser:
         out["laser"] = laser_leg()
         out["laser_recipe"] = laser_recipe_leg()
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log("bench: cpu baseline (%d threads)..." % threads)
-        out["cpu_baseline"] = cpu_baseline(w, kw, m, threads, args.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -161,12 +161,19 @@ class Server:
         return self.results
 
     def summary(self):
+        """Throughput and latency percentiles: latency = done - arrival (what a
+        client sees), split into the queue wait (admit - arrival) and the
+        service time (done - admit, the solve on the device beside the others)."""
         recs = list(self.results.values())
         lat = np.array([r["latency"] for r in recs])
+        wait = np.array([r["admit"] - r["arrive"] for r in recs])
+        serv = np.array([r["done"] - r["admit"] for r in recs])
         work = sum(r["cells"] * r["steps"] for r in recs)
         return dict(requests=len(recs), failed=int(sum(r["failed"] for r in recs)), wall_s=self.wall,
                     cell_steps=float(work), cell_steps_per_s=work / self.wall,
                     p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                    p50_queue_s=float(np.percentile(wait, 50)), p99_queue_s=float(np.percentile(wait, 99)),
+                    p50_service_s=float(np.percentile(serv, 50)), p99_service_s=float(np.percentile(serv, 99)),
                     batches=len(self.groups), hbm_bytes=self.used)
 
     def close(self):

@@ -24,6 +24,7 @@ N = (Z, Y, X).
   no finite stable dt); B_ext components ~ U(-100, 100) mT; 200..5000 RK23
   steps at tolerance 1e-5.
 """
+import functools
 import math
 
 import numpy as np
@@ -123,22 +124,67 @@ def single():
     return w, llbr_params(T=w["T"], seed=w["seed"]), m, me
 
 
+def two_temperature_peak(Q0, gam, Cl, G, T0, sig_t, t0, h=0.25e-15):
+    """Peak Te of the spot centre: dTe/dt = (Q(t) - G (Te - Tp)) / (gam Te),
+    dTp/dt = G (Te - Tp) / Cl, Q(t) = Q0 exp(-(t - t0)^2 / (2 sig_t^2)), from
+    Te = Tp = T0 at t = 0, classical RK4 at step h up to t0 + 8 sig_t.  Lateral
+    diffusion over that time is a few nm against the 200 nm spot, so the
+    centre cell follows this ODE."""
+    def f(t, te, tp):
+        q = Q0 * math.exp(-(t - t0) ** 2 / (2.0 * sig_t * sig_t))
+        x = G * (te - tp)
+        return (q - x) / (gam * te), x / Cl
+
+    te = tp = T0
+    peak = T0
+    n = int(math.ceil((t0 + 8 * sig_t) / h))
+    for i in range(n):
+        t = i * h
+        a1, b1 = f(t, te, tp)
+        a2, b2 = f(t + h / 2, te + h / 2 * a1, tp + h / 2 * b1)
+        a3, b3 = f(t + h / 2, te + h / 2 * a2, tp + h / 2 * b2)
+        a4, b4 = f(t + h, te + h * a3, tp + h * b3)
+        te += h / 6 * (a1 + 2 * a2 + 2 * a3 + a4)
+        tp += h / 6 * (b1 + 2 * b2 + 2 * b3 + b4)
+        peak = max(peak, te)
+    return peak
+
+
+@functools.lru_cache(maxsize=None)
+def pump_q0(gam, Cl, G, T0, Tmax, sig_t, t0):
+    """Pump amplitude Q0 (W/m^3) whose pulse drives the spot centre's electron
+    temperature to a peak of Tmax, electron-phonon losses during the pulse
+    included (bisection on two_temperature_peak, which grows with Q0).  The
+    lossless estimate gam (Tmax^2 - T0^2) / 2 = Q0 sig_t sqrt(2 pi) brackets
+    it from below."""
+    lo = gam * (Tmax * Tmax - T0 * T0) / (2.0 * sig_t * math.sqrt(2.0 * math.pi))
+    hi = 2.0 * lo
+    while two_temperature_peak(hi, gam, Cl, G, T0, sig_t, t0) < Tmax:
+        hi *= 2.0
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if two_temperature_peak(mid, gam, Cl, G, T0, sig_t, t0) < Tmax:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
 def laser(Tc=MATERIAL["Tc"]):
     """Laser-heated LLB-r + two-temperature model on a 256x256x8 film of
     4x4x2 nm cells: Ce = gamma Te (gamma = 6e3 J/m^3K^2), Cl = 3.8e6 J/m^3K,
     G = 1.8e18 W/m^3K, kappa = 90 W/mK, Gaussian pump (sigma_r = 200 nm at the
     film centre, sigma_t = 100 fs, centred at t0 = 300 fs), T0 = 300 K.  Q0 is
-    set so that the absorbed energy at the spot centre, gamma (Tmax^2 - T0^2)/2
-    with Tmax = 1.3 Tc, equals Q0 sigma_t sqrt(2 pi) (the pulse is much shorter
-    than the electron-phonon time gamma Te / G ~ 3 ps).  Span 5 ps, RK23 with
-    bounds 1e-5 (m) and 1e-3 K (Te, Tp).  Returns (w, kw, tm, m0)."""
+    set (`pump_q0`) so that the electron temperature at the spot centre peaks
+    at Tmax = 1.3 Tc.  Span 5 ps, RK23 with bounds 1e-5 (m) and 1e-3 K (Te,
+    Tp).  Returns (w, kw, tm, m0)."""
     N = (8, 256, 256)
     cell = (2e-9, 4e-9, 4e-9)
     T0, gam = 300.0, 6e3
     Tmax = 1.3 * Tc
     sig_t = 100e-15
-    tm = dict(gammaE=gam, Cl=3.8e6, G=1.8e18, kappa=90.0,
-              Q0=gam * (Tmax * Tmax - T0 * T0) / (2.0 * sig_t * math.sqrt(2.0 * math.pi)),
+    Cl, G = 3.8e6, 1.8e18
+    tm = dict(gammaE=gam, Cl=Cl, G=G, kappa=90.0, Q0=pump_q0(gam, Cl, G, T0, Tmax, sig_t, 3 * sig_t),
               sigma_r=200e-9, sigma_t=sig_t, t0=3 * sig_t, xc=N[2] * cell[2] / 2, yc=N[1] * cell[1] / 2,
               maxErrT=1e-3)
     me = equilibrium_m(T0, MATERIAL["Tc"], MATERIAL["J"])
