@@ -440,7 +440,7 @@ def main():
     tiles = (C.c_int * 6)()
     _lib.lib().hs_demag_tiles(_lib.lib().hs_solver_demag(s._h), tiles)
     log("bench: demag tiles (W,NT) rows %s cols %s z %s" % (tuple(tiles[0:2]), tuple(tiles[2:4]), tuple(tiles[4:6])))
-    stage_variant = {0: "march", 1: "flat", 2: "pair"}.get(_lib.lib().hs_solver_stage_variant(s._h), "?")
+    stage_variant = {0: "march", 1: "flat", 2: "pair", 3: "flatz"}.get(_lib.lib().hs_solver_stage_variant(s._h), "?")
     log("bench: stage kernel variant %s" % stage_variant)
     dt = 1e-15
     dt, st = s.run("rk23", args.warmup, dt, maxErr=w["tol"], mindt=1e-18, maxdt=1e-12)

@@ -35,6 +35,7 @@
 
 #include "demag_kernels.hpp"
 #include "demag_tune.hpp"
+#include "fft_any.hpp"
 
 hsdm::Variants& hsdm::variants(int L) {
     static std::map<int, hsdm::Variants> reg;
@@ -332,6 +333,13 @@ struct hs_demag {
     hsdm::ZV z32{};
     bool tuned = false;
     int cap = 1;  // solves the X1/X2 buffers hold
+    // passes without a compiled length family run the run-time-length kernels
+    // of fft_any.hpp: rows (R2C/C2R, FFT of gR.L = P2/2, or P2 when odd),
+    // columns (P1), axis 0 (P0); FP32 and FP64 decided separately
+    bool genR = false, gen1 = false, gen0 = false, genRd = false, gen1d = false, gen0d = false;
+    GenFFT gR{}, g1{}, g0{};
+    cpx<float>* Xzf = nullptr;   // (cap, 3, P0, P1, H2p): the axis-0 spectrum of a run-time-length F3K
+    cpx<double>* Xzd = nullptr;  // the same for the FP64 field
 };
 
 namespace {
@@ -343,11 +351,29 @@ struct Pipe {
     zconv_t<T, T> zc;
 };
 
-bool sizes_ok(const hs_demag* d) {
-    return (d->P[2] % 2 == 0) && fft_supported(d->M) && pick_r2c<float, float>(d->M) &&
-           pick_cols<float, -1>(d->P[1]) && pick_zconv<float, float>(d->P[0]) &&
-           !hsdm::variants(d->M).rows.empty() && !hsdm::variants(d->P[1]).cols.empty() &&
-           !hsdm::variants(d->P[0]).z.empty();
+// Which passes have a compiled length family (FP32 with registered tile
+// variants; FP64 with a Stockham instantiation) and which take the
+// run-time-length kernels.  False when a generic length exceeds their LDS
+// limit (the FP64 kernel set-up transforms every axis).
+bool sizes_ok(hs_demag* d) {
+    const int M = d->M, P1 = d->P[1], P0 = d->P[0];
+    const bool even = d->P[2] % 2 == 0;
+    d->genR = !(even && fft_supported(M) && pick_r2c<float, float>(M) && !hsdm::variants(M).rows.empty());
+    d->gen1 = !(pick_cols<float, -1>(P1) && !hsdm::variants(P1).cols.empty());
+    d->gen0 = !(pick_zconv<float, float>(P0) && !hsdm::variants(P0).z.empty());
+    d->genRd = !(even && fft_supported(M) && pick_r2c<double, double>(M) && pick_c2r<double, double>(M));
+    d->gen1d = !pick_cols<double, -1>(P1);
+    d->gen0d = !pick_zconv<double, double>(P0);
+    d->gR = gen_fft_plan(even ? M : d->P[2]);
+    d->g1 = gen_fft_plan(P1);
+    d->g0 = gen_fft_plan(P0);
+    const int lim = gen_max_len<double>();
+    return d->gR.L <= lim && P1 <= lim && P0 <= lim;
+}
+// FFT table of the row transform: P2/2 points (even P2) or P2 (odd)
+template <typename T>
+const cpx<T>* row_table(const hs_demag* d, const cpx<T>* twM, const cpx<T>* twP2) {
+    return (d->P[2] % 2 == 0) ? twM : twP2;
 }
 
 // live: per-solve flags of a batch (null = all solves); blocks of solves
@@ -364,21 +390,39 @@ int run_field32(hs_demag* d, const float* m, const float* msk, double srcMul, fl
     const long nrows = 3L * N0 * N1 * nsolve;
     Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
     Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
+    const int P0 = d->P[0], P2 = d->P[2];
+    const cpx<float>* twR = row_table<float>(d, d->twMf, d->twP2f);
     hs::prof_begin(HS_PROF_F1, st);
     const Live lrows{live, 3L * N0 * N1}, lpen{live, 3L * N0}, lsol{live, 1};
-    d->row32.r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, d->twMf, d->twP2f, st, lrows);
+    if (d->genR)
+        gen::r2c<float>(m, msk, (long)N0 * N1, X1, nrows, N2, P2, H2p, d->gR, twR, d->twP2f, st, lrows);
+    else
+        d->row32.r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, d->twMf, d->twP2f, st, lrows);
     hs::prof_end(HS_PROF_F1, st);
     hs::prof_begin(HS_PROF_F2, st);
-    d->col32.fwd(X1, X2, g1, g2, 3 * N0 * nsolve, N1, P1, H2, d->tw1f, st, lpen);
+    if (d->gen1)
+        gen::cols<float, -1>(X1, X2, g1, g2, 3 * N0 * nsolve, N1, P1, H2, d->g1, d->tw1f, st, lpen);
+    else
+        d->col32.fwd(X1, X2, g1, g2, 3 * N0 * nsolve, N1, P1, H2, d->tw1f, st, lpen);
     hs::prof_end(HS_PROF_F2, st);
     hs::prof_begin(HS_PROF_F3K, st);
-    d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], d->KP, (float)srcMul, d->tw0f, st, nsolve, lsol);
+    if (d->gen0)
+        gen::zconv<float, float>(X2, d->Xzf, d->kern32, N0, P0, P1, H2, H2p, d->K[1], d->KP, (float)srcMul, d->g0,
+                                 d->tw0f, st, nsolve, lsol);
+    else
+        d->z32.fn(X2, d->kern32, N0, P1, H2, H2p, d->K[1], d->KP, (float)srcMul, d->tw0f, st, nsolve, lsol);
     hs::prof_end(HS_PROF_F3K, st);
     hs::prof_begin(HS_PROF_I2, st);
-    d->col32.inv(X2, X1, g2, g1, 3 * N0 * nsolve, P1, N1, H2, d->tw1f, st, lpen);
+    if (d->gen1)
+        gen::cols<float, +1>(X2, X1, g2, g1, 3 * N0 * nsolve, P1, N1, H2, d->g1, d->tw1f, st, lpen);
+    else
+        d->col32.inv(X2, X1, g2, g1, 3 * N0 * nsolve, P1, N1, H2, d->tw1f, st, lpen);
     hs::prof_end(HS_PROF_I2, st);
     hs::prof_begin(HS_PROF_I1, st);
-    d->rowi32.c2r(X1, B, nrows, N2, H2p, d->twMf, d->twP2f, st, lrows);
+    if (d->genR)
+        gen::c2r<float>(X1, B, nrows, N2, P2, H2p, d->gR, twR, d->twP2f, st, lrows);
+    else
+        d->rowi32.c2r(X1, B, nrows, N2, H2p, d->twMf, d->twP2f, st, lrows);
     hs::prof_end(HS_PROF_I1, st);
     HS_LAUNCH_CHECK("hs_demag_field");
     return hs_last_error();
@@ -406,6 +450,41 @@ std::map<TuneKey, std::array<int, 4>>& tune_cache() {
     return c;
 }
 
+// Tuned choices persisted across processes: HS_TUNE_FILE names a text file
+// of lines "N0 N1 N2 P0 P1 P2 nr nc nz r c z ri" (mesh, padded size, the
+// numbers of registered row/column/axis-0 variants, the picks).  A plan whose
+// shape has a line with the same variant counts takes that choice without
+// timing; a newly tuned shape is appended.  The variants of a pass differ
+// in operation order (FP32 rounding), so processes that share the file give
+// the same bits for the same inputs.
+bool tune_file_read(const hs_demag* d, const std::array<int, 3>& counts, std::array<int, 4>& pick) {
+    const char* path = getenv("HS_TUNE_FILE");
+    if (!path || !*path) return false;
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    int v[13];
+    bool found = false;
+    while (!found && fscanf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                            &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12]) == 13) {
+        found = v[0] == d->N[0] && v[1] == d->N[1] && v[2] == d->N[2] && v[3] == d->P[0] && v[4] == d->P[1] &&
+                v[5] == d->P[2] && v[6] == counts[0] && v[7] == counts[1] && v[8] == counts[2] && v[9] >= 0 &&
+                v[9] < counts[0] && v[10] >= 0 && v[10] < counts[1] && v[11] >= 0 && v[11] < counts[2] &&
+                v[12] >= 0 && v[12] < counts[0];
+        if (found) pick = {v[9], v[10], v[11], v[12]};
+    }
+    fclose(f);
+    return found;
+}
+void tune_file_append(const hs_demag* d, const std::array<int, 3>& counts, const std::array<int, 4>& pick) {
+    const char* path = getenv("HS_TUNE_FILE");
+    if (!path || !*path) return;
+    if (FILE* f = fopen(path, "a")) {
+        fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d\n", d->N[0], d->N[1], d->N[2], d->P[0], d->P[1], d->P[2],
+                counts[0], counts[1], counts[2], pick[0], pick[1], pick[2], pick[3]);
+        fclose(f);
+    }
+}
+
 void apply_force(hs_demag* d, hsdm::Variants& vr, hsdm::Variants& vc, hsdm::Variants& vz) {
     // HS_DEMAG_FORCE="r,c,z": variant indices to use instead of the timed
     // choice (-1 keeps it), for tests that cover every registered variant
@@ -423,14 +502,19 @@ void tune32(hs_demag* d, hipStream_t st) {
     hsdm::Variants& vr = hsdm::variants(d->M);
     hsdm::Variants& vc = hsdm::variants(d->P[1]);
     hsdm::Variants& vz = hsdm::variants(d->P[0]);
-    d->row32 = d->rowi32 = vr.rows.empty() ? hsdm::RowV{} : vr.rows[0];
-    d->col32 = vc.cols.empty() ? hsdm::ColV{} : vc.cols[0];
-    d->z32 = vz.z.empty() ? hsdm::ZV{} : vz.z[0];
+    d->row32 = d->rowi32 = (d->genR || vr.rows.empty()) ? hsdm::RowV{} : vr.rows[0];
+    d->col32 = (d->gen1 || vc.cols.empty()) ? hsdm::ColV{} : vc.cols[0];
+    d->z32 = (d->gen0 || vz.z.empty()) ? hsdm::ZV{} : vz.z[0];
+    // a plan with a run-time-length pass keeps the first variant of its
+    // compiled passes (no timing)
     if (!d->row32.r2c || !d->col32.fwd || !d->z32.fn) return;
     const TuneKey key{{d->N[0], d->N[1], d->N[2]}, {d->P[0], d->P[1], d->P[2]}};
+    const std::array<int, 3> counts{(int)vr.rows.size(), (int)vc.cols.size(), (int)vz.z.size()};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = tune_cache().find(key);
+        std::array<int, 4> filed;
+        if (it == tune_cache().end() && tune_file_read(d, counts, filed)) it = tune_cache().emplace(key, filed).first;
         if (it != tune_cache().end()) {
             d->row32 = vr.rows[it->second[0]];
             d->rowi32 = vr.rows[it->second[3]];
@@ -528,6 +612,7 @@ void tune32(hs_demag* d, hipStream_t st) {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         tune_cache()[key] = pick;
+        tune_file_append(d, counts, pick);
     }
     apply_force(d, vr, vc, vz);
     hipEventDestroy(e0);
@@ -540,40 +625,58 @@ void tune32(hs_demag* d, hipStream_t st) {
 
 template <typename T>
 int run_field(hs_demag* d, const T* m, const T* msk, double srcMul, T* B, hipStream_t st, cpx<T>* X1, cpx<T>* X2,
-              const T* kern, const cpx<T>* twM, const cpx<T>* twP2, const cpx<T>* tw1, const cpx<T>* tw0) {
-    const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1], P0 = d->P[0];
+              cpx<T>* Xz, const T* kern, const cpx<T>* twM, const cpx<T>* twP2, const cpx<T>* tw1,
+              const cpx<T>* tw0) {
+    const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1], P0 = d->P[0], P2 = d->P[2];
     const int H2 = d->H2, H2p = d->H2p, M = d->M;
-    auto r2c = pick_r2c<T, T>(M);
-    auto c2r = pick_c2r<T, T>(M);
-    auto f1 = pick_cols<T, -1>(P1);
-    auto i1 = pick_cols<T, +1>(P1);
-    auto zc = pick_zconv<T, T>(P0);
-    if (!r2c || !c2r || !f1 || !i1 || !zc) {
+    auto r2c = d->genRd ? nullptr : pick_r2c<T, T>(M);
+    auto c2r = d->genRd ? nullptr : pick_c2r<T, T>(M);
+    auto f1 = d->gen1d ? nullptr : pick_cols<T, -1>(P1);
+    auto i1 = d->gen1d ? nullptr : pick_cols<T, +1>(P1);
+    auto zc = d->gen0d ? nullptr : pick_zconv<T, T>(P0);
+    if ((!d->genRd && (!r2c || !c2r)) || (!d->gen1d && (!f1 || !i1)) || (!d->gen0d && !zc) || (d->gen0d && !Xz)) {
         hs::set_error(HS_ESIZE, "demag: unsupported FFT length");
         return HS_ESIZE;
     }
     const long nrows = 3L * N0 * N1;
+    const Live all{nullptr, 1};
+    const cpx<T>* twR = row_table<T>(d, twM, twP2);
     // F1: m .* mask, rows of N2 -> X1 (3, N0, N1, H2p); the mask repeats every N0*N1 rows
     hs::prof_begin(HS_PROF_F1, st);
-    r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, twM, twP2, st);
+    if (d->genRd)
+        gen::r2c<T>(m, msk, (long)N0 * N1, X1, nrows, N2, P2, H2p, d->gR, twR, twP2, st, all);
+    else
+        r2c(m, msk, (long)N0 * N1, X1, nrows, N2, H2p, twM, twP2, st);
     hs::prof_end(HS_PROF_F1, st);
     // F2: pencils a = (c, i), rows j < N1 of X1 -> P1 rows of X2
     Geo g1 = {(long)N1 * H2p, 0, H2p, 1};
     Geo g2 = {(long)P1 * H2p, 0, H2p, 1};
     hs::prof_begin(HS_PROF_F2, st);
-    f1(X1, X2, g1, g2, 3 * N0, N1, P1, H2, tw1, st);
+    if (d->gen1d)
+        gen::cols<T, -1>(X1, X2, g1, g2, 3 * N0, N1, P1, H2, d->g1, tw1, st, all);
+    else
+        f1(X1, X2, g1, g2, 3 * N0, N1, P1, H2, tw1, st);
     hs::prof_end(HS_PROF_F2, st);
     // F3K: axis 0, kernel product, inverse axis 0
     hs::prof_begin(HS_PROF_F3K, st);
-    zc(X2, kern, N0, P1, H2, H2p, d->K[1], (T)srcMul, tw0, st);
+    if (d->gen0d)
+        gen::zconv<T, T>(X2, Xz, kern, N0, P0, P1, H2, H2p, d->K[1], H2, (T)srcMul, d->g0, tw0, st, 1, all);
+    else
+        zc(X2, kern, N0, P1, H2, H2p, d->K[1], (T)srcMul, tw0, st);
     hs::prof_end(HS_PROF_F3K, st);
     // I2: P1 rows of X2 -> rows j < N1 of X1
     hs::prof_begin(HS_PROF_I2, st);
-    i1(X2, X1, g2, g1, 3 * N0, P1, N1, H2, tw1, st);
+    if (d->gen1d)
+        gen::cols<T, +1>(X2, X1, g2, g1, 3 * N0, P1, N1, H2, d->g1, tw1, st, all);
+    else
+        i1(X2, X1, g2, g1, 3 * N0, P1, N1, H2, tw1, st);
     hs::prof_end(HS_PROF_I2, st);
     // I1: C2R rows -> B
     hs::prof_begin(HS_PROF_I1, st);
-    c2r(X1, B, nrows, N2, H2p, twM, twP2, st);
+    if (d->genRd)
+        gen::c2r<T>(X1, B, nrows, N2, P2, H2p, d->gR, twR, twP2, st, all);
+    else
+        c2r(X1, B, nrows, N2, H2p, twM, twP2, st);
     hs::prof_end(HS_PROF_I1, st);
     HS_LAUNCH_CHECK("hs_demag_field");
     return hs_last_error();
@@ -604,7 +707,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     d->K[2] = d->H2;
     d->ncell = (long)N[0] * N[1] * N[2];
     if (!sizes_ok(d)) {
-        hs::set_error(HS_ESIZE, "hs_demag_create: padded lengths must be 2^k, 3*2^k or 5*2^k (even along x)");
+        hs::set_error(HS_ESIZE, "hs_demag_create: a padded length (or half the padded x length) exceeds 4095");
         delete d;
         return nullptr;
     }
@@ -625,6 +728,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     if (d->kern32) hipMemsetAsync(d->kern32, 0, sizeof(float) * 6 * nk32, st);
     hipMalloc(&d->X1f, sizeof(cpx<float>) * 3L * N[0] * N[1] * H2p);
     hipMalloc(&d->X2f, sizeof(cpx<float>) * 3L * N[0] * P1 * H2p);
+    if (d->gen0) hipMalloc(&d->Xzf, sizeof(cpx<float>) * 3L * P0 * P1 * H2p);
     // ---- Kernel_Arne on the device (FP64)
     int r1[3], r2[3];
     for (int c = 0; c < 3; c++) {
@@ -647,7 +751,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
     hipMalloc(&full, sizeof(double) * 6 * P);
     hipMalloc(&X, sizeof(cpx<double>) * 6L * P0 * P1 * H2p);
     hipMalloc(&maxbits, 2 * kExtractLanes * sizeof(unsigned long long));
-    if (!oct || !full || !X || !maxbits || !d->kern64 || !d->kern32 || !d->X1f || !d->X2f) {
+    if (!oct || !full || !X || !maxbits || !d->kern64 || !d->kern32 || !d->X1f || !d->X2f || (d->gen0 && !d->Xzf)) {
         hs::set_error(HS_ENOMEM, "hs_demag_create: out of device memory");
         hipFree(oct); hipFree(full); hipFree(X); hipFree(maxbits);
         hs_demag_destroy(d);
@@ -668,14 +772,23 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
         for (long i = 0; i < noct; i++) cmax[c] = fmax(cmax[c], fabs(hoct[c * noct + i]));
     for (int c = 0; c < 6; c++) gmax = fmax(gmax, cmax[c]);
     // ---- FP64 transform of the 6 real components: R2C rows, then axes 1 and 0 in place
-    auto r2c = pick_r2c<double, double>(M);
-    auto f1 = pick_cols<double, -1>(P1);
-    auto f0 = pick_cols<double, -1>(P0);
-    r2c(full, nullptr, 1, X, 6L * P0 * P1, P2, H2p, d->twMd, d->twP2d, st);
+    // (lengths without a compiled family take the run-time-length passes)
+    const Live all{nullptr, 1};
+    if (d->genRd)
+        gen::r2c<double>(full, nullptr, 1, X, 6L * P0 * P1, P2, P2, H2p, d->gR, row_table<double>(d, d->twMd, d->twP2d),
+                         d->twP2d, st, all);
+    else
+        pick_r2c<double, double>(M)(full, nullptr, 1, X, 6L * P0 * P1, P2, H2p, d->twMd, d->twP2d, st);
     Geo ga1 = {(long)P1 * H2p, 0, H2p, 1};
-    f1(X, X, ga1, ga1, 6 * P0, P1, P1, H2, d->tw1d, st);
+    if (d->gen1d)
+        gen::cols<double, -1>(X, X, ga1, ga1, 6 * P0, P1, P1, H2, d->g1, d->tw1d, st, all);
+    else
+        pick_cols<double, -1>(P1)(X, X, ga1, ga1, 6 * P0, P1, P1, H2, d->tw1d, st);
     Geo ga0 = {(long)P0 * P1 * H2p, (long)H2p, (long)P1 * H2p, P1};
-    f0(X, X, ga0, ga0, 6 * P1, P0, P0, H2, d->tw0d, st);
+    if (auto f0 = pick_cols<double, -1>(P0))
+        f0(X, X, ga0, ga0, 6 * P1, P0, P0, H2, d->tw0d, st);
+    else
+        gen::cols<double, -1>(X, X, ga0, ga0, 6 * P1, P0, P0, H2, d->g0, d->tw0d, st, all);
     hipLaunchKernelGGL(k_kernel_extract, dim3(hs::grid1d(nk), 6), dim3(hs::kBlock), 0, st, X, d->kern64, d->kern32,
                        P0, P1, H2, H2p, d->K[0], d->K[1], d->KP, 1.0 / (double)P, maxbits);
     for (int c = 0; c < 6; c++) {
@@ -712,7 +825,7 @@ extern "C" hs_demag* hs_demag_create(const int N[3], const double cell[3], const
 extern "C" void hs_demag_destroy(hs_demag* d) {
     if (!d) return;
     void* ps[] = {d->kern64, d->kern32, d->X1f, d->X2f, d->X1d, d->X2d, d->twMf, d->twP2f,
-                  d->tw1f,   d->tw0f,   d->twMd, d->twP2d, d->tw1d, d->tw0d};
+                  d->tw1f,   d->tw0f,   d->twMd, d->twP2d, d->tw1d, d->tw0d, d->Xzf, d->Xzd};
     for (void* p : ps)
         if (p) hipFree(p);
     delete d;
@@ -738,9 +851,9 @@ extern "C" double hs_demag_max_imag_ratio(const hs_demag* d) { return d ? d->max
 
 extern "C" int hs_demag_variant_count(const hs_demag* d, int* out3) {
     if (!d || !out3) return HS_EINVAL;
-    out3[0] = (int)hsdm::variants(d->M).rows.size();
-    out3[1] = (int)hsdm::variants(d->P[1]).cols.size();
-    out3[2] = (int)hsdm::variants(d->P[0]).z.size();
+    out3[0] = d->genR ? 0 : (int)hsdm::variants(d->M).rows.size();
+    out3[1] = d->gen1 ? 0 : (int)hsdm::variants(d->P[1]).cols.size();
+    out3[2] = d->gen0 ? 0 : (int)hsdm::variants(d->P[0]).z.size();
     return 0;
 }
 
@@ -773,13 +886,14 @@ extern "C" int hs_demag_field_f64(hs_demag* d, const double* m, const double* ms
     if (!d->X1d) {
         hipMalloc(&d->X1d, sizeof(cpx<double>) * 3L * d->N[0] * d->N[1] * d->H2p);
         hipMalloc(&d->X2d, sizeof(cpx<double>) * 3L * d->N[0] * d->P[1] * d->H2p);
-        if (!d->X1d || !d->X2d) {
+        if (d->gen0d) hipMalloc(&d->Xzd, sizeof(cpx<double>) * 3L * d->P[0] * d->P[1] * d->H2p);
+        if (!d->X1d || !d->X2d || (d->gen0d && !d->Xzd)) {
             hs::set_error(HS_ENOMEM, "hs_demag_field_f64: out of device memory");
             return HS_ENOMEM;
         }
     }
-    return run_field<double>(d, m, msk, HS_MU0 * msat0T0Mul, B, hs::S(stream), d->X1d, d->X2d, d->kern64, d->twMd,
-                             d->twP2d, d->tw1d, d->tw0d);
+    return run_field<double>(d, m, msk, HS_MU0 * msat0T0Mul, B, hs::S(stream), d->X1d, d->X2d, d->Xzd, d->kern64,
+                             d->twMd, d->twP2d, d->tw1d, d->tw0d);
 }
 
 // Internal entry for the solver: FP32 field of a batch of nsolve solves
@@ -793,19 +907,23 @@ int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, d
 extern "C" int hs_demag_reserve(hs_demag* d, int nsolve) {
     if (!d || nsolve < 1) return HS_EINVAL;
     if (nsolve <= d->cap) return 0;
-    cpx<float>*X1 = nullptr, *X2 = nullptr;
+    cpx<float>*X1 = nullptr, *X2 = nullptr, *Xz = nullptr;
     hipMalloc(&X1, sizeof(cpx<float>) * 3L * d->N[0] * d->N[1] * d->H2p * nsolve);
     hipMalloc(&X2, sizeof(cpx<float>) * 3L * d->N[0] * d->P[1] * d->H2p * nsolve);
-    if (!X1 || !X2) {
+    if (d->gen0) hipMalloc(&Xz, sizeof(cpx<float>) * 3L * d->P[0] * d->P[1] * d->H2p * nsolve);
+    if (!X1 || !X2 || (d->gen0 && !Xz)) {
         hipFree(X1);
         hipFree(X2);
+        hipFree(Xz);
         hs::set_error(HS_ENOMEM, "hs_demag_reserve: out of device memory");
         return HS_ENOMEM;
     }
     hipFree(d->X1f);
     hipFree(d->X2f);
+    hipFree(d->Xzf);
     d->X1f = X1;
     d->X2f = X2;
+    d->Xzf = Xz;
     d->cap = nsolve;
     return 0;
 }
@@ -818,6 +936,8 @@ long hs_demag_footprint(const hs_demag* d) {
     b += (long)sizeof(cpx<float>) * 3L * d->N[0] * (d->N[1] + d->P[1]) * d->H2p * d->cap;
     b += 6L * d->K[0] * d->K[1] * (d->KP - d->H2) * (long)sizeof(float);
     if (d->X1d) b += (long)sizeof(cpx<double>) * 3L * d->N[0] * (d->N[1] + d->P[1]) * d->H2p;
+    if (d->Xzf) b += (long)sizeof(cpx<float>) * 3L * d->P[0] * d->P[1] * d->H2p * d->cap;
+    if (d->Xzd) b += (long)sizeof(cpx<double>) * 3L * d->P[0] * d->P[1] * d->H2p;
     b += (long)(sizeof(cpx<float>) + sizeof(cpx<double>)) * (d->M + d->P[2] + d->P[1] + d->P[0]);
     return b;
 }

@@ -54,16 +54,7 @@ __device__ __forceinline__ bool all_dead(const Live& lv, long u0, long u1) {
     return q0 == q1 && !lv.flag[q0];
 }
 
-// XCD-aware block order.  Workgroups are handed to the 8 XCDs round-robin
-// (block b runs on XCD b % 8), and each XCD has its own L2.  Renumbering the
-// blocks of a 2-D grid so that consecutive logical tiles run on the same XCD
-// lets two tiles that share a 128-byte line hit one L2.  Bijective on [0, nb).
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_swizzle(int pid, int nb) {
-    const int q = nb / kXcds, r = nb % kXcds;
-    const int x = pid % kXcds, i = pid / kXcds;
-    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
+#include "xcd.hpp"
 
 struct Geo {  // pencil a starts at (a / A2) * S1 + (a % A2) * S2; element t at + t * ts
     long S1, S2, ts;
@@ -86,8 +77,8 @@ __global__ void __launch_bounds__(NT) k_r2c_rows(const TI* __restrict__ x, const
     long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
     const bool pairs = (n2 & 1) == 0;  // rows of even length: each complex input is one aligned pair
-    // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
-    constexpr int NE = M * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+    // loads in equal batches of at most 24 per thread (no short last batch), each batch in flight before its LDS stores
+    constexpr int NE = M * W, IT = (NE + NT - 1) / NT, NB = (IT + 23) / 24, CH = (IT + NB - 1) / NB;
     for (int q0 = 0; q0 < IT; q0 += CH) {
         cpx<T> g[CH];
 #pragma unroll
@@ -152,8 +143,8 @@ __global__ void __launch_bounds__(NT) k_c2r_rows(const cpx<T>* __restrict__ in, 
     const int tid = threadIdx.x;
     long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
-    {  // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
-        constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+    {  // loads in equal batches of at most 24 per thread (no short last batch), each batch in flight before its LDS stores
+        constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, NB = (IT + 23) / 24, CH = (IT + NB - 1) / NB;
         for (int q0 = 0; q0 < IT; q0 += CH) {
             cpx<T> g[CH];
 #pragma unroll
@@ -234,8 +225,8 @@ __global__ void __launch_bounds__(NT) k_cols(const cpx<T>* in, cpx<T>* out, Geo 
     const int col0 = (lb % gridDim.x) * W;
     const long bi = geo_base(gi, a) + col0;
     const long bo = geo_base(go, a) + col0;
-    {  // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
-        constexpr int NE = L * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+    {  // loads in equal batches of at most 24 per thread (no short last batch), each batch in flight before its LDS stores
+        constexpr int NE = L * W, IT = (NE + NT - 1) / NT, NB = (IT + 23) / 24, CH = (IT + NB - 1) / NB;
         for (int q0 = 0; q0 < IT; q0 += CH) {
             cpx<T> g[CH];
 #pragma unroll

@@ -128,6 +128,20 @@ void add_z2n(Variants& v) {
         v.z.push_back({W, 6 * W, &launch_z2n_32<L, W>});
 }
 
+// the same with the kernel values prefetched into registers (fft2.hpp k_zconv2k)
+template <int L, int W>
+void launch_z2k_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
+                   const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
+    hipLaunchKernelGGL((k_zconv2k<float, float, L, W>), g, dim3(6 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
+                       srcMul, tw, lv);
+}
+template <int L, int W>
+void add_z2k(Variants& v) {
+    if constexpr (reg_dft_ok(L) && L > 1 && L <= 64 && (long)L * 6 * W * 8 <= kLdsMax)
+        v.z.push_back({W, 6 * W, &launch_z2k_32<L, W>});
+}
+
 // two-level register FFT columns (fft2.hpp): W columns, W * B threads
 template <int L, int W>
 void add_col2(Variants& v) {
@@ -264,5 +278,7 @@ void register_size() {
     add_z2s<L, 32>(v);
     add_z2n<L, 16>(v);
     add_z2n<L, 32>(v);
+    add_z2k<L, 16>(v);
+    add_z2k<L, 32>(v);
 }
 }  // namespace hsdm

@@ -531,8 +531,8 @@ __global__ void __launch_bounds__(NT) k_c2r_rows2(const cpx<T>* __restrict__ in,
         tws[e] = twM[e];
         twp[e] = twP[e];
     }
-    {  // loads in batches of up to 16 per thread, each batch in flight before its LDS stores
-        constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, CH = IT < 16 ? IT : 16;
+    {  // loads in equal batches of at most 24 per thread (no short last batch), each batch in flight before its LDS stores
+        constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, NB = (IT + 23) / 24, CH = (IT + NB - 1) / NB;
         for (int q0 = 0; q0 < IT; q0 += CH) {
             cpx<T> g[CH];
 #pragma unroll
@@ -823,6 +823,100 @@ __global__ void __launch_bounds__(6 * W) k_zconv2n(cpx<T>* X, const TK* __restri
         const T kyz0 = T(kern[3 * kplane + K]);
         const T kxz = si * T(kern[4 * kplane + K]);
         const T kxy0 = si * T(kern[5 * kplane + K]);
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const T sj = s2 ? T(-1) : T(1);
+            const T kyz = sj * kyz0, kxy = sj * kxy0;
+            cpx<T>* a = lds + t * W2 + s2 * WP + pp;
+            cpx<T> mx = cscale(a[0], srcMul);
+            cpx<T> my = cscale(a[W], srcMul);
+            cpx<T> mz = cscale(a[2 * W], srcMul);
+            a[0] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
+            a[W] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
+            a[2 * W] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
+        }
+    }
+    __syncthreads();
+    {
+        cpx<T> v[L];
+#pragma unroll
+        for (int t = 0; t < L; t++) v[t] = lds[t * W2 + tid];
+        dft_reg<T, L, +1>(v);
+        if (live) {
+#pragma unroll
+            for (int t = 0; t < L; t++)
+                if (t < N0) {
+                    v2 q;
+                    q.x = v[t].x;
+                    q.y = v[t].y;
+                    __builtin_nontemporal_store(q, (v2*)(base + t * tstride));
+                }
+        }
+    }
+}
+
+// k_zconv2n with the block's kernel values fetched into registers right
+// after the pencils: the 3x3 product then reads no global memory, so a
+// block waits on memory once, at its start, instead of once more per
+// product iteration after the first barrier.  Thread tid's product entries
+// are e = tid + q NT (q < IT), as in k_zconv2n's product loop.
+template <typename T, typename TK, int L, int W>
+__global__ void __launch_bounds__(6 * W) k_zconv2k(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+                                                   int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
+                                                   Live lv) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    constexpr int WP = 3 * W, W2 = 2 * WP, NT = 6 * W;
+    constexpr int NE = L * W, IT = (NE + NT - 1) / NT;
+    static_assert(reg_dft_ok(L), "register DFT length");
+    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
+    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
+    __shared__ cpx<T> lds[L * W2];
+    const int tid = threadIdx.x;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int jp = lb / gridDim.x;
+    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;
+    const int col0 = (lb % gridDim.x) * W;
+    const int p = tid % W, c = (tid / W) % 3, sm = tid / WP;
+    const int j = sm ? jm : jp;
+    const bool live = j >= 0 && col0 + p < H2;
+    const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
+    cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p;
+    const long kplane = (long)(L / 2 + 1) * K1 * kp;
+    T kv[IT][6];
+    {
+        cpx<T> v[L];
+#pragma unroll
+        for (int t = 0; t < L; t++) {
+            cpx<T> x = {T(0), T(0)};
+            if (live && t < N0) {
+                const v2 q = __builtin_nontemporal_load((const v2*)(base + t * tstride));
+                x = {q.x, q.y};
+            }
+            v[t] = x;
+        }
+#pragma unroll
+        for (int q = 0; q < IT; q++) {
+            const int e = tid + q * NT;
+            const bool ok = e < NE && col0 + e % W < H2;
+            const int t = ok ? e / W : 0, col = ok ? col0 + e % W : 0;
+            const int ii = (t > L / 2) ? L - t : t;
+            const long K = ((long)ii * K1 + jp) * kp + col;
+#pragma unroll
+            for (int k = 0; k < 6; k++) kv[q][k] = ok ? T(kern[k * kplane + K]) : T(0);
+        }
+        dft_reg<T, L, -1>(v);
+#pragma unroll
+        for (int t = 0; t < L; t++) lds[t * W2 + tid] = v[t];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < IT; q++) {
+        const int e = tid + q * NT;
+        const int t = e / W, pp = e % W;
+        if (e >= NE || col0 + pp >= H2) continue;
+        const T si = (t > L / 2) ? T(-1) : T(1);
+        const T kxx = kv[q][0], kyy = kv[q][1], kzz = kv[q][2], kyz0 = kv[q][3];
+        const T kxz = si * kv[q][4], kxy0 = si * kv[q][5];
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
             const T sj = s2 ? T(-1) : T(1);

@@ -24,6 +24,7 @@
 
 #include "hs_common.hpp"
 #include "philox.hpp"
+#include "xcd.hpp"
 
 int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,
                                 hipStream_t st, int nsolve, const int* live);
@@ -168,11 +169,11 @@ __device__ __forceinline__ int wrapc(int c, int n, int w) {
     return c;
 }
 
-// Load the (j,k) tile of plane i with its one-cell halo into LDS (the halo
-// rows/columns are wrapped or clamped as the stencil needs).
-__device__ __forceinline__ void load_tile(const Phys& P, const float* m, long N, int i, int jc, int kc, int tx, int ty,
-                                          const float own[3], float lown, float (*tile)[TY + 2][TX + 2],
-                                          float (*ltile)[TX + 2]) {
+// Load the (j,k) tile (bx, by) of plane i with its one-cell halo into LDS
+// (the halo rows/columns are wrapped or clamped as the stencil needs).
+__device__ __forceinline__ void load_tile(const Phys& P, const float* m, long N, int i, int bx, int by, int jc, int kc,
+                                          int tx, int ty, const float own[3], float lown,
+                                          float (*tile)[TY + 2][TX + 2], float (*ltile)[TX + 2]) {
     const int N1 = P.N1, N2 = P.N2;
     const long N12 = (long)N1 * N2;
     for (int c = 0; c < 3; c++) tile[c][ty + 1][tx + 1] = own[c];
@@ -180,14 +181,14 @@ __device__ __forceinline__ void load_tile(const Phys& P, const float* m, long N,
     if (!P.exchange) return;
     const long base = i * N12;
     if (ty == 0 || ty == TY - 1) {
-        int jn = (ty == 0) ? wrapc(blockIdx.y * TY - 1, N1, P.w1) : wrapc(min(blockIdx.y * TY + TY, N1), N1, P.w1);
+        int jn = (ty == 0) ? wrapc(by * TY - 1, N1, P.w1) : wrapc(min(by * TY + TY, N1), N1, P.w1);
         int row = (ty == 0) ? 0 : TY + 1;
         long a2 = base + (long)jn * N2 + kc;
         for (int c = 0; c < 3; c++) tile[c][row][tx + 1] = m[c * N + a2];
         if (P.lex) ltile[row][tx + 1] = P.lex[a2];
     }
     if (tx == 0 || tx == TX - 1) {
-        int kn = (tx == 0) ? wrapc(blockIdx.x * TX - 1, N2, P.w2) : wrapc(min(blockIdx.x * TX + TX, N2), N2, P.w2);
+        int kn = (tx == 0) ? wrapc(bx * TX - 1, N2, P.w2) : wrapc(min(bx * TX + TX, N2), N2, P.w2);
         int col = (tx == 0) ? 0 : TX + 1;
         long a2 = base + (long)jc * N2 + kn;
         for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = m[c * N + a2];
@@ -740,7 +741,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
         }
         Pre q;
         preload<ROLE>(P, io, I, N, q);
-        load_tile(P, m, N, i, jc, kc, tx, ty, a[1], la[1], tile, ltile);
+        load_tile(P, m, N, i, blockIdx.x, blockIdx.y, jc, kc, tx, ty, a[1], la[1], tile, ltile);
         __syncthreads();
         float m0[3] = {a[1][0], a[1][1], a[1][2]};
         float H[3] = {0.f, 0.f, 0.f};
@@ -762,14 +763,27 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
 
 // Flat variant: one plane per block, planes i-1 and i+1 read directly (they
 // are the interior of other blocks and come from L2 / Infinity Cache).
-template <int ROLE>
+// ZORD: a 1-D grid whose blocks, renumbered so that consecutive numbers run
+// on one XCD (xcd_swizzle), take the planes of a 64x4 tile one after the
+// other; the m values of planes i-1 and i+1 that a block reads were loaded
+// into that XCD's L2 by the blocks just before and after it.
+template <int ROLE, bool ZORD = false>
 __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
     __shared__ float tile[3][TY + 2][TX + 2];
     __shared__ float ltile[TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int N0 = P.N0, N1 = P.N1, N2 = P.N2;
-    const int k = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty, i = blockIdx.z % N0;
-    const int sol = blockIdx.z / N0;
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if constexpr (ZORD) {
+        const int gx = (N2 + TX - 1) / TX, gy = (N1 + TY - 1) / TY;
+        const int q = xcd_swizzle(blockIdx.x, gridDim.x);
+        const int r = q / N0;
+        bx = r % gx;
+        by = (r / gx) % gy;
+        bz = (r / (gx * gy)) * N0 + q % N0;
+    }
+    const int k = bx * TX + tx, j = by * TY + ty, i = bz % N0;
+    const int sol = bz / N0;
     if (solve_idle(io0, sol)) return;
     const long N12 = (long)N1 * N2;
     const long N = (long)N0 * N12;
@@ -798,7 +812,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
             lhi = P.lex[Ihi];
         }
     }
-    load_tile(P, m, N, i, jc, kc, tx, ty, m0, l0, tile, ltile);
+    load_tile(P, m, N, i, bx, by, jc, kc, tx, ty, m0, l0, tile, ltile);
     __syncthreads();
     float H[3] = {0.f, 0.f, 0.f};
     if (P.exchange && in) exchange_cell<TX>(P, m, N, i, j, k, jc, tx + 1, ty, m0, lo, hi, l0, llo, lhi, tile, ltile, H);
@@ -1443,7 +1457,7 @@ struct hs_solver {
     unsigned long long* errmax = nullptr;
     std::vector<float*> masks;
     bool b0_valid = false;  // B0 = demag(mA) is current
-    int stage_kind = 1;      // 0 marching planes, 1 one plane per block, 2 two cells per thread
+    int stage_kind = 1;      // 0 marching planes, 1 one plane per block, 2 two cells per thread, 3 = 1 in XCD plane order
     // two-temperature coupling (tm): Te, Tp in two buffer sets (state / candidate), two stage inputs
     bool tm = false;
     hs_tm_params tmp{};
@@ -1494,6 +1508,9 @@ void launch_stage(hs_solver* s, const StageIO& io) {
     } else if (s->stage_kind == 1) {
         dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), P.N0 * s->nsolve);
         hipLaunchKernelGGL(k_stage_flat<ROLE>, grd, blk, 0, s->st, P, io);
+    } else if (s->stage_kind == 3) {
+        dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX) * hs::div_up(P.N1, TY) * P.N0 * s->nsolve);
+        hipLaunchKernelGGL((k_stage_flat<ROLE, true>), grd, blk, 0, s->st, P, io);
     } else {
         dim3 blk(TX, TY), grd(hs::div_up(P.N2, TX), hs::div_up(P.N1, TY), hs::div_up(P.N0, P.chunk) * s->nsolve);
         hipLaunchKernelGGL(k_stage<ROLE>, grd, blk, 0, s->st, P, io);
@@ -1530,8 +1547,12 @@ std::map<std::array<int, 3>, std::array<int, 2>>& stage_cache() {  // shape -> (
 
 void tune_stage(hs_solver* s) {
     const bool pair_ok = s->phys.N2 % 2 == 0;
+    // (variant 3, the flat kernel in XCD plane order, wins this isolated timing
+    // at 512x512x16 yet runs 10 % slower between the demag passes of a step,
+    // so it is taken only when forced)
     const char* env = getenv("HS_STAGE_KERNEL");
     if (env && !strcmp(env, "flat")) { s->stage_kind = 1; return; }
+    if (env && !strcmp(env, "flatz")) { s->stage_kind = 3; return; }
     if (env && !strcmp(env, "march")) { s->stage_kind = 0; return; }
     if (env && !strcmp(env, "pair") && pair_ok) { s->stage_kind = 2; return; }
     const std::array<int, 3> key{s->phys.N0, s->phys.N1, s->phys.N2};

@@ -30,7 +30,7 @@ extern "C" {
 /* 0 = no error since the last hs_clear_error(); otherwise a hipError_t code
  * or one of the HS_E* codes below.  Sticky: the first error is kept. */
 #define HS_EINVAL 10001 /* invalid argument (sizes, NULL where not allowed) */
-#define HS_ESIZE 10002  /* FFT length not supported (padded lengths 2^k, 3*2^k or 5*2^k, even along x) */
+#define HS_ESIZE 10002  /* size not supported (a padded length, or half the padded x length, above 4095) */
 #define HS_ECONV 10003  /* adaptive step did not converge within 10 retries */
 #define HS_ENOMEM 10004
 HS_API int hs_last_error(void);
@@ -46,7 +46,11 @@ typedef struct hs_demag hs_demag;
  * transformed in FP64 by the library's own FFT, real octant kept, scaled by
  * 1/(P0 P1 P2) and stored in FP64 and FP32.  accuracy = 6 in the reference
  * (maxwell_convolution.go:96).  Blocks until the kernel is ready.
- * Returns NULL on error (hs_last_error). */
+ * Any padded lengths up to 4095 (half the padded x length when it is even):
+ * 2^k, 3*2^k and 5*2^k run compiled, tuned passes; other lengths (factors 7,
+ * 9, 25, larger primes, odd periodic x lengths, one-cell axes) run passes
+ * that take the length at run time, as cuFFT plans do in the reference
+ * (gpu/fftplanX.go:55-60).  Returns NULL on error (hs_last_error). */
 HS_API hs_demag* hs_demag_create(const int N[3], const double cell[3], const int pbc[3], int accuracy,
                                  CUstream stream);
 HS_API void hs_demag_destroy(hs_demag* d);
@@ -62,7 +66,11 @@ HS_API double hs_demag_max_imag_ratio(const hs_demag* d);
 HS_API int hs_demag_tiles(const hs_demag* d, int* out6);
 /* Number of registered FP32 variants of the row, column and axis-0 passes
  * for this plan's lengths (the environment variable HS_DEMAG_FORCE="r,c,z"
- * selects variants by index at plan creation instead of timing them). */
+ * selects variants by index at plan creation instead of timing them).  The
+ * variants of a pass round differently (FP32, operation order); the timed
+ * choice is kept per mesh shape for the process, and HS_TUNE_FILE=<path>
+ * persists it across processes (read before timing, appended after), so
+ * runs that share the file give the same bits. */
 HS_API int hs_demag_variant_count(const hs_demag* d, int* out3);
 /* B = mu0 * msat0T0Mul * (K conv (m .* msat0T0)), maxwell_convolution.go:103-131.
  * m, B: 3 device planes of N cells (m and B may not alias); msat0T0: device
@@ -198,8 +206,11 @@ HS_API CUstream hs_solver_stream(hs_solver* s);
 /* the solver's demag plan (NULL when demag is off) */
 HS_API hs_demag* hs_solver_demag(hs_solver* s);
 /* stage-kernel variant picked by timing at creation: 0 = marching planes,
- * 1 = one plane per block, 2 = two cells per thread (even N2 only); env
- * HS_STAGE_KERNEL=march|flat|pair overrides */
+ * 1 = one plane per block, 2 = two cells per thread (even N2 only); 3 = one
+ * plane per block with the planes of a tile consecutive on one XCD (only
+ * when forced); env HS_STAGE_KERNEL=march|flat|pair|flatz overrides.  Every
+ * variant does the same arithmetic per cell, so results do not depend on
+ * the choice. */
 HS_API int hs_solver_stage_variant(hs_solver* s);
 
 /* ------------------------------------------------ two-temperature coupling

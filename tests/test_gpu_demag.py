@@ -131,11 +131,80 @@ def test_every_fp32_pass_variant_matches_fp64(monkeypatch, N, pbc):
             d.close()
 
 
-@pytest.mark.parametrize("N", [(11, 8, 8), (7, 7, 14), (4, 8, 1)])
-def test_unsupported_fft_length_is_an_error(N):
-    """Padded lengths must be 2^k, 3 2^k or 5 2^k, even along x: 11 or 7 cells
-    (padded 22, 14) and a one-cell x extent (P2 = 1) are refused with
-    HS_ESIZE at plan creation, before anything runs."""
+# Padded lengths without a compiled codelet family run the run-time-length
+# passes (fft_any.hpp): prime factors 7, 11 and 13, 25 = 5^2 (a 100-cell axis
+# pads to 200 = 8 * 25), an odd x length of a periodic axis (P2 = 9), a one-cell
+# x extent (P2 = 1), and such lengths next to compiled ones.
+GENERIC = [
+    ((11, 8, 8), (2e-9, 2e-9, 2e-9), (0, 0, 0)),       # P0 = 22
+    ((7, 7, 14), (2e-9, 3e-9, 2e-9), (0, 0, 0)),       # P = 14, 14, 28 (row FFT of 14)
+    ((4, 8, 1), (2e-9, 2e-9, 3e-9), (0, 0, 0)),        # P2 = 1
+    ((2, 100, 50), (3e-9, 2e-9, 2e-9), (0, 0, 0)),     # P1 = 200, row FFT of 50
+    ((4, 8, 9), (2e-9, 2e-9, 2e-9), (0, 0, 1)),        # periodic x: P2 = 9 (odd)
+    ((3, 9, 12), (1e-9, 2e-9, 1e-9), (0, 1, 0)),       # periodic y: P1 = 9
+    ((13, 4, 6), (2e-9, 2e-9, 2e-9), (0, 0, 0)),       # P0 = 26
+]
+
+
+@pytest.mark.parametrize("N,cell,pbc", GENERIC)
+def test_generic_lengths_match_oracle(N, cell, pbc):
+    """Kernel octant (1e-12), FP64 field (1e-12) and FP32 field (2e-6) against
+    the oracle, whose FFT takes any length (checked against numpy's)."""
+    d = Demag(N, cell, pbc)
+    o = po.Demag(N, cell, pbc)
+    assert d.P == o.P and d.K == o.K
+    assert relerr(d.kernel(), o.kernel()) <= 1e-12
+    assert d.max_imag_ratio() < 1e-12
+    rng = np.random.default_rng(sum(N))
+    n = N[0] * N[1] * N[2]
+    m = rng.standard_normal((3, n))
+    msk = rng.random(n)
+    want = o.B(m, msk, 8e5).reshape(3, n)
+    dm, dmsk = dev(m), dev(msk)  # held: a tensor freed after its pointer is taken may be reused
+    B = empty((3, n))
+    d.field_f64(P(dm), P(dmsk), 8e5, P(B))
+    assert relerr(host(B), want) <= 1e-12
+    dm32, dmsk32 = dev(m, np.float32), dev(msk, np.float32)
+    B32 = empty((3, n), torch.float32)
+    d.field_f32(P(dm32), P(dmsk32), 8e5, P(B32))
+    assert relerr(host(B32), want) <= 2e-6
+    d.close()
+
+
+def test_generic_length_solves_alone_in_a_batch_and_vs_oracle():
+    """An LLB-r batch on a mesh whose axis-0 pass runs at a run-time length
+    (P0 = 14, the spectrum of all P0 rows held per solve): each solve equals
+    its solve alone bit for bit, and the oracle after 6 RK4 steps."""
+    from hotspin_amd import workloads as W
+    from hotspin_amd.api import Batch, Solver
+    N, cell = (7, 8, 10), (2e-9, 2e-9, 2e-9)
+    mat = W.MATERIAL_WIDE_T
+    kw = W.llbr_params(T=300.0, seed=5, material=mat, Bext=(0.01, -0.02, 0.03))
+    rng = np.random.default_rng(2)
+    n = N[0] * N[1] * N[2]
+    T = rng.uniform(200, 900, 3)
+    m = np.stack([W.random_state(n, W.equilibrium_m(t, mat["Tc"], mat["J"]), rng) for t in T])
+    seeds = np.arange(3, dtype=np.uint64) + 11
+    b = Batch(N, cell, 3, TMuls=T, seeds=seeds, **kw)
+    b.set_m(m)
+    b.run("rk4", 6, 2e-15)
+    mb = b.get_m()
+    kq = dict(kw)
+    kq.update(TMul=float(T[1]), seed=int(seeds[1]))
+    s = Solver(N, cell, **kq)
+    s.set_m(m[1])
+    s.run("rk4", 6, 2e-15)
+    assert np.array_equal(s.get_m(), mb[1])
+    o = po.LLB(N, cell, **kq)
+    mo = m[1].copy()
+    for i in range(6):
+        o.step("rk4", mo, 2e-15, i)
+    assert np.abs(mb[1] - mo).max() <= 1e-5
+
+
+def test_length_beyond_the_generic_passes_is_an_error():
+    """Half the padded x length above 4095 (a 4097-cell x extent) is refused
+    with HS_ESIZE at plan creation, before anything runs."""
     from hotspin_amd._lib import HotspinError
     with pytest.raises(HotspinError, match="hs error 10002"):  # HS_ESIZE
-        Demag(N, (2e-9, 2e-9, 2e-9))
+        Demag((1, 1, 4097), (2e-9, 2e-9, 2e-9))

@@ -145,12 +145,13 @@ def test_llb_rk23_adaptive_matches_oracle(masks):
         assert np.abs(g.get_m() - mo).max() <= 1e-5, i
 
 
-@pytest.mark.parametrize("variant", ["march", "flat", "pair"])
+@pytest.mark.parametrize("variant", ["march", "flat", "pair", "flatz"])
 @pytest.mark.parametrize("pbc", [(0, 0, 0), (1, 1, 1)])
 def test_stage_kernel_variants_match_oracle(variant, pbc, monkeypatch):
     """Each stage-kernel variant (marching planes, one plane per block, two
-    cells per thread) against the oracle: RHS and 20 thermal RK4 steps, on a
-    mesh whose rows are not a multiple of the tile widths."""
+    cells per thread, one plane per block in XCD plane order) against the
+    oracle: RHS and 20 thermal RK4 steps, on a mesh whose rows are not a
+    multiple of the tile widths."""
     monkeypatch.setenv("HS_STAGE_KERNEL", variant)
     N, cell = (5, 12, 160), (2e-9, 2e-9, 2e-9)
     kw, m = llb_problem(N, cell, masks=True, pbc=pbc)
@@ -159,7 +160,7 @@ def test_stage_kernel_variants_match_oracle(variant, pbc, monkeypatch):
     Ho = o.heff()
     g = Solver(N, cell, pbc, **kw)
     from hotspin_amd._lib import lib
-    assert lib().hs_solver_stage_variant(g._h) == {"march": 0, "flat": 1, "pair": 2}[variant]
+    assert lib().hs_solver_stage_variant(g._h) == {"march": 0, "flat": 1, "pair": 2, "flatz": 3}[variant]
     g.set_m(m)
     kg, Hg = g.rhs(step=3, attempt=0, dt=1e-15)
     assert relerr(Hg, Ho) <= 2e-5
@@ -169,3 +170,23 @@ def test_stage_kernel_variants_match_oracle(variant, pbc, monkeypatch):
         o.step("rk4", mo, 2e-15, i)
     g.run("rk4", 20, 2e-15)
     assert np.abs(g.get_m() - mo).max() <= 1e-5
+
+
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (1, 1, 1)])
+def test_stage_kernel_variants_are_bitwise_identical(pbc, monkeypatch):
+    """The stage-kernel variant is picked by timing per process; every
+    variant does the same arithmetic per cell, so the choice never changes a
+    result: 12 thermal RK23 steps give the same bits with each forced."""
+    N, cell = (5, 12, 160), (2e-9, 2e-9, 2e-9)
+    kw, m = llb_problem(N, cell, masks=True, pbc=pbc)
+    got = {}
+    for variant in ("march", "flat", "pair", "flatz"):
+        monkeypatch.setenv("HS_STAGE_KERNEL", variant)
+        g = Solver(N, cell, pbc, **kw)
+        g.set_m(m)
+        g.run("rk23", 12, 1e-15, maxErr=1e-5, mindt=1e-18, maxdt=1e-12)
+        got[variant] = g.get_m()
+        g.close()
+    for variant, mv in got.items():
+        assert np.isfinite(mv).all()
+        assert np.array_equal(mv, got["flat"]), (variant, np.abs(mv - got["flat"]).max())

@@ -44,8 +44,11 @@ def _brute_conv(kern_real, P, N, m, msk, Ms):
     return MU0 * Ms * B
 
 
-@pytest.mark.parametrize("N,pbc", [((2, 3, 4), (0, 0, 0)), ((1, 4, 3), (0, 0, 0)), ((3, 2, 2), (0, 1, 0))])
+@pytest.mark.parametrize("N,pbc", [((2, 3, 4), (0, 0, 0)), ((1, 4, 3), (0, 0, 0)), ((3, 2, 2), (0, 1, 0)),
+                                   ((2, 3, 5), (0, 0, 1)), ((3, 4, 1), (0, 0, 0)), ((7, 2, 3), (0, 0, 0))])
 def test_demag_fft_equals_real_space_convolution(N, pbc):
+    """Includes an odd periodic x length (P2 = 5), a one-cell x extent (P2 = 1)
+    and radix 7 (P0 = 14)."""
     cell = (2e-9, 3e-9, 2.5e-9)
     d = po.Demag(N, cell, pbc)
     kern = po.kernel_arne(d.P, cell, pbc)
