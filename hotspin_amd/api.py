@@ -432,6 +432,11 @@ class Demag:
         check("hs_demag_field_f64")
         return rc
 
+    def field_f64_noalloc(self, m_ptr, msk_ptr, msat0T0Mul, B_ptr, stream=None):
+        rc = lib().hs_demag_field_f64_noalloc(self._h, m_ptr, msk_ptr, float(msat0T0Mul), B_ptr, stream)
+        check("hs_demag_field_f64_noalloc")
+        return rc
+
     def field_f32(self, m_ptr, msk_ptr, msat0T0Mul, B_ptr, stream=None):
         rc = lib().hs_demag_field_f32(self._h, m_ptr, msk_ptr, float(msat0T0Mul), B_ptr, stream)
         check("hs_demag_field_f32")

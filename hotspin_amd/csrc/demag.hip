@@ -896,6 +896,72 @@ extern "C" int hs_demag_field_f64(hs_demag* d, const double* m, const double* ms
                              d->twMd, d->twP2d, d->tw1d, d->tw0d);
 }
 
+// ---- FP64 field without allocation: spectra in a static device array
+// `scratch` holds X1 (3, N0, N1, H2p) and, from the next 256-byte boundary,
+// X2 (3, N0, P1, H2p) in FP64 complex.  Every call on one GPU uses that GPU's
+// copy of the array, so calls that overlap on one GPU (two streams, two host
+// threads) race.  No pass reads a spectrum element that the same call has
+// not written first, so the contents left by an earlier call do not matter.
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Device address of the current device's `scratch` (the array's name is a
+// host-side stand-in that kernels must never receive).  Looked up once per
+// device with that device current.
+void* scratch_address() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch[dev] = p;
+    }
+    return g_scratch[dev];
+}
+
+// FP64 spectra of one solve of the plan's mesh: X1, then X2 and (run-time
+// axis-0 length only) Xz, each from a 256-byte boundary
+size_t x_bytes(const hs_demag* d, int rows) { return sizeof(cpx<double>) * 3ull * rows * d->P[1] * d->H2p; }
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+size_t noalloc_bytes(const hs_demag* d) {
+    const size_t x1 = sizeof(cpx<double>) * 3ull * d->N[0] * d->N[1] * d->H2p;
+    return up256(x1) + up256(x_bytes(d, d->N[0])) + (d->gen0d ? x_bytes(d, d->P[0]) : 0);
+}
+}  // namespace
+
+extern "C" int hs_demag_field_f64_noalloc(hs_demag* d, const double* m, const double* msk, double msat0T0Mul,
+                                          double* B, CUstream stream) {
+    if (!d || !m || !B) {
+        hs::set_error(HS_EINVAL, "hs_demag_field_f64_noalloc");
+        return HS_EINVAL;
+    }
+    const size_t need = noalloc_bytes(d);
+    if (need > kScratchBytes) {
+        hs::set_error(HS_ESIZE, "hs_demag_field_f64_noalloc: the mesh's FP64 spectra exceed the 64 MiB static "
+                                "scratch; use hs_demag_field_f64");
+        return HS_ESIZE;
+    }
+    unsigned char* base = static_cast<unsigned char*>(scratch_address());
+    if (!base) {
+        hs::set_error(HS_EINVAL, "hs_demag_field_f64_noalloc: no device address for the static scratch");
+        return HS_EINVAL;
+    }
+    const size_t x1 = up256(sizeof(cpx<double>) * 3ull * d->N[0] * d->N[1] * d->H2p);
+    cpx<double>* X1 = reinterpret_cast<cpx<double>*>(base);
+    cpx<double>* X2 = reinterpret_cast<cpx<double>*>(base + x1);
+    cpx<double>* Xz = d->gen0d ? reinterpret_cast<cpx<double>*>(base + x1 + up256(x_bytes(d, d->N[0]))) : nullptr;
+    return run_field<double>(d, m, msk, HS_MU0 * msat0T0Mul, B, hs::S(stream), X1, X2, Xz, d->kern64, d->twMd,
+                             d->twP2d, d->tw1d, d->tw0d);
+}
+
 // Internal entry for the solver: FP32 field of a batch of nsolve solves
 // (layout [solve][3][cells]) with an FP32 mask; live = per-solve flags or null.
 int hs_demag_field_f32_internal(hs_demag* d, const float* m, const float* msk, double msat0T0Mul, float* B,

@@ -82,6 +82,17 @@ HS_API int hs_demag_field_f32(hs_demag* d, const float* m, const float* msat0T0,
 /* Grow the plan's FP32 spectrum buffers to hold nsolve solves of the mesh
  * (batched solves share one plan and one kernel). */
 HS_API int hs_demag_reserve(hs_demag* d, int nsolve);
+/* hs_demag_field_f64 without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The FP64
+ * spectra live in a 64 MiB static device array, one per GPU, so the mesh must
+ * satisfy 48 * N0 * (N1 + P1) * H2p <= 64 MiB (H2p = P2/2 + 1 rounded up to a
+ * multiple of 16; 256-byte alignment aside; plus 48 * P0 * P1 * H2p when P0
+ * has no compiled axis-0 pass); larger meshes get HS_ESIZE.
+ * Results equal hs_demag_field_f64's bit for bit.  Calls that overlap on one
+ * GPU share that array and race; serialise them. */
+HS_API int hs_demag_field_f64_noalloc(hs_demag* d, const double* m, const double* msat0T0, double msat0T0Mul,
+                                      double* B, CUstream stream);
 
 /* ---------------------------------------------------------------- thermal noise */
 /* Philox4x32-10 (Salmon et al., SC'11) keyed by seed, counter
