@@ -108,6 +108,19 @@ void launch_z2s_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int
     hipLaunchKernelGGL((k_zconv2s<float, float, L, W>), g, dim3(6 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
                        srcMul, tw, lv);
 }
+// ... and with at most 128 VGPRs (4 waves per SIMD)
+template <int L, int W>
+void launch_z2s4_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
+                    const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
+    hipLaunchKernelGGL((k_zconv2s<float, float, L, W, 4>), g, dim3(6 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
+                       srcMul, tw, lv);
+}
+template <int L, int W>
+void add_z2s4(Variants& v) {
+    if constexpr (reg_dft_ok(L) && L > 1 && L <= 64 && (long)L * 6 * W * 4 <= kLdsMax)
+        v.z.push_back({W, 6 * W, &launch_z2s4_32<L, W>});
+}
 template <int L, int W>
 void add_z2s(Variants& v) {
     if constexpr (reg_dft_ok(L) && L > 1 && L <= 64 && (long)L * 6 * W * 4 <= kLdsMax)
@@ -126,20 +139,6 @@ template <int L, int W>
 void add_z2n(Variants& v) {
     if constexpr (reg_dft_ok(L) && L > 1 && L <= 64 && (long)L * 6 * W * 8 <= kLdsMax)
         v.z.push_back({W, 6 * W, &launch_z2n_32<L, W>});
-}
-
-// the same with the kernel values prefetched into registers (fft2.hpp k_zconv2k)
-template <int L, int W>
-void launch_z2k_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
-                   const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
-    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
-    hipLaunchKernelGGL((k_zconv2k<float, float, L, W>), g, dim3(6 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
-                       srcMul, tw, lv);
-}
-template <int L, int W>
-void add_z2k(Variants& v) {
-    if constexpr (reg_dft_ok(L) && L > 1 && L <= 64 && (long)L * 6 * W * 8 <= kLdsMax)
-        v.z.push_back({W, 6 * W, &launch_z2k_32<L, W>});
 }
 
 // two-level register FFT columns (fft2.hpp): W columns, W * B threads
@@ -174,7 +173,7 @@ template <int L, int S, int W>
 void launch_cols2p_32(const cpx<float>* in, cpx<float>* out, Geo gi, Geo go, int npencil, int nin, int nkeep,
                       int ncol, const cpx<float>* tw, hipStream_t st, Live lv) {
     dim3 g((unsigned)((ncol + W - 1) / W), (unsigned)npencil);
-    hipLaunchKernelGGL((k_cols2p<float, L, S, W>), g, dim3(W * split2(L).B), 0, st, in, out, gi, go, nin, nkeep,
+    hipLaunchKernelGGL((k_cols2p<float, L, S, W, 3>), g, dim3(W * split2(L).B), 0, st, in, out, gi, go, nin, nkeep,
                        ncol, tw, lv);
 }
 template <int L, int W>
@@ -186,7 +185,6 @@ void add_col2p(Variants& v) {
             v.cols.push_back({W, NT, &launch_cols2p_32<L, -1, W>, &launch_cols2p_32<L, 1, W>});
     }
 }
-
 // half-length column pass (fft2.hpp k_cols2h) for a zero-padded axis: the
 // forward pass needs nin <= L/2 and the inverse nkeep <= L/2; otherwise (a
 // periodic axis, no padding) the launcher runs the length-L k_cols2
@@ -278,7 +276,6 @@ void register_size() {
     add_z2s<L, 32>(v);
     add_z2n<L, 16>(v);
     add_z2n<L, 32>(v);
-    add_z2k<L, 16>(v);
-    add_z2k<L, 32>(v);
+    add_z2s4<L, 32>(v);
 }
 }  // namespace hsdm

@@ -223,26 +223,51 @@ __global__ void __launch_bounds__(NT) k_cols2s(const cpx<T>* in, cpx<T>* out, Ge
     const int col0 = (lb % gridDim.x) * W;
     const long bi = geo_base(gi, a) + col0;
     const long bo = geo_base(go, a) + col0;
-    for (int e = tid; e < L; e += NT) tws[e] = tw[e];
     const int p = tid % W, n2 = tid / W;
     const bool pin = col0 + p < ncol;
+    // every global load in flight before the first wait: the pencil data from
+    // clamped (valid) addresses, zeroed afterwards outside the input rows,
+    // then the twiddle table into registers
     cpx<T> v[A];
-#pragma unroll
-    for (int n1 = 0; n1 < A; n1++) {
+    // three straight-line load paths chosen by a uniform branch: all L rows
+    // are input (inverse pass), the first half (forward pass of a padded
+    // axis: the zero half is not read), or any other count
+    auto load = [&](int n1) {
         const int t = B * n1 + n2;
-        cpx<T> x = {T(0), T(0)};
-        if (pin && t < nin) x = in[bi + (long)t * gi.ts + p];
-        v[n1] = x;
+        const bool ok = pin && t < nin;
+        v[n1] = in[ok ? bi + (long)t * gi.ts + p : bi];
+        if (!ok) v[n1] = {T(0), T(0)};
+    };
+    if (nin == L) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) load(n1);
+    } else if (A % 2 == 0 && 2 * nin == L) {
+#pragma unroll
+        for (int n1 = 0; n1 < A / 2; n1++) load(n1);
+#pragma unroll
+        for (int n1 = A / 2; n1 < A; n1++) v[n1] = {T(0), T(0)};
+    } else {
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) load(n1);
+    }
+    {
+        constexpr int TW = (L + NT - 1) / NT;
+        cpx<T> tv[TW];
+#pragma unroll
+        for (int q = 0; q < TW; q++) tv[q] = tw[tid + q * NT < L ? tid + q * NT : 0];
+#pragma unroll
+        for (int q = 0; q < TW; q++)
+            if (tid + q * NT < L) tws[tid + q * NT] = tv[q];
     }
     __syncthreads();  // twiddles visible
     dft_reg<T, A, S>(v);
+    // w_L^(n2 k1) for every k1 > 0 (tws[0] = 1 exactly, so n2 = 0 needs no
+    // branch and the table reads go out together)
 #pragma unroll
     for (int k1 = 1; k1 < A; k1++) {
-        if (n2 * k1) {
-            cpx<T> w = tws[n2 * k1];
-            if (S > 0) w.y = -w.y;
-            v[k1] = cmul(v[k1], w);
-        }
+        cpx<T> w = tws[n2 * k1];
+        if (S > 0) w.y = -w.y;
+        v[k1] = cmul(v[k1], w);
     }
 #pragma unroll
     for (int k1 = 0; k1 < A; k1++) lds[(k1 * B + n2) * W + p] = v[k1].x;
@@ -276,8 +301,8 @@ __global__ void __launch_bounds__(NT) k_cols2s(const cpx<T>* in, cpx<T>* out, Ge
 // conflicts: each k1 row of the exchange array is padded by W floats, so the
 // 32 lanes of a half-wave (32 / W values of k1, W columns each) fall on 32
 // distinct banks (three W = 8 blocks per CU instead of four).
-template <typename T, int L, int S, int W, int NT = W * split2(L).B>
-__global__ void __launch_bounds__(NT) k_cols2p(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
+template <typename T, int L, int S, int W, int MINW = 1, int NT = W * split2(L).B>
+__global__ void __launch_bounds__(NT, MINW) k_cols2p(const cpx<T>* in, cpx<T>* out, Geo gi, Geo go, int nin, int nkeep,
                                                int ncol, const cpx<T>* __restrict__ tw, Live lv) {
     constexpr int A = split2(L).A, B = split2(L).B, RS = B * W + W;  // padded k1 row stride
     static_assert(A * B == L && NT == W * B && A <= B, "two-level split");
@@ -290,26 +315,51 @@ __global__ void __launch_bounds__(NT) k_cols2p(const cpx<T>* in, cpx<T>* out, Ge
     const int col0 = (lb % gridDim.x) * W;
     const long bi = geo_base(gi, a) + col0;
     const long bo = geo_base(go, a) + col0;
-    for (int e = tid; e < L; e += NT) tws[e] = tw[e];
     const int p = tid % W, n2 = tid / W;
     const bool pin = col0 + p < ncol;
+    // every global load in flight before the first wait: the pencil data from
+    // clamped (valid) addresses, zeroed afterwards outside the input rows,
+    // then the twiddle table into registers
     cpx<T> v[A];
-#pragma unroll
-    for (int n1 = 0; n1 < A; n1++) {
+    // three straight-line load paths chosen by a uniform branch: all L rows
+    // are input (inverse pass), the first half (forward pass of a padded
+    // axis: the zero half is not read), or any other count
+    auto load = [&](int n1) {
         const int t = B * n1 + n2;
-        cpx<T> x = {T(0), T(0)};
-        if (pin && t < nin) x = in[bi + (long)t * gi.ts + p];
-        v[n1] = x;
+        const bool ok = pin && t < nin;
+        v[n1] = in[ok ? bi + (long)t * gi.ts + p : bi];
+        if (!ok) v[n1] = {T(0), T(0)};
+    };
+    if (nin == L) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) load(n1);
+    } else if (A % 2 == 0 && 2 * nin == L) {
+#pragma unroll
+        for (int n1 = 0; n1 < A / 2; n1++) load(n1);
+#pragma unroll
+        for (int n1 = A / 2; n1 < A; n1++) v[n1] = {T(0), T(0)};
+    } else {
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) load(n1);
+    }
+    {
+        constexpr int TW = (L + NT - 1) / NT;
+        cpx<T> tv[TW];
+#pragma unroll
+        for (int q = 0; q < TW; q++) tv[q] = tw[tid + q * NT < L ? tid + q * NT : 0];
+#pragma unroll
+        for (int q = 0; q < TW; q++)
+            if (tid + q * NT < L) tws[tid + q * NT] = tv[q];
     }
     __syncthreads();  // twiddles visible
     dft_reg<T, A, S>(v);
+    // w_L^(n2 k1) for every k1 > 0 (tws[0] = 1 exactly, so n2 = 0 needs no
+    // branch and the table reads go out together)
 #pragma unroll
     for (int k1 = 1; k1 < A; k1++) {
-        if (n2 * k1) {
-            cpx<T> w = tws[n2 * k1];
-            if (S > 0) w.y = -w.y;
-            v[k1] = cmul(v[k1], w);
-        }
+        cpx<T> w = tws[n2 * k1];
+        if (S > 0) w.y = -w.y;
+        v[k1] = cmul(v[k1], w);
     }
 #pragma unroll
     for (int k1 = 0; k1 < A; k1++) lds[k1 * RS + n2 * W + p] = v[k1].x;
@@ -455,40 +505,70 @@ __global__ void __launch_bounds__(NT) k_r2c_rows2(const T* __restrict__ x, const
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
-    for (int e = tid; e < M; e += NT) tws[e] = twM[e];
+    // Every global load of the block is issued before the first wait: the
+    // twiddles of both tables into registers, the row data from clamped
+    // (always valid) addresses with the out-of-range lanes zeroed after.
+    constexpr int TA = (M + NT - 1) / NT, TB = (M + 1 + NT - 1) / NT;
+    cpx<T> ta[TA], tb[TB];
+#pragma unroll
+    for (int q = 0; q < TA; q++) ta[q] = twM[tid + q * NT < M ? tid + q * NT : 0];
+#pragma unroll
+    for (int q = 0; q < TB; q++) tb[q] = twP[tid + q * NT <= M ? tid + q * NT : 0];
     {  // phase 1: thread (n2, row p) gathers z[B n1 + n2] = x[2n] + i x[2n+1]
         const int nb = tid % B, p = tid / B;
         const long r = row0 + p;
         const bool rin = r < nrows;
-        const T* xr = x + r * n2;
-        const T* mr = msk ? msk + (r % mrows) * n2 : nullptr;
+        const long rc = rin ? r : row0;  // a valid row for the lanes past the end
+        const T* xr = x + rc * n2;
+        const T* mr = msk ? msk + (rc % mrows) * n2 : nullptr;
         cpx<T> v[A];
+        if ((n2 & 1) == 0) {  // uniform: pairs of reals as one 8- or 16-byte word
+            using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+            V2 q[A];
 #pragma unroll
-        for (int n1 = 0; n1 < A; n1++) {
-            const int n = B * n1 + nb;
-            cpx<T> z = {T(0), T(0)};
-            if (rin && (n2 & 1) == 0 && 2 * n < n2) {
-                using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-                const V2 q = *(const V2*)(xr + 2 * n);
-                z = {T(q.x), T(q.y)};
-                if (mr) {
-                    const V2 w = *(const V2*)(mr + 2 * n);
-                    z.x *= T(w.x);
-                    z.y *= T(w.y);
-                }
-            } else if (rin && (n2 & 1)) {
-                if (2 * n < n2) z.x = xr[2 * n] * (mr ? mr[2 * n] : T(1));
-                if (2 * n + 1 < n2) z.y = xr[2 * n + 1] * (mr ? mr[2 * n + 1] : T(1));
+            for (int n1 = 0; n1 < A; n1++) {
+                const int n = B * n1 + nb;
+                q[n1] = *(const V2*)(xr + 2 * (2 * n < n2 ? n : 0));
             }
-            v[n1] = z;
+            if (mr) {  // uniform
+#pragma unroll
+                for (int n1 = 0; n1 < A; n1++) {
+                    const int n = B * n1 + nb;
+                    const V2 w = *(const V2*)(mr + 2 * (2 * n < n2 ? n : 0));
+                    q[n1].x *= w.x;
+                    q[n1].y *= w.y;
+                }
+            }
+#pragma unroll
+            for (int n1 = 0; n1 < A; n1++) {
+                const int n = B * n1 + nb;
+                const bool ok = rin && 2 * n < n2;
+                v[n1] = {ok ? T(q[n1].x) : T(0), ok ? T(q[n1].y) : T(0)};
+            }
+        } else {
+#pragma unroll
+            for (int n1 = 0; n1 < A; n1++) {
+                const int n = B * n1 + nb;
+                cpx<T> z = {T(0), T(0)};
+                if (rin) {
+                    if (2 * n < n2) z.x = xr[2 * n] * (mr ? mr[2 * n] : T(1));
+                    if (2 * n + 1 < n2) z.y = xr[2 * n + 1] * (mr ? mr[2 * n + 1] : T(1));
+                }
+                v[n1] = z;
+            }
         }
+#pragma unroll
+        for (int q = 0; q < TA; q++)
+            if (tid + q * NT < M) tws[tid + q * NT] = ta[q];
         __syncthreads();  // twiddles visible
         dft_reg<T, A, -1>(v);
 #pragma unroll
         for (int k1 = 0; k1 < A; k1++) lds[(p * A + k1) * BP + nb] = (nb * k1) ? cmul(v[k1], tws[nb * k1]) : v[k1];
     }
     __syncthreads();  // phase 1 done with twM: the table takes twP for the split
-    for (int e = tid; e <= M; e += NT) tws[e] = twP[e];
+#pragma unroll
+    for (int q = 0; q < TB; q++)
+        if (tid + q * NT <= M) tws[tid + q * NT] = tb[q];
     cpx<T> u[B];
     const int k1 = tid % A, p2 = tid / A;
     if (tid < W * A) {  // phase 2: thread (k1, row) gathers Y[k1][n2]
@@ -515,8 +595,8 @@ __global__ void __launch_bounds__(NT) k_r2c_rows2(const T* __restrict__ x, const
     }
 }
 
-template <typename T, int M, int W, int NT = W * split2(M).B>
-__global__ void __launch_bounds__(NT) k_c2r_rows2(const cpx<T>* __restrict__ in, T* __restrict__ x, long nrows,
+template <typename T, int M, int W, int MINW = 1, int NT = W * split2(M).B>
+__global__ void __launch_bounds__(NT, MINW) k_c2r_rows2(const cpx<T>* __restrict__ in, T* __restrict__ x, long nrows,
                                                   int n2, int H2p, const cpx<T>* __restrict__ twM,
                                                   const cpx<T>* __restrict__ twP, Live lv) {
     constexpr int A = split2(M).A, B = split2(M).B, BP = B + 1;
@@ -527,12 +607,12 @@ __global__ void __launch_bounds__(NT) k_c2r_rows2(const cpx<T>* __restrict__ in,
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
-    for (int e = tid; e < M; e += NT) {
-        tws[e] = twM[e];
-        twp[e] = twP[e];
-    }
-    {  // loads in equal batches of at most 24 per thread (no short last batch), each batch in flight before its LDS stores
+    {  // every global load in flight before the first wait: spectrum rows in
+       // equal batches of at most 24 per thread from clamped (valid) addresses,
+       // the twiddles of both tables into registers
         constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, NB = (IT + 23) / 24, CH = (IT + NB - 1) / NB;
+        constexpr int TA = (M + NT - 1) / NT;
+        cpx<T> ta[TA], tb[TA];
         for (int q0 = 0; q0 < IT; q0 += CH) {
             cpx<T> g[CH];
 #pragma unroll
@@ -540,13 +620,30 @@ __global__ void __launch_bounds__(NT) k_c2r_rows2(const cpx<T>* __restrict__ in,
                 const int e = tid + (q0 + q) * NT;
                 const int p = e / (M + 1), k = e % (M + 1);
                 const long r = row0 + p;
-                g[q] = {T(0), T(0)};
-                if (e < NE && r < nrows) g[q] = in[r * H2p + k];
+                const bool ok = e < NE && r < nrows;
+                g[q] = in[ok ? r * H2p + k : row0 * H2p];
+                if (!ok) g[q] = {T(0), T(0)};
+            }
+            if (q0 == 0) {
+#pragma unroll
+                for (int q = 0; q < TA; q++) {
+                    const int e = tid + q * NT;
+                    ta[q] = twM[e < M ? e : 0];
+                    tb[q] = twP[e < M ? e : 0];
+                }
             }
 #pragma unroll
             for (int q = 0; q < CH; q++) {
                 const int e = tid + (q0 + q) * NT;
                 if (e < NE) lds[e] = g[q];  // e = p (M + 1) + k
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TA; q++) {
+            const int e = tid + q * NT;
+            if (e < M) {
+                tws[e] = ta[q];
+                twp[e] = tb[q];
             }
         }
     }
@@ -721,8 +818,8 @@ __device__ __forceinline__ void zconv_half(T* lds, const TK* __restrict__ kern, 
         }
     }
 }
-template <typename T, typename TK, int L, int W>
-__global__ void __launch_bounds__(6 * W) k_zconv2s(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+template <typename T, typename TK, int L, int W, int MINW = 1>
+__global__ void __launch_bounds__(6 * W, MINW) k_zconv2s(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
                                                    int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
                                                    Live lv) {
     constexpr int WP = 3 * W, W2 = 2 * WP;
@@ -811,18 +908,31 @@ __global__ void __launch_bounds__(6 * W) k_zconv2n(cpx<T>* X, const TK* __restri
         for (int t = 0; t < L; t++) lds[t * W2 + tid] = v[t];
     }
     __syncthreads();
+    // the product entries of this thread, e = tid + q NT, unrolled with
+    // unconditional kernel loads (clamped to a valid column) so that all of
+    // them are in flight together instead of one iteration at a time
     const long kplane = (long)(L / 2 + 1) * K1 * kp;
-    for (int e = tid; e < L * W; e += NT) {
-        const int t = e / W, pp = e % W;
-        const int col = col0 + pp;
-        if (col >= H2) continue;
+    constexpr int NE = L * W, IT = (NE + NT - 1) / NT;
+    T kv[IT][6];
+#pragma unroll
+    for (int q = 0; q < IT; q++) {
+        const int e = tid + q * NT;
+        const int t = e < NE ? e / W : 0, pp = e % W;
+        const bool ok = e < NE && col0 + pp < H2;
         const int ii = (t > L / 2) ? L - t : t;
+        const long K = ((long)ii * K1 + jp) * kp + (ok ? col0 + pp : col0);
+#pragma unroll
+        for (int k = 0; k < 6; k++) kv[q][k] = T(kern[k * kplane + K]);
+    }
+#pragma unroll
+    for (int q = 0; q < IT; q++) {
+        const int e = tid + q * NT;
+        const int t = e < NE ? e / W : 0, pp = e % W;
+        const bool ok = e < NE && col0 + pp < H2;
         const T si = (t > L / 2) ? T(-1) : T(1);
-        const long K = ((long)ii * K1 + jp) * kp + col;
-        const T kxx = T(kern[K]), kyy = T(kern[kplane + K]), kzz = T(kern[2 * kplane + K]);
-        const T kyz0 = T(kern[3 * kplane + K]);
-        const T kxz = si * T(kern[4 * kplane + K]);
-        const T kxy0 = si * T(kern[5 * kplane + K]);
+        const T kxx = kv[q][0], kyy = kv[q][1], kzz = kv[q][2], kyz0 = kv[q][3];
+        const T kxz = si * kv[q][4], kxy0 = si * kv[q][5];
+        if (!ok) continue;
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
             const T sj = s2 ? T(-1) : T(1);
@@ -855,96 +965,3 @@ __global__ void __launch_bounds__(6 * W) k_zconv2n(cpx<T>* X, const TK* __restri
     }
 }
 
-// k_zconv2n with the block's kernel values fetched into registers right
-// after the pencils: the 3x3 product then reads no global memory, so a
-// block waits on memory once, at its start, instead of once more per
-// product iteration after the first barrier.  Thread tid's product entries
-// are e = tid + q NT (q < IT), as in k_zconv2n's product loop.
-template <typename T, typename TK, int L, int W>
-__global__ void __launch_bounds__(6 * W) k_zconv2k(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
-                                                   int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
-                                                   Live lv) {
-    typedef T v2 __attribute__((ext_vector_type(2)));
-    constexpr int WP = 3 * W, W2 = 2 * WP, NT = 6 * W;
-    constexpr int NE = L * W, IT = (NE + NT - 1) / NT;
-    static_assert(reg_dft_ok(L), "register DFT length");
-    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
-    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
-    __shared__ cpx<T> lds[L * W2];
-    const int tid = threadIdx.x;
-    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int jp = lb / gridDim.x;
-    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;
-    const int col0 = (lb % gridDim.x) * W;
-    const int p = tid % W, c = (tid / W) % 3, sm = tid / WP;
-    const int j = sm ? jm : jp;
-    const bool live = j >= 0 && col0 + p < H2;
-    const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
-    cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p;
-    const long kplane = (long)(L / 2 + 1) * K1 * kp;
-    T kv[IT][6];
-    {
-        cpx<T> v[L];
-#pragma unroll
-        for (int t = 0; t < L; t++) {
-            cpx<T> x = {T(0), T(0)};
-            if (live && t < N0) {
-                const v2 q = __builtin_nontemporal_load((const v2*)(base + t * tstride));
-                x = {q.x, q.y};
-            }
-            v[t] = x;
-        }
-#pragma unroll
-        for (int q = 0; q < IT; q++) {
-            const int e = tid + q * NT;
-            const bool ok = e < NE && col0 + e % W < H2;
-            const int t = ok ? e / W : 0, col = ok ? col0 + e % W : 0;
-            const int ii = (t > L / 2) ? L - t : t;
-            const long K = ((long)ii * K1 + jp) * kp + col;
-#pragma unroll
-            for (int k = 0; k < 6; k++) kv[q][k] = ok ? T(kern[k * kplane + K]) : T(0);
-        }
-        dft_reg<T, L, -1>(v);
-#pragma unroll
-        for (int t = 0; t < L; t++) lds[t * W2 + tid] = v[t];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < IT; q++) {
-        const int e = tid + q * NT;
-        const int t = e / W, pp = e % W;
-        if (e >= NE || col0 + pp >= H2) continue;
-        const T si = (t > L / 2) ? T(-1) : T(1);
-        const T kxx = kv[q][0], kyy = kv[q][1], kzz = kv[q][2], kyz0 = kv[q][3];
-        const T kxz = si * kv[q][4], kxy0 = si * kv[q][5];
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) {
-            const T sj = s2 ? T(-1) : T(1);
-            const T kyz = sj * kyz0, kxy = sj * kxy0;
-            cpx<T>* a = lds + t * W2 + s2 * WP + pp;
-            cpx<T> mx = cscale(a[0], srcMul);
-            cpx<T> my = cscale(a[W], srcMul);
-            cpx<T> mz = cscale(a[2 * W], srcMul);
-            a[0] = {kxx * mx.x + kxy * my.x + kxz * mz.x, kxx * mx.y + kxy * my.y + kxz * mz.y};
-            a[W] = {kxy * mx.x + kyy * my.x + kyz * mz.x, kxy * mx.y + kyy * my.y + kyz * mz.y};
-            a[2 * W] = {kxz * mx.x + kyz * my.x + kzz * mz.x, kxz * mx.y + kyz * my.y + kzz * mz.y};
-        }
-    }
-    __syncthreads();
-    {
-        cpx<T> v[L];
-#pragma unroll
-        for (int t = 0; t < L; t++) v[t] = lds[t * W2 + tid];
-        dft_reg<T, L, +1>(v);
-        if (live) {
-#pragma unroll
-            for (int t = 0; t < L; t++)
-                if (t < N0) {
-                    v2 q;
-                    q.x = v[t].x;
-                    q.y = v[t].y;
-                    __builtin_nontemporal_store(q, (v2*)(base + t * tstride));
-                }
-        }
-    }
-}

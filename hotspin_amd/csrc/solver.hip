@@ -196,6 +196,49 @@ __device__ __forceinline__ void load_tile(const Phys& P, const float* m, long N,
     }
 }
 
+// The same tile in two halves, so that a kernel can issue the halo loads
+// together with its other loads and write the LDS tile after one wait:
+// load_halo reads the halo values of an edge lane into registers, store_tile
+// writes the tile.
+struct Halo {
+    float y[3], x[3], ly, lx;
+};
+__device__ __forceinline__ void load_halo(const Phys& P, const float* m, long N, int i, int bx, int by, int jc, int kc,
+                                          int tx, int ty, Halo& h) {
+    const int N1 = P.N1, N2 = P.N2;
+    const long N12 = (long)N1 * N2;
+    const long base = i * N12;
+    if (!P.exchange) return;
+    if (ty == 0 || ty == TY - 1) {
+        int jn = (ty == 0) ? wrapc(by * TY - 1, N1, P.w1) : wrapc(min(by * TY + TY, N1), N1, P.w1);
+        long a2 = base + (long)jn * N2 + kc;
+        for (int c = 0; c < 3; c++) h.y[c] = m[c * N + a2];
+        if (P.lex) h.ly = P.lex[a2];
+    }
+    if (tx == 0 || tx == TX - 1) {
+        int kn = (tx == 0) ? wrapc(bx * TX - 1, N2, P.w2) : wrapc(min(bx * TX + TX, N2), N2, P.w2);
+        long a2 = base + (long)jc * N2 + kn;
+        for (int c = 0; c < 3; c++) h.x[c] = m[c * N + a2];
+        if (P.lex) h.lx = P.lex[a2];
+    }
+}
+__device__ __forceinline__ void store_tile(const Phys& P, int tx, int ty, const float own[3], float lown,
+                                           const Halo& h, float (*tile)[TY + 2][TX + 2], float (*ltile)[TX + 2]) {
+    for (int c = 0; c < 3; c++) tile[c][ty + 1][tx + 1] = own[c];
+    if (P.lex) ltile[ty + 1][tx + 1] = lown;
+    if (!P.exchange) return;
+    if (ty == 0 || ty == TY - 1) {
+        int row = (ty == 0) ? 0 : TY + 1;
+        for (int c = 0; c < 3; c++) tile[c][row][tx + 1] = h.y[c];
+        if (P.lex) ltile[row][tx + 1] = h.ly;
+    }
+    if (tx == 0 || tx == TX - 1) {
+        int col = (tx == 0) ? 0 : TX + 1;
+        for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = h.x[c];
+        if (P.lex) ltile[ty + 1][col] = h.lx;
+    }
+}
+
 // 6-neighbour exchange field of cell (i,j,k), exchange6.cu:11-127 in FP32
 // with the differences formed first: pre_d (pre1 (m1-m0) + pre2 (m2-m0)).
 // lo/hi: planes i-1, i+1 (registers); the j,k neighbours come from the tile
@@ -812,7 +855,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
             lhi = P.lex[Ihi];
         }
     }
-    load_tile(P, m, N, i, bx, by, jc, kc, tx, ty, m0, l0, tile, ltile);
+    Halo hl{};
+    load_halo(P, m, N, i, bx, by, jc, kc, tx, ty, hl);
+    store_tile(P, tx, ty, m0, l0, hl, tile, ltile);
     __syncthreads();
     float H[3] = {0.f, 0.f, 0.f};
     if (P.exchange && in) exchange_cell<TX>(P, m, N, i, j, k, jc, tx + 1, ty, m0, lo, hi, l0, llo, lhi, tile, ltile, H);

@@ -46,6 +46,7 @@ def test_noalloc_matches_allocating_entry_two_calls():
     _case((8, 48, 96), (3e-9, 3e-9, 3e-9), (0, 0, 0), 1)
     _case((4, 6, 8), (2e-9, 3e-9, 2.5e-9), (0, 0, 0), 2)
     _case((4, 8, 16), (1e-9, 2e-9, 4e-9), (2, 0, 3), 3)
+    _case((7, 8, 10), (2e-9, 2e-9, 2e-9), (0, 0, 0), 4)  # run-time axis-0 length: the P0-row spectrum too
 
 
 def test_noalloc_refuses_meshes_beyond_the_static_array():
