@@ -219,9 +219,10 @@ def serving_leg(nreq, plain_steps=20, check_steps=5):
                  "max_abs_dm_vs_oracle": errs,
                  "oracle_check": "first request of each shape class solved alone in a slot for %d adaptive RK23 "
                                  "steps vs the FP64 oracle" % check_steps,
-                 "latency_note": "latency = done - arrival; the stream offers ~3e11 cell-steps in 10 s, more than "
-                                 "the device serves in that time, so latency is dominated by the queue wait; "
-                                 "service = done - admission",
+                 "latency_note": "latency = done - arrival, queue = admission - arrival, service = done - "
+                                 "admission; the stream offers ~3e11 cell-steps in 10 s, about 12x what the device "
+                                 "serves in that time: requests are admitted within a second and then share the "
+                                 "device, so service time, not queue wait, makes up the latency",
                  "plain_sample": "%d RK23 steps of one request per shape class through hs_solver_run" % plain_steps})
     return summ
 

@@ -82,3 +82,46 @@ __device__ __forceinline__ void brillouin_w_dw(T S, T x, T& W, T& dW) {
         dW = (gpre2 - lpre2) + lpre2 * cl * cl - gpre2 * cg * cg;
     }
 }
+
+// Ridders' root bracketing as the reference's findroot_Ridders and
+// findroot_Ridders_Ts (brillouin.cu:13-76, Ts.cu:13-64) run it: the same
+// update sequence, |f| < HS_ZERO at an end point returns it, and the
+// iteration stops when the relative change of x drops below HS_EPS or after
+// 1000 iterations.  f: the model as a device callable of x.
+template <typename F>
+__device__ double ridders_root(F f, double xa, double xb) {
+    auto sgn = [](double x) { return x == 0.0 ? 0.0 : (signbit(x) == 0 ? 1.0 : -1.0); };
+    double ya = f(xa);
+    if (fabs(ya) < HS_ZERO) return xa;
+    double yb = f(xb);
+    if (fabs(yb) < HS_ZERO) return xb;
+    double y1 = ya, x1 = xa, y2 = yb, x2 = xb;
+    double x = 1.0e10, tx = x, teps = x;
+    int iter = 0;
+    while (teps > HS_EPS && iter < 1000) {
+        double x3 = 0.5 * (x2 + x1);
+        double y3 = f(x3);
+        double dy = (y3 * y3 - y1 * y2);
+        if (dy == 0.0) {
+            x = x3;
+            break;
+        }
+        double dx = (x3 - x1) * sgn(y1 - y2) * y3 / sqrt(dy);
+        x = x3 + dx;
+        double y = f(x);
+        bool sy = signbit(y) != 0, sy3 = signbit(y3) != 0, sy1 = signbit(y1) != 0;
+        double ny2 = (sy == sy3) ? y2 : y3, nx2 = (sy == sy3) ? x2 : x3;
+        y2 = ny2;
+        x2 = nx2;
+        ny2 = (sy == sy1 || x2 == x3) ? y2 : y1;
+        nx2 = (sy == sy1 || x2 == x3) ? x2 : x1;
+        y2 = ny2;
+        x2 = nx2;
+        y1 = y;
+        x1 = x;
+        teps = fabs((x - tx) / (tx + x));
+        tx = x;
+        iter++;
+    }
+    return x;
+}

@@ -350,41 +350,8 @@ __device__ double d_model(double n, double J, double pre) {
     brillouin_w_dw(J, pre * n, W, dW);
     return W - n;
 }
-__device__ double d_sign(double x) {
-    double v = signbit(x) == 0 ? 1.0 : -1.0;
-    return x == 0.0 ? 0.0 : v;
-}
 __device__ double d_ridders(double J, double mult, double xa, double xb) {
-    double ya = d_model(xa, J, mult);
-    if (fabs(ya) < HS_ZERO) return xa;
-    double yb = d_model(xb, J, mult);
-    if (fabs(yb) < HS_ZERO) return xb;
-    double y1 = ya, x1 = xa, y2 = yb, x2 = xb;
-    double x = 1.0e10, tx = x, teps = x;
-    int iter = 0;
-    while (teps > HS_EPS && iter < 1000) {
-        double x3 = 0.5 * (x2 + x1);
-        double y3 = d_model(x3, J, mult);
-        double dy = (y3 * y3 - y1 * y2);
-        if (dy == 0.0) {
-            x = x3;
-            break;
-        }
-        double dx = (x3 - x1) * d_sign(y1 - y2) * y3 / sqrt(dy);
-        x = x3 + dx;
-        double y = d_model(x, J, mult);
-        bool sy = signbit(y) != 0, sy3 = signbit(y3) != 0, sy1 = signbit(y1) != 0;
-        double ny2 = (sy == sy3) ? y2 : y3, nx2 = (sy == sy3) ? x2 : x3;
-        y2 = ny2; x2 = nx2;
-        ny2 = (sy == sy1 || x2 == x3) ? y2 : y1;
-        nx2 = (sy == sy1 || x2 == x3) ? x2 : x1;
-        y2 = ny2; x2 = nx2;
-        y1 = y; x1 = x;
-        teps = fabs((x - tx) / (tx + x));
-        tx = x;
-        iter++;
-    }
-    return x;
+    return ridders_root([=](double n) { return d_model(n, J, mult); }, xa, xb);
 }
 __global__ void k_brillouin(double* msat0, const double* msMsk, const double* T, const double* TcMsk,
                             const double* SMsk, double msat0Mul, double msMul, double TcMul, double SMul, int N) {

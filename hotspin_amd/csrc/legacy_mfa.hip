@@ -50,11 +50,6 @@ __device__ double bj_only(double S, double x) {
     return gpre / tanh(gpre * x) - lpre / tanh(lpre * x);
 }
 
-__device__ double sgn(double x) {
-    double v = signbit(x) == 0 ? 1.0 : -1.0;
-    return x == 0.0 ? 0.0 : v;
-}
-
 // findroot_Ridders_Ts with ModelTs (Ts.cu:13-82): the temperature T in [0, Tc]
 // with B_S(pre/T) = m.  Same update sequence as brillouinAsync's root finder.
 __device__ double ts_model(double T, double S, double pre, double m) {
@@ -62,36 +57,7 @@ __device__ double ts_model(double T, double S, double pre, double m) {
     return bj_only(S, x) - m;
 }
 __device__ double ts_root(double S, double pre, double m, double xa, double xb) {
-    double ya = ts_model(xa, S, pre, m);
-    if (fabs(ya) < HS_ZERO) return xa;
-    double yb = ts_model(xb, S, pre, m);
-    if (fabs(yb) < HS_ZERO) return xb;
-    double y1 = ya, x1 = xa, y2 = yb, x2 = xb;
-    double x = 1.0e10, tx = x, teps = x;
-    int iter = 0;
-    while (teps > HS_EPS && iter < 1000) {
-        double x3 = 0.5 * (x2 + x1);
-        double y3 = ts_model(x3, S, pre, m);
-        double dy = (y3 * y3 - y1 * y2);
-        if (dy == 0.0) {
-            x = x3;
-            break;
-        }
-        double dx = (x3 - x1) * sgn(y1 - y2) * y3 / sqrt(dy);
-        x = x3 + dx;
-        double y = ts_model(x, S, pre, m);
-        bool sy = signbit(y) != 0, sy3 = signbit(y3) != 0, sy1 = signbit(y1) != 0;
-        double ny2 = (sy == sy3) ? y2 : y3, nx2 = (sy == sy3) ? x2 : x3;
-        y2 = ny2; x2 = nx2;
-        ny2 = (sy == sy1 || x2 == x3) ? y2 : y1;
-        nx2 = (sy == sy1 || x2 == x3) ? x2 : x1;
-        y2 = ny2; x2 = nx2;
-        y1 = y; x1 = x;
-        teps = fabs((x - tx) / (tx + x));
-        tx = x;
-        iter++;
-    }
-    return x;
+    return ridders_root([=](double T) { return ts_model(T, S, pre, m); }, xa, xb);
 }
 
 // Debye integrand x^4 e^x / (e^x - 1)^2, 0 at x <= 1e-32 (common_func.h:168-173).

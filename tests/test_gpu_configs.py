@@ -81,7 +81,7 @@ def test_laser_film_rhs_and_coupled_steps():
         print("configs: laser step %d dt %.3e" % (i, dt_in), flush=True)
         dtg, stg = g.run("rk23", 1, dt_in, maxErr=w["tol"], mindt=1e-20, maxdt=5e-14, step0=i)
         assert stg.rejected == int(st[3]) - 1, (i, stg.rejected, st)
-        assert abs(dtg / dt - 1) <= 2e-2, (i, dtg, dt)
+        assert abs(dtg / dt - 1) <= 1e-3, (i, dtg, dt)
         assert abs(g.time - t) <= 1e-9 * t
         teg, tpg = g.get_T()
         assert np.abs(g.get_m() - mo).max() <= 1e-5, i

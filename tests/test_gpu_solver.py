@@ -141,7 +141,7 @@ def test_llb_rk23_adaptive_matches_oracle(masks):
         g.set_m(m_before)
         dtg, stg = g.run("rk23", 1, dt_in, maxErr=1e-5, mindt=1e-18, maxdt=1e-12, step0=i)
         assert stg.rejected == tries - 1, (i, stg.rejected, tries)
-        assert abs(dtg / dt - 1) <= 2e-2, (i, dtg, dt)
+        assert abs(dtg / dt - 1) <= 2e-3, (i, dtg, dt)  # observed up to 6.4e-4
         assert np.abs(g.get_m() - mo).max() <= 1e-5, i
 
 

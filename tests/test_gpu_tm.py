@@ -88,7 +88,7 @@ def test_coupled_rk23_replays_oracle_steps(extra):
         g.time = t_b
         dtg, stg = g.run("rk23", 1, dt_in, maxErr=1e-5, mindt=1e-20, maxdt=5e-14, step0=i)
         assert stg.rejected == int(st[3]) - 1, (i, stg.rejected, st)
-        assert abs(dtg / dt - 1) <= 2e-2, (i, dtg, dt)
+        assert abs(dtg / dt - 1) <= 1e-2, (i, dtg, dt)  # observed up to 6.6e-3 (spin heating on)
         assert abs(g.time - t) <= 1e-9 * t
         teg, tpg = g.get_T()
         tb = 1e-6 + (2e-5 * np.abs(Te - Te_b).max() if extra else 0.0)
@@ -131,7 +131,7 @@ def test_coupled_am12_matches_oracle_step_by_step(extra):
         # level, so err, and the proposal, are good to ~10% there (perturbing the oracle's
         # Qs by 1e-6 relative moves those proposals by 1-4%)
         floor = extra and min(st[:3]) < 1e-4
-        assert abs(dtg - dt) <= (0.15 if floor else 1e-3) * dt, (i, dtg, dt, st)
+        assert abs(dtg - dt) <= (0.1 if floor else 1e-3) * dt, (i, dtg, dt, st)  # observed 7.1e-2 / 7.8e-4
         teg, tpg = g.get_T()
         tb = 1e-6 + (2e-5 * np.abs(Te - Te_b).max() if extra else 0.0)
         assert np.abs(g.get_m() - mo).max() <= 1e-5, i
