@@ -93,7 +93,12 @@ struct Phys {
     float aMul[3];
     float sqa[3];  // sqrt(aMul[c]): the thermal amplitude of a component without an alpha mask
     const float* alpha[3];
-    LfConst lf;
+    // The uniform longitudinal field's constants sit in device memory and are
+    // loaded inside the branch that uses them: as kernel arguments their 11
+    // doubles stay live in scalar registers through the whole stage kernel,
+    // which then spills scalar registers into vector lanes.
+    int lf_uniform;
+    const LfConst* lfc;
     double sh_w, sh_q;  // spin heating: 3 kB TcMul nMul gammaLL and mu0 Ms0T0Mul gammaLL
 };
 
@@ -368,7 +373,7 @@ __device__ __forceinline__ void preload(const Phys& P, const StageIO& io, long I
 
 // Everything after the exchange term for one cell: B/mu0, B_ext/mu0, H_lf,
 // H_therm and k = gamma (m x H + alpha H).
-__device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, const StageIO& io, long I,
+__device__ __forceinline__ void field_torque(const Phys& P, const SolveP* sp, const StageIO& io, long I,
                                              const float m0[3], float H[3], const Pre& q, float kk[3]) {
     const float ms = q.ms;
     if (P.demag) {
@@ -377,12 +382,12 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
         H[2] += P.invMu0 * q.B[2];
     }
     if (P.zeeman) {
-        H[0] += sp.bext[0];
-        H[1] += sp.bext[1];
-        H[2] += sp.bext[2];
+        H[0] += sp->bext[0];
+        H[1] += sp->bext[1];
+        H[2] += sp->bext[2];
     }
-    if (P.longfield && P.lf.uniform) {
-        const LfConst& C = P.lf;
+    if (P.longfield && P.lf_uniform) {
+        const LfConst C = *P.lfc;
         double mx = m0[0], my = m0[1], mz = m0[2];
         double am2 = mx * mx + my * my + mz * mz;
         if (!(ms == 0.f || am2 <= HS_ZERO * HS_ZERO)) {
@@ -396,15 +401,21 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
                 am = sqrt(am2);
                 ia = 1.0 / am;
             }
-            double x = am * sp.lf_b;
-            double W, dW;
+            double x = am * sp->lf_b;
+            // H_lf = c1 (W - |m|) / (ms dW) m/|m| as num / den: one FP64 division
+            double num, den;
             if (x < C.lim) {
                 double x2 = x * x;
-                W = x * (C.a1 - x2 * (C.a3 - x2 * (C.a5 - x2 * C.a7)));
-                dW = C.d0 - x2 * (C.d2 - x2 * (C.d4 - x2 * C.d6));
+                double W = x * (C.a1 - x2 * (C.a3 - x2 * (C.a5 - x2 * C.a7)));
+                num = W - am;
+                den = C.d0 - x2 * (C.d2 - x2 * (C.d4 - x2 * C.d6));
             } else if (!C.langevin) {
                 // coth(gx), coth(lx) through e^{-2gx}, e^{-2lx}; g = (2S+1) l, so for
-                // integer 2S+1 the first is a power of the second (one exp)
+                // integer 2S+1 the first is a power of the second (one exp).  With
+                // a = 1 - e^{-2gx}, b = 1 - e^{-2lx}:  W = Wn / (ab),  dW = dWn / (ab)^2,
+                // Wn = g (1 + e^{-2gx}) b - l (1 + e^{-2lx}) a,
+                // dWn = 4 l^2 e^{-2lx} a^2 - 4 g^2 e^{-2gx} b^2,
+                // so (W - |m|) / dW = (Wn - |m| ab) ab / dWn.
                 double el = exp(-2.0 * C.l * x), eg;
                 if (C.gpow) {
                     eg = el;
@@ -412,16 +423,17 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
                 } else {
                     eg = exp(-2.0 * C.g * x);
                 }
-                double inv = 1.0 / ((1.0 - eg) * (1.0 - el));
-                double ig = (1.0 - el) * inv, il = (1.0 - eg) * inv;
-                W = C.g * (1.0 + eg) * ig - C.l * (1.0 + el) * il;
-                dW = 4.0 * (C.l * C.l) * el * (il * il) - 4.0 * (C.g * C.g) * eg * (ig * ig);
+                const double a = 1.0 - eg, b = 1.0 - el, D = a * b;
+                num = (C.g * (1.0 + eg) * b - C.l * (1.0 + el) * a - am * D) * D;
+                den = 4.0 * (C.l * C.l) * el * (a * a) - 4.0 * (C.g * C.g) * eg * (b * b);
             } else {
-                double e = exp(-2.0 * x), i1 = 1.0 / (1.0 - e);
-                W = (1.0 + e) * i1 - 1.0 / x;
-                dW = 1.0 / (x * x) - 4.0 * e * (i1 * i1);
+                // L(x) = coth x - 1/x, L' = 1/x^2 - 1/sinh^2 x; with a = 1 - e^{-2x}:
+                // (L - |m|) / L' = ((1 + e^{-2x}) x - a - |m| a x) a x / (a^2 - 4 e^{-2x} x^2)
+                const double e = exp(-2.0 * x), a = 1.0 - e;
+                num = ((1.0 + e) * x - a - am * a * x) * (a * x);
+                den = a * a - 4.0 * e * (x * x);
             }
-            double mult = sp.lf_c1 * (W - am) * ia / ((double)ms * dW);
+            double mult = sp->lf_c1 * num * ia / ((double)ms * den);
             H[0] += (float)(mult * mx);
             H[1] += (float)(mult * my);
             H[2] += (float)(mult * mz);
@@ -432,7 +444,7 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
         double S = (double)P.JMul * q.J;
         double n = (double)P.nMul * q.n;
         double Tc = (double)P.TcMul * q.Tc;
-        double Ts = (double)sp.TMul * q.T;
+        double Ts = (double)sp->TMul * q.T;
         double mx = m0[0], my = m0[1], mz = m0[2];
         double am = sqrt(mx * mx + my * my + mz * mz);
         if (!(Ms0T0 == 0.0 || n == 0.0 || am <= HS_ZERO)) {
@@ -448,9 +460,9 @@ __device__ __forceinline__ void field_torque(const Phys& P, const SolveP& sp, co
     }
     if (P.thermal && ms != 0.f) {
         float x0, x1, x2;
-        thermal_normals_hw((uint32_t)I, io.step, io.attempt, sp.stream, sp.seed, x0, x1, x2);
+        thermal_normals_hw((uint32_t)I, io.step, io.attempt, sp->stream, sp->seed, x0, x1, x2);
         const float rdt = __builtin_amdgcn_rsqf((float)fmax((double)P.cutoff_dt, io.h));
-        float pre = __builtin_amdgcn_sqrtf(q.T * __builtin_amdgcn_rcpf(ms)) * (sp.sthC * rdt);
+        float pre = __builtin_amdgcn_sqrtf(q.T * __builtin_amdgcn_rcpf(ms)) * (sp->sthC * rdt);
         float s0 = P.alpha[0] ? __builtin_amdgcn_sqrtf(P.aMul[0] * q.al[0]) : P.sqa[0];
         float s1 = P.alpha[1] ? __builtin_amdgcn_sqrtf(P.aMul[1] * q.al[1]) : P.sqa[1];
         float s2 = P.alpha[2] ? __builtin_amdgcn_sqrtf(P.aMul[2] * q.al[2]) : P.sqa[2];
@@ -705,7 +717,7 @@ __device__ __forceinline__ void store_out2(const Phys& P, const StageIO& io, lon
 }
 
 template <int ROLE>
-__device__ __forceinline__ void finish_cell(const Phys& P, const SolveP& sp, const StageIO& io, long I, long N,
+__device__ __forceinline__ void finish_cell(const Phys& P, const SolveP* sp, const StageIO& io, long I, long N,
                                             const float m0[3], float H[3], const Pre& q, double& errloc) {
     float kk[3];
     field_torque(P, sp, io, I, m0, H, q, kk);
@@ -748,7 +760,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
     const int sol = blockIdx.z / nchunk;
     if (solve_idle(io0, sol)) return;
     const StageIO io = solve_io(io0, sol, N);
-    const SolveP sp = P.sp[sol];
+    // per-solve constants read where they are used (a copy here would hold
+    // them all in scalar registers through the kernel)
+    const SolveP* sp = P.sp + sol;
     const bool in = (k < N2) && (j < N1);
     const float* m = io.m;
     const int kc = min(k, N2 - 1), jc = min(j, N1 - 1);
@@ -811,7 +825,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(Phys P, StageIO io0) {
 // other; the m values of planes i-1 and i+1 that a block reads were loaded
 // into that XCD's L2 by the blocks just before and after it.
 template <int ROLE, bool ZORD = false>
-__global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
+__global__ void __launch_bounds__(TX* TY, 6) k_stage_flat(Phys P, StageIO io0) {
     __shared__ float tile[3][TY + 2][TX + 2];
     __shared__ float ltile[TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -831,7 +845,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage_flat(Phys P, StageIO io0) {
     const long N12 = (long)N1 * N2;
     const long N = (long)N0 * N12;
     const StageIO io = solve_io(io0, sol, N);
-    const SolveP sp = P.sp[sol];
+    // per-solve constants read where they are used (a copy here would hold
+    // them all in scalar registers through the kernel)
+    const SolveP* sp = P.sp + sol;
     const bool in = (k < N2) && (j < N1);
     const float* m = io.m;
     const int kc = min(k, N2 - 1), jc = min(j, N1 - 1);
@@ -940,7 +956,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage_pair(Phys P, StageIO io0) {
     const long N12 = (long)N1 * N2;
     const long N = (long)N0 * N12;
     const StageIO io = solve_io(io0, sol, N);
-    const SolveP sp = P.sp[sol];
+    // per-solve constants read where they are used (a copy here would hold
+    // them all in scalar registers through the kernel)
+    const SolveP* sp = P.sp + sol;
     const bool in = (k < N2) && (j < N1);  // N2 even: k < N2 implies k + 1 < N2
     const float* m = io.m;
     const int kc = min(k, N2 - 2), jc = min(j, N1 - 1);
@@ -963,6 +981,26 @@ __global__ void __launch_bounds__(TX* TY) k_stage_pair(Phys P, StageIO io0) {
             lhi = ld2(P.lex, Ihi);
         }
     }
+    // halo values into registers with the other loads: every load of the
+    // block is in flight before the first wait (the LDS stores below)
+    float2 hy[3] = {}, hly = make_float2(1.f, 1.f);
+    float hx[3] = {}, hlx = 1.f;
+    const bool yedge = ty == 0 || ty == TY - 1, xedge = tx == 0 || tx == TX - 1;
+    if (P.exchange) {
+        if (yedge) {  // halo rows, two cells per thread
+            const int jn = (ty == 0) ? wrapc(blockIdx.y * TY - 1, N1, P.w1)
+                                     : wrapc(min(blockIdx.y * TY + TY, N1), N1, P.w1);
+            const long a2 = i * N12 + (long)jn * N2 + kc;
+            for (int c = 0; c < 3; c++) hy[c] = ld2(m, c * N + a2);
+            if (P.lex) hly = ld2(P.lex, a2);
+        }
+        if (xedge) {  // halo columns kb-1 and kb+128
+            const int kn = (tx == 0) ? wrapc(kb - 1, N2, P.w2) : wrapc(min(kb + TX2, N2), N2, P.w2);
+            const long a2 = i * N12 + (long)jc * N2 + kn;
+            for (int c = 0; c < 3; c++) hx[c] = m[c * N + a2];
+            if (P.lex) hlx = P.lex[a2];
+        }
+    }
     const int cc = 1 + 2 * tx;  // tile column of cell k
     for (int c = 0; c < 3; c++) {
         tile[c][ty + 1][cc] = m0[c].x;
@@ -973,28 +1011,21 @@ __global__ void __launch_bounds__(TX* TY) k_stage_pair(Phys P, StageIO io0) {
         ltile[ty + 1][cc + 1] = l0.y;
     }
     if (P.exchange) {
-        if (ty == 0 || ty == TY - 1) {  // halo rows, two cells per thread
-            const int jn = (ty == 0) ? wrapc(blockIdx.y * TY - 1, N1, P.w1)
-                                     : wrapc(min(blockIdx.y * TY + TY, N1), N1, P.w1);
+        if (yedge) {
             const int row = (ty == 0) ? 0 : TY + 1;
-            const long a2 = i * N12 + (long)jn * N2 + kc;
             for (int c = 0; c < 3; c++) {
-                float2 v = ld2(m, c * N + a2);
-                tile[c][row][cc] = v.x;
-                tile[c][row][cc + 1] = v.y;
+                tile[c][row][cc] = hy[c].x;
+                tile[c][row][cc + 1] = hy[c].y;
             }
             if (P.lex) {
-                float2 v = ld2(P.lex, a2);
-                ltile[row][cc] = v.x;
-                ltile[row][cc + 1] = v.y;
+                ltile[row][cc] = hly.x;
+                ltile[row][cc + 1] = hly.y;
             }
         }
-        if (tx == 0 || tx == TX - 1) {  // halo columns kb-1 and kb+128
-            const int kn = (tx == 0) ? wrapc(kb - 1, N2, P.w2) : wrapc(min(kb + TX2, N2), N2, P.w2);
+        if (xedge) {
             const int col = (tx == 0) ? 0 : TX2 + 1;
-            const long a2 = i * N12 + (long)jc * N2 + kn;
-            for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = m[c * N + a2];
-            if (P.lex) ltile[ty + 1][col] = P.lex[a2];
+            for (int c = 0; c < 3; c++) tile[c][ty + 1][col] = hx[c];
+            if (P.lex) ltile[ty + 1][col] = hlx;
         }
     }
     __syncthreads();
@@ -1478,6 +1509,7 @@ struct hs_solver {
     long N;       // cells per solve
     int nsolve = 1;
     SolveP* sp = nullptr;      // device, nsolve entries
+    LfConst* lfc = nullptr;    // device, the uniform longitudinal field's constants
     Clock* clk = nullptr;      // device, nsolve entries (graph replay, slots)
     hipGraphExec_t graph = nullptr;
     // slots: one captured RK23 iteration per buffer-set parity
@@ -2238,7 +2270,13 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
     P.Tc = upload_mask(s, p->Tc);
     P.T = upload_mask(s, p->T);
     P.gamma = (float)p->gammaLL;
-    P.lf = lf_constants(p);
+    {
+        const LfConst C = lf_constants(p);
+        P.lf_uniform = C.uniform;
+        hipMalloc(&s->lfc, sizeof(LfConst));
+        hipMemcpy(s->lfc, &C, sizeof(LfConst), hipMemcpyHostToDevice);
+        P.lfc = s->lfc;
+    }
     for (int c = 0; c < 3; c++) {
         P.aMul[c] = (float)p->alphaMul[c];
         P.sqa[c] = sqrtf(P.aMul[c]);
@@ -2277,7 +2315,7 @@ extern "C" void hs_solver_destroy(hs_solver* s) {
     if (s->err_host) hipHostFree(s->err_host);
     if (s->demag) hs_demag_destroy(s->demag);
     void* ps[] = {s->y0, s->acc, s->mA, s->mB, s->mC, s->B0, s->Bs, s->e, s->k1, s->k2, s->k3, s->kout, s->hout,
-                  s->errmax, s->sp, s->clk, s->live, s->imp, s->y1, s->qs, s->epart, s->err_dev};
+                  s->errmax, s->sp, s->lfc, s->clk, s->live, s->imp, s->y1, s->qs, s->epart, s->err_dev};
     for (void* q : ps)
         if (q) hipFree(q);
     for (float* q : s->masks) hipFree(q);
@@ -3279,7 +3317,7 @@ extern "C" hs_solver* hs_solver_create_tm(const hs_llb_params* p, const hs_tm_pa
     if (!s) return nullptr;
     s->tm = true;
     s->tmp = *tm;
-    s->phys.lf.uniform = 0;  // the longitudinal field sees a per-cell temperature
+    s->phys.lf_uniform = 0;  // the longitudinal field sees a per-cell temperature
     TmP& c = s->tmc;
     c.gammaE = tm->gammaE, c.Cl = tm->Cl, c.G = tm->G, c.kappa = tm->kappa, c.Q0 = tm->Q0;
     c.ir2 = tm->sigma_r > 0 ? 1.0 / (2.0 * tm->sigma_r * tm->sigma_r) : 0.0;
