@@ -147,6 +147,51 @@ constexpr Split2 split2(int L) {
 constexpr bool two_level_ok(int L) { return split2(L).A > 0; }
 
 // ---- column pass (F2 / I2) ------------------------------------------------
+// Phase-1 loads of the two-level column passes: thread (n2, column p) takes
+// rows t = B n1 + n2, its own row plus a uniform step per n1.  For each n1 a
+// uniform test decides: every lane's row is an input row (B (n1+1) <= nin),
+// none is (B n1 >= nin: zero), or the lanes differ (a lane past the input
+// rows loads row 0 of column col0 and zeroes the value).  A column past ncol reads column col0: its values stay in
+// their own column of the tile and are never stored.  Every load is in flight
+// before the first wait, and the arrays stay in registers (one unrolled loop
+// of constant indices).
+template <typename T, int A, int B>
+__device__ __forceinline__ void cols2_load(cpx<T> (&v)[A], const cpx<T>* in, long bi, long ts, int n2, int p,
+                                           bool pin, int nin) {
+    const cpx<T>* lin = in + (pin ? bi + (long)n2 * ts + p : bi);
+    const long bstep = (long)B * ts;
+#pragma unroll
+    for (int n1 = 0; n1 < A; n1++) {
+        if (B * (n1 + 1) <= nin) {
+            v[n1] = lin[n1 * bstep];
+        } else if (B * n1 >= nin) {
+            v[n1] = {T(0), T(0)};
+        } else {  // a lane past the input rows reads row 0 of column col0 (always present)
+            const bool ok = B * n1 + n2 < nin;
+            v[n1] = *(ok ? lin + n1 * bstep : in + bi);
+            if (!ok) v[n1] = {T(0), T(0)};
+        }
+    }
+}
+
+// Phase-2 stores: thread (q1, column p) holds rows t = q1 + A k2 (q1 < A),
+// its own row plus a uniform step per k2; rows t >= nkeep are dropped (a
+// uniform test when A (k2+1) <= nkeep or A k2 >= nkeep).
+template <typename T, int A, int B>
+__device__ __forceinline__ void cols2_store(const cpx<T> (&u)[B], cpx<T>* out, long bo, long ts, int q1, int p,
+                                            int nkeep) {
+    cpx<T>* lout = out + bo + (long)q1 * ts + p;
+    const long astep = (long)A * ts;
+#pragma unroll
+    for (int k2 = 0; k2 < B; k2++) {
+        if (A * (k2 + 1) <= nkeep) {
+            lout[k2 * astep] = u[k2];
+        } else if (A * k2 < nkeep) {
+            if (q1 + A * k2 < nkeep) lout[k2 * astep] = u[k2];
+        }
+    }
+}
+
 // Pencil a, columns col0..col0+W; input rows t < nin (zero beyond), output
 // rows t < nkeep; in and out may alias only if they are the same buffer with
 // nin == nkeep == L (each block reads its tile completely before writing).
@@ -229,27 +274,7 @@ __global__ void __launch_bounds__(NT) k_cols2s(const cpx<T>* in, cpx<T>* out, Ge
     // clamped (valid) addresses, zeroed afterwards outside the input rows,
     // then the twiddle table into registers
     cpx<T> v[A];
-    // three straight-line load paths chosen by a uniform branch: all L rows
-    // are input (inverse pass), the first half (forward pass of a padded
-    // axis: the zero half is not read), or any other count
-    auto load = [&](int n1) {
-        const int t = B * n1 + n2;
-        const bool ok = pin && t < nin;
-        v[n1] = in[ok ? bi + (long)t * gi.ts + p : bi];
-        if (!ok) v[n1] = {T(0), T(0)};
-    };
-    if (nin == L) {
-#pragma unroll
-        for (int n1 = 0; n1 < A; n1++) load(n1);
-    } else if (A % 2 == 0 && 2 * nin == L) {
-#pragma unroll
-        for (int n1 = 0; n1 < A / 2; n1++) load(n1);
-#pragma unroll
-        for (int n1 = A / 2; n1 < A; n1++) v[n1] = {T(0), T(0)};
-    } else {
-#pragma unroll
-        for (int n1 = 0; n1 < A; n1++) load(n1);
-    }
+    cols2_load<T, A, B>(v, in, bi, gi.ts, n2, p, pin, nin);
     {
         constexpr int TW = (L + NT - 1) / NT;
         cpx<T> tv[TW];
@@ -287,13 +312,7 @@ __global__ void __launch_bounds__(NT) k_cols2s(const cpx<T>* in, cpx<T>* out, Ge
 #pragma unroll
         for (int m = 0; m < B; m++) u[m].y = lds[(q1 * B + m) * W + p];
         dft_reg<T, B, S>(u);
-        if (pin) {
-#pragma unroll
-            for (int k2 = 0; k2 < B; k2++) {
-                const int t = q1 + A * k2;
-                if (t < nkeep) out[bo + (long)t * go.ts + p] = u[k2];
-            }
-        }
+        if (pin) cols2_store<T, A, B>(u, out, bo, go.ts, q1, p, nkeep);
     }
 }
 
@@ -321,27 +340,7 @@ __global__ void __launch_bounds__(NT, MINW) k_cols2p(const cpx<T>* in, cpx<T>* o
     // clamped (valid) addresses, zeroed afterwards outside the input rows,
     // then the twiddle table into registers
     cpx<T> v[A];
-    // three straight-line load paths chosen by a uniform branch: all L rows
-    // are input (inverse pass), the first half (forward pass of a padded
-    // axis: the zero half is not read), or any other count
-    auto load = [&](int n1) {
-        const int t = B * n1 + n2;
-        const bool ok = pin && t < nin;
-        v[n1] = in[ok ? bi + (long)t * gi.ts + p : bi];
-        if (!ok) v[n1] = {T(0), T(0)};
-    };
-    if (nin == L) {
-#pragma unroll
-        for (int n1 = 0; n1 < A; n1++) load(n1);
-    } else if (A % 2 == 0 && 2 * nin == L) {
-#pragma unroll
-        for (int n1 = 0; n1 < A / 2; n1++) load(n1);
-#pragma unroll
-        for (int n1 = A / 2; n1 < A; n1++) v[n1] = {T(0), T(0)};
-    } else {
-#pragma unroll
-        for (int n1 = 0; n1 < A; n1++) load(n1);
-    }
+    cols2_load<T, A, B>(v, in, bi, gi.ts, n2, p, pin, nin);
     {
         constexpr int TW = (L + NT - 1) / NT;
         cpx<T> tv[TW];
@@ -379,13 +378,7 @@ __global__ void __launch_bounds__(NT, MINW) k_cols2p(const cpx<T>* in, cpx<T>* o
 #pragma unroll
         for (int m = 0; m < B; m++) u[m].y = lds[q1 * RS + m * W + p];
         dft_reg<T, B, S>(u);
-        if (pin) {
-#pragma unroll
-            for (int k2 = 0; k2 < B; k2++) {
-                const int t = q1 + A * k2;
-                if (t < nkeep) out[bo + (long)t * go.ts + p] = u[k2];
-            }
-        }
+        if (pin) cols2_store<T, A, B>(u, out, bo, go.ts, q1, p, nkeep);
     }
 }
 
