@@ -575,16 +575,28 @@ __global__ void __launch_bounds__(NT) k_r2c_rows2(const T* __restrict__ x, const
         for (int k2 = 0; k2 < B; k2++) lds[p2 * M + k1 + A * k2] = u[k2];
     }
     __syncthreads();
-    // split: X[k] = E + w^k O, k = 0..M
-    for (int e = tid; e < (M + 1) * W; e += NT) {
-        const int p = e / (M + 1), k = e % (M + 1);
-        const long r = row0 + p;
-        if (r >= nrows) continue;
+    // split: X[k] = E + w^k O, k = 0..M; row p's k < M by threads k = tid +
+    // q NT (rows in order), then the M-th points of the W rows
+    auto split = [&](int p, int k) {
         cpx<T> zk = lds[p * M + (k % M)];
         cpx<T> zm = lds[p * M + ((M - k) % M)];
         cpx<T> E = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
         cpx<T> O = {T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x)};
-        out[r * H2p + k] = cadd(E, cmul(tws[k], O));
+        out[(row0 + p) * H2p + k] = cadd(E, cmul(tws[k], O));
+    };
+    if (M % NT == 0) {
+#pragma unroll
+        for (int p = 0; p < W; p++)
+            if (row0 + p < nrows) {  // uniform
+#pragma unroll
+                for (int q = 0; q < M / NT; q++) split(p, tid + q * NT);
+            }
+        if (tid < W && row0 + tid < nrows) split(tid, M);
+    } else {
+        for (int e = tid; e < (M + 1) * W; e += NT) {
+            const int p = e / (M + 1), k = e % (M + 1);
+            if (row0 + p < nrows) split(p, k);
+        }
     }
 }
 
@@ -673,7 +685,19 @@ __global__ void __launch_bounds__(NT, MINW) k_c2r_rows2(const cpx<T>* __restrict
 #pragma unroll
         for (int m = 0; m < B; m++) u[m] = lds[(p2 * A + k1) * BP + m];
         dft_reg<T, B, +1>(u);
-        if (r < nrows) {
+        if (r < nrows && n2 == M && B % 2 == 0) {
+            // zero-padded x axis (P2 = 2 n2): the pairs n = k1 + A k2 < n2 / 2 are
+            // exactly k2 < B / 2
+            using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+            T* xr = x + r * n2 + 2 * k1;
+#pragma unroll
+            for (int k2 = 0; k2 < B / 2; k2++) {
+                V2 q;
+                q.x = u[k2].x;
+                q.y = u[k2].y;
+                *(V2*)(xr + 2 * A * k2) = q;
+            }
+        } else if (r < nrows) {
             T* xr = x + r * n2;
 #pragma unroll
             for (int k2 = 0; k2 < B; k2++) {
