@@ -612,7 +612,39 @@ __global__ void __launch_bounds__(NT, MINW) k_c2r_rows2(const cpx<T>* __restrict
     const int tid = threadIdx.x;
     const long row0 = (long)blockIdx.x * W;
     if (all_dead(lv, row0, (row0 + W <= nrows ? row0 + W : nrows) - 1)) return;
-    {  // every global load in flight before the first wait: spectrum rows in
+    if constexpr (M % NT == 0) {
+        // row by row: thread k = tid + q NT of each row (the row's start is
+        // uniform), then the M-th points of the W rows; rows past nrows read
+        // row row0 and are zeroed.  Every global load is in flight before the
+        // first wait; the twiddles of both tables go through registers.
+        constexpr int QR = M / NT;
+        cpx<T> g[W][QR], gm, ta[QR], tb[QR];
+#pragma unroll
+        for (int p = 0; p < W; p++) {
+            const cpx<T>* rp = in + (row0 + p < nrows ? row0 + p : row0) * H2p;
+#pragma unroll
+            for (int q = 0; q < QR; q++) g[p][q] = rp[tid + q * NT];
+        }
+        const bool mrow = tid < W && row0 + tid < nrows;
+        gm = in[(mrow ? row0 + tid : row0) * H2p + M];
+#pragma unroll
+        for (int q = 0; q < QR; q++) {
+            ta[q] = twM[tid + q * NT];
+            tb[q] = twP[tid + q * NT];
+        }
+#pragma unroll
+        for (int p = 0; p < W; p++) {
+            const bool rin = row0 + p < nrows;  // uniform
+#pragma unroll
+            for (int q = 0; q < QR; q++) lds[p * (M + 1) + tid + q * NT] = rin ? g[p][q] : cpx<T>{T(0), T(0)};
+        }
+        if (tid < W) lds[tid * (M + 1) + M] = mrow ? gm : cpx<T>{T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < QR; q++) {
+            tws[tid + q * NT] = ta[q];
+            twp[tid + q * NT] = tb[q];
+        }
+    } else {  // every global load in flight before the first wait: spectrum rows in
        // equal batches of at most 24 per thread from clamped (valid) addresses,
        // the twiddles of both tables into registers
         constexpr int NE = (M + 1) * W, IT = (NE + NT - 1) / NT, NB = (IT + 23) / 24, CH = (IT + NB - 1) / NB;
