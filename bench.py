@@ -313,6 +313,63 @@ def laser_recipe_leg(nsteps=200):
             "cell_steps_per_s": ncell * st.steps / el, "wall_s": el, "t_end_s": t_end,
             "peak_Te_K": float(te.max()), "peak_Tp_K": float(tp.max())}
 
+def legacy_leg(N=(16, 512, 512), reps=20):
+    """The FP64 libhotspin entry points an unchanged Go core calls, one at a
+    time on the 512x512x16 mesh (exchange6.cu, long_field.cu, reduce.cu and
+    the FP64 demag convolution of MaxwellPlan.UpdateB): time per call from
+    HIP events on the null stream (the legacy entry points take a stream
+    argument; None is the null stream), and algorithmic GB/s where the bytes
+    are simple (each input read once, each output written once)."""
+    import ctypes as C
+
+    import torch
+    from hotspin_amd._lib import lib
+    from hotspin_amd.api import Demag
+
+    L = lib()
+    n = N[0] * N[1] * N[2]
+    g = torch.Generator(device="cuda").manual_seed(7)
+    m = torch.randn((3, n), dtype=torch.float64, device="cuda", generator=g)
+    m /= m.norm(dim=0)
+    m *= 0.8
+    h = torch.empty_like(m)
+    Ts = torch.full((n,), 300.0, dtype=torch.float64, device="cuda")
+    nb = 1024
+    part = torch.empty(nb, dtype=torch.float64, device="cuda")
+    P = lambda t: C.c_void_p(t.data_ptr())
+    res = {}
+
+    def timed(name, fn, nbytes):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        res[name] = {"us_per_call": ms * 1e3}
+        if nbytes:
+            res[name]["algorithmic_bytes"] = nbytes
+            res[name]["GBps"] = nbytes / (ms * 1e-3) / 1e9
+        log("bench: legacy %-22s %8.1f us%s" % (name, ms * 1e3, ("  %5.0f GB/s" % res[name]["GBps"]) if nbytes else ""))
+
+    pre = (1.0 / 4e-18,) * 3
+    timed("exchange6Async", lambda: L.exchange6Async(P(h[0]), P(h[1]), P(h[2]), P(m[0]), P(m[1]), P(m[2]), None,
+                                                     None, N[0], N[1], N[2], 0, 0, 0, 1.7e6, *pre, None), 48 * n)
+    timed("long_field_async", lambda: L.long_field_async(P(h[0]), P(h[1]), P(h[2]), P(m[0]), P(m[1]), P(m[2]), None,
+                                                         None, None, None, P(Ts), 1.7e6, 0.5, 1e28, 1044.0, 1.0, n,
+                                                         None), 56 * n)
+    timed("partialMaxAbsAsync", lambda: L.partialMaxAbsAsync(P(m), P(part), nb, 256, 3 * n, None), 24 * n)
+    d = Demag(N, (2e-9,) * 3)
+    timed("hs_demag_field_f64", lambda: d.field_f64(P(m), None, 1.7e6, P(h)), None)
+    del d
+    return {"workload": "FP64 legacy entry points on the 512x512x16 mesh (2 nm cells), %d calls each" % reps,
+            "calls": res}
+
+
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "llbr_512x512x16_rk23_hbm_traffic.md")
 
 
@@ -412,6 +469,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-laser", action="store_true")
+    ap.add_argument("--no-legacy", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (copyUnPad3DAsync with equal extents), "
                          "to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -566,6 +624,8 @@ def main():
     if not args.no_laser:
         out["laser"] = laser_leg()
         out["laser_recipe"] = laser_recipe_leg()
+    if not args.no_legacy:
+        out["legacy_abi"] = legacy_leg()
     print(json.dumps(out), flush=True)
 
 
