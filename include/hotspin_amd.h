@@ -224,6 +224,13 @@ HS_API hs_demag* hs_solver_demag(hs_solver* s);
  * the choice. */
 HS_API int hs_solver_stage_variant(hs_solver* s);
 
+/* 1 when the solver's LLB stage launches take the lean kernel form (no
+ * per-cell masks, no two-temperature coupling, longitudinal field off or
+ * uniform: the masks' loads and the general longitudinal-field path are
+ * compiled out), 0 for the general form; env HS_STAGE_LEAN=0 at creation
+ * forces the general form.  Both forms give the same result. */
+HS_API int hs_solver_stage_lean(hs_solver* s);
+
 /* ------------------------------------------------ two-temperature coupling
  * Laser-heated LLB-r: the LLB temperature is the electron temperature Te of
  * each cell, and Te, Tp follow the two-temperature model of the reference's

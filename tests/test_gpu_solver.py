@@ -147,20 +147,24 @@ def test_llb_rk23_adaptive_matches_oracle(masks):
 
 @pytest.mark.parametrize("variant", ["march", "flat", "pair", "flatz"])
 @pytest.mark.parametrize("pbc", [(0, 0, 0), (1, 1, 1)])
-def test_stage_kernel_variants_match_oracle(variant, pbc, monkeypatch):
+@pytest.mark.parametrize("masks", [True, False])
+def test_stage_kernel_variants_match_oracle(variant, pbc, masks, monkeypatch):
     """Each stage-kernel variant (marching planes, one plane per block, two
     cells per thread, one plane per block in XCD plane order) against the
     oracle: RHS and 20 thermal RK4 steps, on a mesh whose rows are not a
-    multiple of the tile widths."""
+    multiple of the tile widths.  With per-cell masks the general kernel form
+    runs; without them the lean form (masks and the general longitudinal
+    field compiled out)."""
     monkeypatch.setenv("HS_STAGE_KERNEL", variant)
     N, cell = (5, 12, 160), (2e-9, 2e-9, 2e-9)
-    kw, m = llb_problem(N, cell, masks=True, pbc=pbc)
+    kw, m = llb_problem(N, cell, masks=masks, pbc=pbc)
     o = po.LLB(N, cell, pbc, **kw)
     mo, ko = o.rhs(m, step=3, attempt=0, dt=1e-15)
     Ho = o.heff()
     g = Solver(N, cell, pbc, **kw)
     from hotspin_amd._lib import lib
     assert lib().hs_solver_stage_variant(g._h) == {"march": 0, "flat": 1, "pair": 2, "flatz": 3}[variant]
+    assert lib().hs_solver_stage_lean(g._h) == (0 if masks else 1)
     g.set_m(m)
     kg, Hg = g.rhs(step=3, attempt=0, dt=1e-15)
     assert relerr(Hg, Ho) <= 2e-5
@@ -190,3 +194,32 @@ def test_stage_kernel_variants_are_bitwise_identical(pbc, monkeypatch):
     for variant, mv in got.items():
         assert np.isfinite(mv).all()
         assert np.array_equal(mv, got["flat"]), (variant, np.abs(mv - got["flat"]).max())
+
+
+@pytest.mark.parametrize("variant", ["march", "flat", "pair", "flatz"])
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (0, 1, 1)])
+def test_lean_stage_form_equals_general_form(variant, pbc, monkeypatch):
+    """The lean stage kernels against the general form (HS_STAGE_LEAN=0) on
+    a mask-free thermal LLB-r problem: the RHS and every stepping role
+    (Euler, RK12, RK4, RK23) give the same bits."""
+    from hotspin_amd._lib import lib
+    monkeypatch.setenv("HS_STAGE_KERNEL", variant)
+    N, cell = (5, 12, 160), (2e-9, 2e-9, 2e-9)
+    kw, m = llb_problem(N, cell, masks=False, pbc=pbc)
+    got = {}
+    for lean in ("1", "0"):
+        monkeypatch.setenv("HS_STAGE_LEAN", lean)
+        g = Solver(N, cell, pbc, **kw)
+        assert lib().hs_solver_stage_lean(g._h) == int(lean)
+        out = []
+        g.set_m(m)
+        out += list(g.rhs(step=3, attempt=0, dt=1e-15))
+        for meth, n in (("euler", 3), ("rk12", 3), ("rk4", 4), ("rk23", 8)):
+            g.set_m(m)
+            g.run(meth, n, 1e-15, maxErr=1e-5, mindt=1e-18, maxdt=1e-12, step0=0)
+            out.append(g.get_m())
+        got[lean] = out
+        g.close()
+    for a, b in zip(got["1"], got["0"]):
+        assert np.isfinite(a).all()
+        assert np.array_equal(a, b), np.abs(a - b).max()
