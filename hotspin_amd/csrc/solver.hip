@@ -753,13 +753,42 @@ __device__ __forceinline__ void finish_cell(const Phys& P, const SolveP* sp, con
     store_out<ROLE>(P, io, I, N, o, q);
 }
 
-// Error max of the block: wave shuffles, then the block's waves through LDS,
-// then one atomic max (ordered double bits) into lane blockIdx % kErrLanes.
+// max of a non-negative double with the value of another lane, picked by a
+// DPP control (both 32-bit halves moved, then one FP64 max)
+template <int CTRL>
+__device__ __forceinline__ double dpp_max(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return fmax(v, __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+}
+// Max over the 64 lanes of a wave (every lane gets it): within each row of
+// 16 lanes by DPP (lane swaps in pairs and quads, then the mirrored half row
+// and row), then the four row maxima read into scalar registers.
+__device__ __forceinline__ double wave_max(double v) {
+    v = dpp_max<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = dpp_max<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = dpp_max<0x141>(v);  // row_half_mirror: lane i <-> 7-i within 8
+    v = dpp_max<0x140>(v);  // row_mirror: lane i <-> 15-i within 16
+    const long long b = __double_as_longlong(v);
+    double r = 0.0;
+#pragma unroll
+    for (int row = 0; row < 4; row++) {
+        const int lo = __builtin_amdgcn_readlane((int)b, 16 * row);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 16 * row);
+        r = fmax(r, __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+    }
+    return r;
+}
+
+// Error max of the block: a wave max (wave_max), then the block's waves
+// through LDS, then one atomic max (ordered double bits) into lane
+// blockIdx % kErrLanes.  The error values are >= 0 (err_abs).
 template <int ROLE>
 __device__ __forceinline__ void flush_err(double errloc, const StageIO& io) {
     if constexpr (ROLE == R_BS_4 || ROLE == R_RK12_2) {
         __shared__ double wmax[TX * TY / 64];
-        for (int o = 32; o > 0; o >>= 1) errloc = fmax(errloc, __shfl_xor(errloc, o, 64));
+        errloc = wave_max(errloc);
         const int t = threadIdx.y * TX + threadIdx.x;
         if ((t & 63) == 0) wmax[t >> 6] = errloc;
         __syncthreads();

@@ -223,3 +223,32 @@ def test_lean_stage_form_equals_general_form(variant, pbc, monkeypatch):
     for a, b in zip(got["1"], got["0"]):
         assert np.isfinite(a).all()
         assert np.array_equal(a, b), np.abs(a - b).max()
+
+
+@pytest.mark.parametrize("variant", ["march", "flat", "pair"])
+def test_rk23_error_max_sees_every_lane(variant, monkeypatch):
+    """The RK23 error estimate is a max over every cell (wave, block, then
+    device reductions).  On a state at the equilibrium magnetisation (no
+    demag, no applied field: zero error in the oracle), one cell shortened
+    and tilted dominates the error; placed at k = 0, 5, 21, 42, 63
+    (different lanes of a wave) and in different rows and planes, the GPU's
+    error estimate of one step equals the oracle's."""
+    from hotspin_amd import workloads as W
+    monkeypatch.setenv("HS_STAGE_KERNEL", variant)
+    N, cell = (2, 8, 64), (2e-9, 2e-9, 2e-9)
+    kw, _ = llb_problem(N, cell, thermal=False)
+    kw.update(demag=0, zeeman=0)
+    me = W.equilibrium_m(kw["TMul"], kw["TcMul"], kw["JMul"])
+    g = Solver(N, cell, **kw)
+    o = po.LLB(N, cell, **kw)
+    for (i, j, k) in ((0, 0, 0), (1, 3, 5), (0, 6, 21), (1, 7, 42), (0, 2, 63)):
+        m = np.zeros((3, N[0] * N[1] * N[2]))
+        m[2] = me
+        c = (i * N[1] + j) * N[2] + k
+        m[:, c] = (0.9 * math.sin(1.2), 0.0, 0.9 * math.cos(1.2))
+        mo = m.copy()
+        _, sto = o.step("rk23", mo, 1e-15, 0, maxErr=1.0, mindt=1e-18, maxdt=1e-15)
+        g.set_m(m)
+        _, stg = g.run("rk23", 1, 1e-15, maxErr=1.0, mindt=1e-18, maxdt=1e-15, step0=0)
+        assert sto[0] > 1e-6
+        assert abs(stg.last_err / sto[0] - 1) <= 1e-2, ((i, j, k), stg.last_err, sto[0])
