@@ -527,16 +527,38 @@ void tune32(hs_demag* d, hipStream_t st) {
     }
     const int N0 = d->N[0], N1 = d->N[1], N2 = d->N[2], P1 = d->P[1];
     const int H2 = d->H2, H2p = d->H2p;
-    const long nrows = 3L * N0 * N1;
+    // Time the passes on a batch of nt solves of this mesh, about 4 M cells
+    // (one solve of a large mesh; up to 256 of a small one): batches,
+    // ensembles and slot batches of small meshes run the passes over many
+    // solves at once, and a tile that is fastest for one small solve
+    // (latency-bound grid) is often not the fastest for a full device.  The
+    // batch depends on the mesh only, so the choice stays per shape.
+    const int nt = (int)std::min<long>(256, std::max<long>(1, ((1L << 22) + d->ncell - 1) / d->ncell));
+    const long nrows = 3L * N0 * N1 * nt;
     float *m = nullptr, *B = nullptr;
-    hipMalloc(&m, sizeof(float) * 3 * d->ncell);
-    hipMalloc(&B, sizeof(float) * 3 * d->ncell);
-    if (!m || !B) {
+    cpx<float>*X1t = nullptr, *X2t = nullptr;
+    hipMalloc(&m, sizeof(float) * 3 * d->ncell * nt);
+    hipMalloc(&B, sizeof(float) * 3 * d->ncell * nt);
+    if (nt > d->cap) {
+        hipMalloc(&X1t, sizeof(cpx<float>) * 3L * N0 * N1 * H2p * nt);
+        hipMalloc(&X2t, sizeof(cpx<float>) * 3L * N0 * P1 * H2p * nt);
+    }
+    if (!m || !B || (nt > d->cap && (!X1t || !X2t))) {
         hipFree(m);
         hipFree(B);
-        return;
+        hipFree(X1t);
+        hipFree(X2t);
+        return;  // keeps the first variant of each pass
     }
-    hipMemsetAsync(m, 0, sizeof(float) * 3 * d->ncell, st);
+    // the plan's own spectrum buffers are swapped back in after the timing
+    cpx<float>*X1own = d->X1f, *X2own = d->X2f;
+    const int capown = d->cap;
+    if (nt > d->cap) {
+        d->X1f = X1t;
+        d->X2f = X2t;
+        d->cap = nt;
+    }
+    hipMemsetAsync(m, 0, sizeof(float) * 3 * d->ncell * nt, st);
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -571,14 +593,14 @@ void tune32(hs_demag* d, hipStream_t st) {
     for (size_t i = 0; i < vc.cols.size(); i++) {
         auto& v = vc.cols[i];
         const Live all{nullptr, 1};
-        float t = timeit([&] { v.fwd(d->X1f, d->X2f, g1, g2, 3 * N0, N1, P1, H2, d->tw1f, st, all); }) +
-                  timeit([&] { v.inv(d->X2f, d->X1f, g2, g1, 3 * N0, P1, N1, H2, d->tw1f, st, all); });
+        float t = timeit([&] { v.fwd(d->X1f, d->X2f, g1, g2, 3 * N0 * nt, N1, P1, H2, d->tw1f, st, all); }) +
+                  timeit([&] { v.inv(d->X2f, d->X1f, g2, g1, 3 * N0 * nt, P1, N1, H2, d->tw1f, st, all); });
         if (t < best) { best = t; d->col32 = v; pick[1] = (int)i; }
     }
     best = 1e30f;
     for (size_t i = 0; i < vz.z.size(); i++) {
         auto& v = vz.z[i];
-        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], d->KP, 1.0f, d->tw0f, st, 1, Live{nullptr, 1}); });
+        float t = timeit([&] { v.fn(d->X2f, d->kern32, N0, P1, H2, H2p, d->K[1], d->KP, 1.0f, d->tw0f, st, nt, Live{nullptr, 1}); });
         if (t < best) { best = t; d->z32 = v; pick[2] = (int)i; }
     }
     // One sweep of refinement inside the whole five-pass convolution: each
@@ -588,7 +610,7 @@ void tune32(hs_demag* d, hipStream_t st) {
     // replaces the current one only when it is more than 1 % faster (closer
     // than that they are equivalent, and timing noise would decide).
     {
-        auto pipe = [&] { run_field32(d, m, nullptr, 1.0, B, st); };
+        auto pipe = [&] { run_field32(d, m, nullptr, 1.0, B, st, nt); };
         float cur = timeit(pipe, 3);
         auto sweep = [&](auto& slot, auto& list, int which) {
             for (size_t i = 0; i < list.size(); i++) {
@@ -618,6 +640,11 @@ void tune32(hs_demag* d, hipStream_t st) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     hipStreamSynchronize(st);
+    d->X1f = X1own;
+    d->X2f = X2own;
+    d->cap = capown;
+    hipFree(X1t);
+    hipFree(X2t);
     hipFree(m);
     hipFree(B);
     d->tuned = true;

@@ -141,6 +141,20 @@ void add_z2n(Variants& v) {
         v.z.push_back({W, 6 * W, &launch_z2n_32<L, W>});
 }
 
+// the same for L = 2H > 32 with two lanes per pencil (fft2.hpp k_zconv2h)
+template <int L, int W>
+void launch_z2h_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
+                   const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
+    hipLaunchKernelGGL((k_zconv2h<float, float, L, W>), g, dim3(12 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
+                       srcMul, tw, lv);
+}
+template <int L, int W>
+void add_z2h(Variants& v) {
+    if constexpr (L % 2 == 0 && L > 32 && L <= 64 && reg_dft_ok(L / 2) && (long)L * 6 * W * 8 <= kLdsMax)
+        v.z.push_back({W, 12 * W, &launch_z2h_32<L, W>});
+}
+
 // two-level register FFT columns (fft2.hpp): W columns, W * B threads
 template <int L, int W>
 void add_col2(Variants& v) {
@@ -277,5 +291,7 @@ void register_size() {
     add_z2n<L, 16>(v);
     add_z2n<L, 32>(v);
     add_z2s4<L, 32>(v);
+    add_z2h<L, 8>(v);
+    add_z2h<L, 16>(v);
 }
 }  // namespace hsdm
