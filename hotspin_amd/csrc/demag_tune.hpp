@@ -141,7 +141,7 @@ void add_z2n(Variants& v) {
         v.z.push_back({W, 6 * W, &launch_z2n_32<L, W>});
 }
 
-// the same for L = 2H > 32 with two lanes per pencil (fft2.hpp k_zconv2h)
+// the same with two lanes per pencil, L = 2H from 16 to 64 (fft2.hpp k_zconv2h)
 template <int L, int W>
 void launch_z2h_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
                    const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
@@ -151,7 +151,8 @@ void launch_z2h_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int
 }
 template <int L, int W>
 void add_z2h(Variants& v) {
-    if constexpr (L % 2 == 0 && L > 32 && L <= 64 && reg_dft_ok(L / 2) && (long)L * 6 * W * 8 <= kLdsMax)
+    if constexpr (L % 2 == 0 && L >= 16 && L <= 64 && reg_dft_ok(L / 2) && (long)L * 6 * W * 8 <= kLdsMax &&
+                  12 * W <= 1024)
         v.z.push_back({W, 12 * W, &launch_z2h_32<L, W>});
 }
 
@@ -293,5 +294,6 @@ void register_size() {
     add_z2s4<L, 32>(v);
     add_z2h<L, 8>(v);
     add_z2h<L, 16>(v);
+    add_z2h<L, 32>(v);
 }
 }  // namespace hsdm

@@ -1015,8 +1015,9 @@ __global__ void __launch_bounds__(6 * W) k_zconv2n(cpx<T>* X, const TK* __restri
 }
 
 
-// Axis-0 pass for an even L = 2H with H <= 32 and two lanes per pencil (the
-// L-point pencil in one lane would hold 2L floats in registers).  Lane r
+// Axis-0 pass for an even L = 2H with H <= 32 and two lanes per pencil (for
+// L > 32 the L-point pencil in one lane would hold 2L floats in registers;
+// for smaller L the block is 12 W lanes, whole waves for W = 16 and 32).  Lane r
 // (r = 0, 1) loads the rows t = 2m + r of its pencil and does the H-point DFT
 // of them in registers, E_r.  The product phase forms the spectrum
 //     X[k] = E_0[k] + w^k E_1[k],   X[k+H] = E_0[k] - w^k E_1[k],  w = w_L,
