@@ -36,7 +36,7 @@ class LLBParams(C.Structure):
         ("thermal", C.c_int), ("seed", C.c_uint64), ("stream", C.c_uint32), ("cutoff_dt", C.c_double),
         ("gammaLL", C.c_double), ("torque", C.c_int), ("local00nc", C.c_int),
         ("alphaMul", C.c_double * 3), ("alpha", C.c_void_p * 3),
-        ("normalize", C.c_int),
+        ("normalize", C.c_int), ("demag_kernel", C.c_int),
     ]
 
 
@@ -401,14 +401,21 @@ class Ensemble:
 
 
 class Demag:
-    """Demag plan: Kernel_Arne on the device (FP64) + pruned FFT convolution."""
+    """Demag plan: Kernel_Arne (kernel="arne", the reference's tensor) or the
+    Newell tensor (kernel="newell") on the device in FP64 + pruned FFT
+    convolution."""
 
-    def __init__(self, N, cell, pbc=(0, 0, 0), accuracy=6, stream=None):
+    def __init__(self, N, cell, pbc=(0, 0, 0), accuracy=6, stream=None, kernel="arne"):
         self.N = tuple(int(x) for x in N)
         n = (C.c_int * 3)(*self.N)
         c = (C.c_double * 3)(*[float(x) for x in cell])
         p = (C.c_int * 3)(*[int(x) for x in pbc])
-        self._h = lib().hs_demag_create(n, c, p, int(accuracy), stream)
+        if kernel == "newell":
+            self._h = lib().hs_demag_create_newell(n, c, p, stream)
+        elif kernel == "arne":
+            self._h = lib().hs_demag_create(n, c, p, int(accuracy), stream)
+        else:
+            raise ValueError("kernel must be 'arne' or 'newell'")
         if not self._h:
             check("hs_demag_create")
             raise HotspinError("hs_demag_create failed")

@@ -2359,8 +2359,10 @@ extern "C" hs_solver* hs_batch_create(const hs_llb_params* p, int nsolve, const 
         P.alpha[c] = upload_mask(s, p->alpha[c]);
     }
     if (p->demag) {
-        s->demag = hs_demag_create(p->N, p->cell, p->pbc, p->demag_accuracy > 0 ? p->demag_accuracy : 6,
-                                   (CUstream)s->st);
+        s->demag = p->demag_kernel == 1
+                       ? hs_demag_create_newell(p->N, p->cell, p->pbc, (CUstream)s->st)
+                       : hs_demag_create(p->N, p->cell, p->pbc, p->demag_accuracy > 0 ? p->demag_accuracy : 6,
+                                         (CUstream)s->st);
         if (!s->demag || hs_demag_reserve(s->demag, nsolve) != 0) {
             hs_solver_destroy(s);
             return nullptr;

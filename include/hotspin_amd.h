@@ -53,6 +53,16 @@ typedef struct hs_demag hs_demag;
  * (gpu/fftplanX.go:55-60).  Returns NULL on error (hs_last_error). */
 HS_API hs_demag* hs_demag_create(const int N[3], const double cell[3], const int pbc[3], int accuracy,
                                  CUstream stream);
+/* The same plan with the Newell demag tensor instead of Kernel_Arne: no
+ * counterpart in the reference (which integrates Kernel_Arne only); for
+ * callers that want the closed-form tensor of A. J. Newell, W. Williams and
+ * D. J. Dunlop, J. Geophys. Res. 98 (1993) 9551.  Second differences of the
+ * f and g functions in FP64 up to |R| = 35 hmax^0.6 hmin^0.4, the dipole
+ * tensor averaged over both cells to O(h^4/R^4) beyond (a few 1e-6 relative
+ * to the exact tensor at the switch; Kernel_Arne at accuracy 6 differs from
+ * it by up to 1e-3 of the kernel's largest value).  Same image sums, layout
+ * and passes as hs_demag_create. */
+HS_API hs_demag* hs_demag_create_newell(const int N[3], const double cell[3], const int pbc[3], CUstream stream);
 HS_API void hs_demag_destroy(hs_demag* d);
 /* P = logic (padded) size, K = kernel octant size (P0/2+1, P1/2+1, P2/2+1) */
 HS_API int hs_demag_sizes(const hs_demag* d, int P[3], int K[3]);
@@ -141,6 +151,7 @@ typedef struct {
     double alphaMul[3];
     const double* alpha[3];
     int normalize;
+    int demag_kernel; /* 0 = Kernel_Arne (demag_accuracy), 1 = Newell (hs_demag_create_newell) */
 } hs_llb_params;
 
 #define HS_EULER 0

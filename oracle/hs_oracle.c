@@ -1081,6 +1081,128 @@ void or_kernel_arne(const int size[3], const double cs[3], const int pbc[3], int
 #undef KI
 }
 
+/* Newell demag tensor (no counterpart in the reference, which integrates
+ * Kernel_Arne): A. J. Newell, W. Williams, D. J. Dunlop, J. Geophys. Res. 98
+ * (1993) 9551.  N_xx of two uniformly magnetised cuboids at displacement R is
+ * the 27-point second difference of f over the cell sizes, N_xy that of g,
+ * divided by 4 pi V; the other components permute the axes.  Beyond
+ * R_far = 35 hmax^0.6 hmin^0.4 the differences lose digits in FP64 and the
+ * cell-averaged dipole tensor is used instead:
+ * N ~ -V/(4 pi) [T(R)/2 + sum_a (T(R + h_a e_a) + T(R - h_a e_a))/12],
+ * T = (3 R R^T - R^2 I)/R^5, i.e. T averaged over both cells to O(h^4/R^4).
+ * The kernel value is -N (the sign of Kernel_Arne's field kernel).  Image
+ * ranges and the wrapped layout are those of or_kernel_arne. */
+static double nw_f(double x, double y, double z) {
+    x = fabs(x);
+    y = fabs(y);
+    z = fabs(z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z, R = sqrt(x2 + y2 + z2);
+    double r = (2 * x2 - y2 - z2) * R / 6;
+    if (y > 0 && x2 + z2 > 0) r += 0.5 * y * (z2 - x2) * asinh(y / sqrt(x2 + z2));
+    if (z > 0 && x2 + y2 > 0) r += 0.5 * z * (y2 - x2) * asinh(z / sqrt(x2 + y2));
+    if (x > 0 && y > 0 && z > 0) r -= x * y * z * atan(y * z / (x * R));
+    return r;
+}
+static double nw_g(double x, double y, double z) {
+    double sg = 1;
+    if (x < 0) { sg = -sg; x = -x; }
+    if (y < 0) { sg = -sg; y = -y; }
+    z = fabs(z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z, R = sqrt(x2 + y2 + z2);
+    double r = -x * y * R / 3;
+    if (z > 0 && x2 + y2 > 0) r += x * y * z * asinh(z / sqrt(x2 + y2));
+    if (x > 0 && y2 + z2 > 0) r += y / 6 * (3 * z2 - y2) * asinh(x / sqrt(y2 + z2));
+    if (y > 0 && x2 + z2 > 0) r += x / 6 * (3 * z2 - x2) * asinh(y / sqrt(x2 + z2));
+    if (z > 0) r -= z2 * z / 6 * atan(x * y / (z * R));
+    if (y > 0) r -= z * y2 / 2 * atan(x * z / (y * R));
+    if (x > 0) r -= z * x2 / 2 * atan(y * z / (x * R));
+    return sg * r;
+}
+/* sum over i,j,k in {-1,0,1} of w_i w_j w_k F(X + i a, Y + j b, Z + k c), w_0 = 2, w_1 = -1 */
+static double nw_stencil(double (*F)(double, double, double), double X, double Y, double Z, double a, double b,
+                         double c) {
+    double s = 0;
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) {
+                const double w = (i ? -1.0 : 2.0) * (j ? -1.0 : 2.0) * (k ? -1.0 : 2.0);
+                s += w * F(X + i * a, Y + j * b, Z + k * c);
+            }
+    return s;
+}
+static void nw_dipole(const double R[3], double t[6]) {
+    const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2], r5 = r2 * r2 * sqrt(r2);
+    t[0] = (3 * R[0] * R[0] - r2) / r5;
+    t[1] = (3 * R[1] * R[1] - r2) / r5;
+    t[2] = (3 * R[2] * R[2] - r2) / r5;
+    t[3] = 3 * R[1] * R[2] / r5;
+    t[4] = 3 * R[0] * R[2] / r5;
+    t[5] = 3 * R[0] * R[1] / r5;
+}
+/* N (XX,YY,ZZ,YZ,XZ,XY) of displacement R between cells of size h */
+void or_newell_tensor(const double R[3], const double h[3], double N[6]) {
+    const double V = h[0] * h[1] * h[2];
+    double hmax = h[0], hmin = h[0];
+    for (int a = 1; a < 3; a++) {
+        if (h[a] > hmax) hmax = h[a];
+        if (h[a] < hmin) hmin = h[a];
+    }
+    const double rfar = 35.0 * pow(hmax, 0.6) * pow(hmin, 0.4);
+    const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    if (r >= rfar) {
+        double t[6], acc[6];
+        nw_dipole(R, t);
+        for (int c = 0; c < 6; c++) acc[c] = 0.5 * t[c];
+        for (int a = 0; a < 3; a++)
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                double Q[3] = {R[0], R[1], R[2]};
+                Q[a] += sgn * h[a];
+                nw_dipole(Q, t);
+                for (int c = 0; c < 6; c++) acc[c] += t[c] / 12;
+            }
+        for (int c = 0; c < 6; c++) N[c] = -V / (4 * OR_PI) * acc[c];
+        return;
+    }
+    const double k = 1.0 / (4 * OR_PI * V);
+    const double X = R[0], Y = R[1], Z = R[2], a = h[0], b = h[1], c = h[2];
+    N[0] = k * nw_stencil(nw_f, X, Y, Z, a, b, c);
+    N[1] = k * nw_stencil(nw_f, Y, X, Z, b, a, c);
+    N[2] = k * nw_stencil(nw_f, Z, Y, X, c, b, a);
+    N[3] = k * nw_stencil(nw_g, Y, Z, X, b, c, a);
+    N[4] = k * nw_stencil(nw_g, X, Z, Y, a, c, b);
+    N[5] = k * nw_stencil(nw_g, X, Y, Z, a, b, c);
+}
+/* the full padded kernel, every element summed over its images directly
+ * (no octant symmetry used) */
+void or_kernel_newell(const int size[3], const double cs[3], const int pbc[3], double* out) {
+    const long P = (long)size[0] * size[1] * size[2];
+    int r1[3], r2[3];
+    for (int c = 0; c < 3; c++) {
+        if (pbc[c] == 0) { r1[c] = -(size[c] - 1) / 2; r2[c] = (size[c] - 1) / 2; }
+        else { r1[c] = -(size[c] * pbc[c] - 1); r2[c] = (size[c] * pbc[c] - 1); }
+    }
+    if (size[2] == 1 && pbc[2] == 0) r2[2] = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long g = 0; g < P; g++) {
+        const int zw = (int)(g % size[2]), yw = (int)((g / size[2]) % size[1]), xw = (int)(g / ((long)size[1] * size[2]));
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int z = r1[2]; z <= r2[2]; z++) {
+            if (wrapi(z, size[2]) != zw) continue;
+            for (int y = r1[1]; y <= r2[1]; y++) {
+                if (wrapi(y, size[1]) != yw) continue;
+                for (int x = r1[0]; x <= r2[0]; x++) {
+                    if (wrapi(x, size[0]) != xw) continue;
+                    const double R[3] = {x * cs[0], y * cs[1], z * cs[2]};
+                    double N[6];
+                    or_newell_tensor(R, cs, N);
+                    for (int c = 0; c < 6; c++) acc[c] -= N[c];
+                }
+            }
+        }
+        for (int c = 0; c < 6; c++) out[c * P + g] = acc[c];
+    }
+}
+
 /* ------------------------------------------------------------------ demag plan */
 struct or_demag {
     int N[3], P[3], K[3];
@@ -1095,6 +1217,10 @@ struct or_demag {
  * src/hotspin-core/modules/maxwell_convolution.go:42-100,135-223,350-388;
  * PadSize, src/hotspin-core/engine/zeropad.go:12-22. */
 or_demag* or_demag_new(const int N[3], const double cell[3], const int pbc[3], int accuracy) {
+    return or_demag_new_kernel(N, cell, pbc, accuracy, 0);
+}
+/* kernel: 0 = Kernel_Arne (accuracy), 1 = Newell */
+or_demag* or_demag_new_kernel(const int N[3], const double cell[3], const int pbc[3], int accuracy, int kernel) {
     or_demag* d = (or_demag*)calloc(1, sizeof(or_demag));
     for (int c = 0; c < 3; c++) {
         d->N[c] = N[c];
@@ -1113,7 +1239,8 @@ or_demag* or_demag_new(const int N[3], const double cell[3], const int pbc[3], i
     d->kern = (double*)calloc(6 * d->Kcell, sizeof(double));
 
     double* real = (double*)malloc(sizeof(double) * 6 * d->Pcell);
-    or_kernel_arne(d->P, cell, pbc, accuracy, real);
+    if (kernel == 1) or_kernel_newell(d->P, cell, pbc, real);
+    else or_kernel_arne(d->P, cell, pbc, accuracy, real);
     /* IsZero (maxwell_convolution.go:227-232) uses the max over all components */
     double gmax = 0;
     for (long i = 0; i < 6 * d->Pcell; i++) if (fabs(real[i]) > gmax) gmax = fabs(real[i]);
@@ -1229,7 +1356,9 @@ or_llb* or_llb_new(const or_llb_params* p) {
     s->p = *p;
     s->N = (long)p->N[0] * p->N[1] * p->N[2];
     long n3 = 3 * s->N;
-    if (p->demag) s->demag = or_demag_new(p->N, p->cell, p->pbc, p->demag_accuracy > 0 ? p->demag_accuracy : 6);
+    if (p->demag)
+        s->demag = or_demag_new_kernel(p->N, p->cell, p->pbc, p->demag_accuracy > 0 ? p->demag_accuracy : 6,
+                                       p->demag_kernel);
     double** bufs[] = {&s->H, &s->Hex, &s->B, &s->Hlf, &s->Hth, &s->tau, &s->loc,
                        &s->k1, &s->k2, &s->k3, &s->k4, &s->y0, &s->ys};
     for (unsigned i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++) *bufs[i] = (double*)calloc(n3, sizeof(double));

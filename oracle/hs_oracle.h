@@ -135,7 +135,14 @@ int or_fft1d(double* re_im, int n, int sign);
 void or_kernel_arne(const int size[3], const double cell[3], const int pbc[3],
                     int accuracy, double* out);
 typedef struct or_demag or_demag;
+/* Newell tensor (Newell, Williams & Dunlop 1993; no counterpart in the
+ * reference): N of displacement R between cells h, and the full padded
+ * kernel -N with or_kernel_arne's image ranges and layout. */
+void or_newell_tensor(const double R[3], const double h[3], double N[6]);
+void or_kernel_newell(const int size[3], const double cell[3], const int pbc[3], double* out);
 or_demag* or_demag_new(const int N[3], const double cell[3], const int pbc[3], int accuracy);
+/* kernel 0 = Kernel_Arne, 1 = Newell */
+or_demag* or_demag_new_kernel(const int N[3], const double cell[3], const int pbc[3], int accuracy, int kernel);
 void or_demag_free(or_demag* d);
 void or_demag_sizes(const or_demag* d, int P[3], int K[3]);
 /* 6 octant planes (XX,YY,ZZ,YZ,XZ,XY), each K0*K1*K2, scaled 1/(P0P1P2) */
@@ -185,6 +192,8 @@ typedef struct {
     const double* alpha[3];
     /* normalizer (LLG limit) */
     int normalize;
+    /* demag tensor: 0 = Kernel_Arne, 1 = Newell */
+    int demag_kernel;
 } or_llb_params;
 
 typedef struct or_llb or_llb;

@@ -64,6 +64,10 @@ def _declare(L):
     L.or_kernel_arne.argtypes = [v, v, v, i, v]
     L.or_demag_new.argtypes = [v, v, v, i]
     L.or_demag_new.restype = v
+    L.or_demag_new_kernel.argtypes = [v, v, v, i, i]
+    L.or_demag_new_kernel.restype = v
+    L.or_kernel_newell.argtypes = [v, v, v, v]
+    L.or_newell_tensor.argtypes = [v, v, v]
     L.or_demag_free.argtypes = [v]
     L.or_demag_sizes.argtypes = [v, v, v]
     L.or_demag_kernel.argtypes = [v]
@@ -140,15 +144,36 @@ def kernel_arne(size, cell, pbc, accuracy=6):
     return out
 
 
-class Demag:
-    """FP64 restatement of MaxwellPlan (demag only)."""
+def kernel_newell(size, cell, pbc):
+    """Full padded Newell kernel -N (6 planes), Kernel_Arne's layout."""
+    size = np.asarray(size, dtype=np.int32)
+    out = np.zeros((6,) + tuple(int(s) for s in size), dtype=np.float64)
+    cs = np.asarray(cell, dtype=np.float64)
+    pb = np.asarray(pbc, dtype=np.int32)
+    lib().or_kernel_newell(size.ctypes.data, cs.ctypes.data, pb.ctypes.data, out.ctypes.data)
+    return out
 
-    def __init__(self, N, cell, pbc=(0, 0, 0), accuracy=6):
+
+def newell_tensor(R, h):
+    """Newell N (XX,YY,ZZ,YZ,XZ,XY) of displacement R between cells h."""
+    r = np.asarray(R, dtype=np.float64)
+    hh = np.asarray(h, dtype=np.float64)
+    out = np.zeros(6)
+    lib().or_newell_tensor(r.ctypes.data, hh.ctypes.data, out.ctypes.data)
+    return out
+
+
+class Demag:
+    """FP64 restatement of MaxwellPlan (demag only); kernel "arne" (the
+    reference's Kernel_Arne) or "newell"."""
+
+    def __init__(self, N, cell, pbc=(0, 0, 0), accuracy=6, kernel="arne"):
         self.N = tuple(int(n) for n in N)
         n = np.asarray(N, dtype=np.int32)
         cs = np.asarray(cell, dtype=np.float64)
         pb = np.asarray(pbc, dtype=np.int32)
-        self._h = lib().or_demag_new(n.ctypes.data, cs.ctypes.data, pb.ctypes.data, int(accuracy))
+        kind = {"arne": 0, "newell": 1}[kernel]
+        self._h = lib().or_demag_new_kernel(n.ctypes.data, cs.ctypes.data, pb.ctypes.data, int(accuracy), kind)
         P = np.zeros(3, np.int32)
         K = np.zeros(3, np.int32)
         lib().or_demag_sizes(self._h, P.ctypes.data, K.ctypes.data)
@@ -203,7 +228,7 @@ class LLBParams(C.Structure):
         ("thermal", C.c_int), ("seed", C.c_uint64), ("stream", C.c_uint32), ("cutoff_dt", C.c_double),
         ("gammaLL", C.c_double), ("torque", C.c_int), ("local00nc", C.c_int),
         ("alphaMul", C.c_double * 3), ("alpha", C.c_void_p * 3),
-        ("normalize", C.c_int),
+        ("normalize", C.c_int), ("demag_kernel", C.c_int),
     ]
 
 

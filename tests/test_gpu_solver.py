@@ -252,3 +252,26 @@ def test_rk23_error_max_sees_every_lane(variant, monkeypatch):
         _, stg = g.run("rk23", 1, 1e-15, maxErr=1.0, mindt=1e-18, maxdt=1e-15, step0=0)
         assert sto[0] > 1e-6
         assert abs(stg.last_err / sto[0] - 1) <= 1e-2, ((i, j, k), stg.last_err, sto[0])
+
+
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (0, 2, 0)])
+def test_rhs_and_rk4_with_newell_demag_match_oracle(pbc):
+    """The solver with the Newell demag tensor (demag_kernel = 1) against the
+    oracle with the same tensor: H_eff and k, then 10 thermal RK4 steps."""
+    N, cell = (4, 16, 24), (2e-9, 3e-9, 2.5e-9)
+    kw, m = llb_problem(N, cell, pbc=pbc)
+    kw["demag_kernel"] = 1
+    o = po.LLB(N, cell, pbc, **kw)
+    mo, ko = o.rhs(m, step=3, attempt=0, dt=1e-15)
+    Ho = o.heff()
+    g = Solver(N, cell, pbc, **kw)
+    g.set_m(m)
+    kg, Hg = g.rhs(step=3, attempt=0, dt=1e-15)
+    assert relerr(Hg, Ho) <= 2e-5
+    assert relerr(kg, ko) <= 2e-5
+    mo = m.copy()
+    for i in range(10):
+        o.step("rk4", mo, 2e-15, i)
+    g.set_m(m)
+    g.run("rk4", 10, 2e-15, step0=0)
+    assert np.abs(g.get_m() - mo).max() <= 1e-5
