@@ -363,11 +363,30 @@ def legacy_leg(N=(16, 512, 512), reps=20):
                                                          None, None, None, P(Ts), 1.7e6, 0.5, 1e28, 1044.0, 1.0, n,
                                                          None), 56 * n)
     timed("partialMaxAbsAsync", lambda: L.partialMaxAbsAsync(P(m), P(part), nb, 256, 3 * n, None), 24 * n)
+    t0 = time.time()
     d = Demag(N, (2e-9,) * 3)
+    torch.cuda.synchronize()
+    t_arne = time.time() - t0
     timed("hs_demag_field_f64", lambda: d.field_f64(P(m), None, 1.7e6, P(h)), None)
-    del d
+    # the optional Newell tensor beside Kernel_Arne: plan set-up and the FP64
+    # field of the same magnetisation
+    t0 = time.time()
+    dn = Demag(N, (2e-9,) * 3, kernel="newell")
+    torch.cuda.synchronize()
+    t_newell = time.time() - t0
+    hn = torch.empty_like(h)
+    d.field_f64(P(m), None, 1.7e6, P(h))
+    dn.field_f64(P(m), None, 1.7e6, P(hn))
+    torch.cuda.synchronize()
+    diff = ((hn - h).abs().max() / h.abs().max()).item()
+    log("bench: Newell plan set-up %.2f s (Kernel_Arne %.2f s), FP64 field max|dB|/max|B| vs Kernel_Arne %.1e"
+        % (t_newell, t_arne, diff))
+    del d, dn
     return {"workload": "FP64 legacy entry points on the 512x512x16 mesh (2 nm cells), %d calls each" % reps,
-            "calls": res}
+            "calls": res,
+            "newell": {"setup_s": t_newell, "arne_setup_s": t_arne, "field_max_rel_diff_vs_arne": diff,
+                       "note": "plan set-up incl. kernel, FP64 FFT and tile tuning; field of a random m, "
+                               "difference = Kernel_Arne's integration error at accuracy 6"}}
 
 
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "llbr_512x512x16_rk23_hbm_traffic.md")
