@@ -71,3 +71,20 @@ def test_empty_calls_accept_null():
     L.addAsync(None, None, None, None, 0)
     L.llbar_torque_async(*([None] * 11), 0)
     assert L.hs_last_error() == 0
+
+
+def test_demag_plan_constructors_refuse_a_bad_mesh():
+    """hs_demag_create and hs_demag_create_newell return NULL with HS_EINVAL
+    for a missing or empty mesh before any device call."""
+    import ctypes as C
+    L = _lib.lib()
+    cell = (C.c_double * 3)(2e-9, 2e-9, 2e-9)
+    pbc = (C.c_int * 3)(0, 0, 0)
+    for bad in (None, (C.c_int * 3)(4, 0, 8)):
+        L.hs_clear_error()
+        assert not L.hs_demag_create(bad, cell, pbc, 6, None)
+        assert L.hs_last_error() == HS_EINVAL
+        L.hs_clear_error()
+        assert not L.hs_demag_create_newell(bad, cell, pbc, None)
+        assert L.hs_last_error() == HS_EINVAL
+    L.hs_clear_error()
