@@ -156,6 +156,21 @@ void add_z2h(Variants& v) {
         v.z.push_back({W, 12 * W, &launch_z2h_32<L, W>});
 }
 
+// ... and with four lanes per pencil, L = 4H from 16 to 128 (fft2.hpp k_zconv2q)
+template <int L, int W>
+void launch_z2q_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
+                   const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
+    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
+    hipLaunchKernelGGL((k_zconv2q<float, float, L, W>), g, dim3(24 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
+                       srcMul, tw, lv);
+}
+template <int L, int W>
+void add_z2q(Variants& v) {
+    if constexpr (L % 4 == 0 && L >= 16 && L <= 128 && reg_dft_ok(L / 4) && (long)L * 6 * W * 8 <= kLdsMax &&
+                  24 * W <= 1024)
+        v.z.push_back({W, 24 * W, &launch_z2q_32<L, W>});
+}
+
 // two-level register FFT columns (fft2.hpp): W columns, W * B threads
 template <int L, int W>
 void add_col2(Variants& v) {
@@ -295,5 +310,7 @@ void register_size() {
     add_z2h<L, 8>(v);
     add_z2h<L, 16>(v);
     add_z2h<L, 32>(v);
+    add_z2q<L, 8>(v);
+    add_z2q<L, 16>(v);
 }
 }  // namespace hsdm

@@ -1144,3 +1144,131 @@ __global__ void __launch_bounds__(12 * W) k_zconv2h(cpx<T>* X, const TK* __restr
         }
     }
 }
+
+// The same with four lanes per pencil, L = 4H: lane r (r = 0..3) takes the
+// rows t = 4m + r and does the H-point DFT E_r in registers.  The product
+// phase forms X[k + qH] = sum_r w4^(rq) (w^(rk) E_r[k]) (a 4-point DFT of
+// the twiddled E_r), applies the kernel at the four t = k + qH, and writes
+// back G_r[k] = w^(-rk) sum_q w4^(-rq) Y[k + qH], so that lane r's inverse
+// H-point DFT gives the rows t = 4m + r.  NT = 24 W threads,
+// tid = r * 6W + (s * 3 + c) * W + p; tw holds w_L^t, t < L.
+template <typename T, typename TK, int L, int W>
+__global__ void __launch_bounds__(24 * W) k_zconv2q(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+                                                    int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
+                                                    Live lv) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    constexpr int Q = 4, H = L / Q, WP = 3 * W, W2 = 2 * WP, NT = 24 * W;
+    static_assert(L % Q == 0 && reg_dft_ok(H), "quarter-length register DFT");
+    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
+    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
+    __shared__ cpx<T> lds[L * W2];
+    const int tid = threadIdx.x;
+    const int r = tid / W2, u = tid % W2;
+    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int jp = lb / gridDim.x;
+    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;
+    const int col0 = (lb % gridDim.x) * W;
+    const int p = u % W, c = (u / W) % 3, sm = u / WP;
+    const int j = sm ? jm : jp;
+    const bool live = j >= 0 && col0 + p < H2;
+    const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
+    // row r of the pencil (row 0 when N0 <= r leaves no row r: never read then)
+    cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p + (r < N0 ? r : 0) * tstride;
+    {
+        cpx<T> v[H];
+#pragma unroll
+        for (int m = 0; m < H; m++) {
+            cpx<T> x = {T(0), T(0)};
+            if (live && Q * m + r < N0) {
+                const v2 q = __builtin_nontemporal_load((const v2*)(base + Q * m * tstride));
+                x = {q.x, q.y};
+            }
+            v[m] = x;
+        }
+        dft_reg<T, H, -1>(v);
+#pragma unroll
+        for (int k = 0; k < H; k++) lds[(r * H + k) * W2 + u] = v[k];
+    }
+    __syncthreads();
+    const long kplane = (long)(L / 2 + 1) * K1 * kp;
+    constexpr int NE = H * W, IT = (NE + NT - 1) / NT;
+    T kv[IT][Q][6];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int e = tid + it * NT;
+        const int k = e < NE ? e / W : 0, pp = e % W;
+        const bool ok = e < NE && col0 + pp < H2;
+        const long colk = ok ? col0 + pp : col0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int t = k + q * H;
+            const int ii = t <= L / 2 ? t : L - t;
+            const long K = ((long)ii * K1 + jp) * kp + colk;
+#pragma unroll
+            for (int n = 0; n < 6; n++) kv[it][q][n] = T(kern[n * kplane + K]);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int e = tid + it * NT;
+        const int k = e < NE ? e / W : 0, pp = e % W;
+        const bool ok = e < NE && col0 + pp < H2;
+        if (!ok) continue;
+        cpx<T> w[Q];  // w^(rk), r = 0..3
+        w[0] = {T(1), T(0)};
+#pragma unroll
+        for (int rr = 1; rr < Q; rr++) w[rr] = tw[rr * k];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const T sj = s2 ? T(-1) : T(1);
+            cpx<T> xq[3][Q];
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) {
+#pragma unroll
+                for (int rr = 0; rr < Q; rr++) {
+                    const cpx<T> a = cscale(lds[(rr * H + k) * W2 + s2 * WP + cc * W + pp], srcMul);
+                    xq[cc][rr] = rr ? cmul(w[rr], a) : a;
+                }
+                dft_reg<T, Q, -1>(xq[cc]);  // xq[cc][q] = X[k + qH]
+            }
+            cpx<T> yq[3][Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int t = k + q * H;
+                const T si = t > L / 2 ? T(-1) : T(1);
+                const T kxx = kv[it][q][0], kyy = kv[it][q][1], kzz = kv[it][q][2];
+                const T kyz = sj * kv[it][q][3], kxz = si * kv[it][q][4], kxy = sj * si * kv[it][q][5];
+                const cpx<T> x0 = xq[0][q], x1 = xq[1][q], x2 = xq[2][q];
+                yq[0][q] = {kxx * x0.x + kxy * x1.x + kxz * x2.x, kxx * x0.y + kxy * x1.y + kxz * x2.y};
+                yq[1][q] = {kxy * x0.x + kyy * x1.x + kyz * x2.x, kxy * x0.y + kyy * x1.y + kyz * x2.y};
+                yq[2][q] = {kxz * x0.x + kyz * x1.x + kzz * x2.x, kxz * x0.y + kyz * x1.y + kzz * x2.y};
+            }
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) {
+                dft_reg<T, Q, +1>(yq[cc]);  // yq[cc][r] = sum_q w4^(-rq) Y[k + qH]
+#pragma unroll
+                for (int rr = 0; rr < Q; rr++) {
+                    const cpx<T> wc = {w[rr].x, -w[rr].y};
+                    lds[(rr * H + k) * W2 + s2 * WP + cc * W + pp] = rr ? cmul(wc, yq[cc][rr]) : yq[cc][rr];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    {
+        cpx<T> v[H];
+#pragma unroll
+        for (int k = 0; k < H; k++) v[k] = lds[(r * H + k) * W2 + u];
+        dft_reg<T, H, +1>(v);
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < H; m++)
+                if (Q * m + r < N0) {
+                    v2 q;
+                    q.x = v[m].x;
+                    q.y = v[m].y;
+                    __builtin_nontemporal_store(q, (v2*)(base + Q * m * tstride));
+                }
+        }
+    }
+}
