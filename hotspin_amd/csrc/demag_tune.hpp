@@ -141,34 +141,20 @@ void add_z2n(Variants& v) {
         v.z.push_back({W, 6 * W, &launch_z2n_32<L, W>});
 }
 
-// the same with two lanes per pencil, L = 2H from 16 to 64 (fft2.hpp k_zconv2h)
-template <int L, int W>
-void launch_z2h_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
+// the same with Q = 2 or 4 lanes per pencil, L = Q H with 4 <= H <= 32
+// (fft2.hpp k_zconv2m)
+template <int L, int W, int Q>
+void launch_z2m_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
                    const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
     dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
-    hipLaunchKernelGGL((k_zconv2h<float, float, L, W>), g, dim3(12 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
-                       srcMul, tw, lv);
+    hipLaunchKernelGGL((k_zconv2m<float, float, L, W, Q>), g, dim3(6 * Q * W), 0, st, X, kern, N0, P1, H2, H2p, K1,
+                       kp, srcMul, tw, lv);
 }
-template <int L, int W>
-void add_z2h(Variants& v) {
-    if constexpr (L % 2 == 0 && L >= 16 && L <= 64 && reg_dft_ok(L / 2) && (long)L * 6 * W * 8 <= kLdsMax &&
-                  12 * W <= 1024)
-        v.z.push_back({W, 12 * W, &launch_z2h_32<L, W>});
-}
-
-// ... and with four lanes per pencil, L = 4H from 16 to 128 (fft2.hpp k_zconv2q)
-template <int L, int W>
-void launch_z2q_32(cpx<float>* X, const float* kern, int N0, int P1, int H2, int H2p, int K1, int kp, float srcMul,
-                   const cpx<float>* tw, hipStream_t st, int nsolve, Live lv) {
-    dim3 g((unsigned)((H2 + W - 1) / W), (unsigned)(P1 / 2 + 1), (unsigned)nsolve);
-    hipLaunchKernelGGL((k_zconv2q<float, float, L, W>), g, dim3(24 * W), 0, st, X, kern, N0, P1, H2, H2p, K1, kp,
-                       srcMul, tw, lv);
-}
-template <int L, int W>
-void add_z2q(Variants& v) {
-    if constexpr (L % 4 == 0 && L >= 16 && L <= 128 && reg_dft_ok(L / 4) && (long)L * 6 * W * 8 <= kLdsMax &&
-                  24 * W <= 1024)
-        v.z.push_back({W, 24 * W, &launch_z2q_32<L, W>});
+template <int L, int W, int Q>
+void add_z2m(Variants& v) {
+    if constexpr (L % Q == 0 && L / Q >= 4 && L / Q <= 32 && L >= 16 && reg_dft_ok(L / Q) &&
+                  (long)L * 6 * W * 8 <= kLdsMax && 6 * Q * W <= 1024)
+        v.z.push_back({W, 6 * Q * W, &launch_z2m_32<L, W, Q>});
 }
 
 // two-level register FFT columns (fft2.hpp): W columns, W * B threads
@@ -307,10 +293,10 @@ void register_size() {
     add_z2n<L, 16>(v);
     add_z2n<L, 32>(v);
     add_z2s4<L, 32>(v);
-    add_z2h<L, 8>(v);
-    add_z2h<L, 16>(v);
-    add_z2h<L, 32>(v);
-    add_z2q<L, 8>(v);
-    add_z2q<L, 16>(v);
+    add_z2m<L, 8, 2>(v);
+    add_z2m<L, 16, 2>(v);
+    add_z2m<L, 32, 2>(v);
+    add_z2m<L, 8, 4>(v);
+    add_z2m<L, 16, 4>(v);
 }
 }  // namespace hsdm

@@ -1014,151 +1014,25 @@ __global__ void __launch_bounds__(6 * W) k_zconv2n(cpx<T>* X, const TK* __restri
     }
 }
 
-
-// Axis-0 pass for an even L = 2H with H <= 32 and two lanes per pencil (for
+// Axis-0 pass with Q = 2 or 4 lanes per pencil, L = Q H with H <= 32 (for
 // L > 32 the L-point pencil in one lane would hold 2L floats in registers;
-// for smaller L the block is 12 W lanes, whole waves for W = 16 and 32).  Lane r
-// (r = 0, 1) loads the rows t = 2m + r of its pencil and does the H-point DFT
+// with two lanes the block is 12 W lanes, whole waves for W = 16).  Lane r
+// (r < Q) loads the rows t = Q m + r of its pencil and does the H-point DFT
 // of them in registers, E_r.  The product phase forms the spectrum
-//     X[k] = E_0[k] + w^k E_1[k],   X[k+H] = E_0[k] - w^k E_1[k],  w = w_L,
-// applies the 3x3 kernel at t = k and t = k + H (Y = K X), and writes back
-//     G_0[k] = Y[k] + Y[k+H],   G_1[k] = (Y[k] - Y[k+H]) w^-k,
-// so that lane r's inverse H-point DFT of G_r gives the rows t = 2m + r.
-// NT = 12 W threads, tid = r * 6W + (s * 3 + c) * W + p; LDS [t][s][c][p]
+//     X[k + qH] = sum_r w_Q^(rq) (w^(rk) E_r[k]),   w = w_L
+// (a Q-point DFT of the twiddled E_r), applies the 3x3 kernel at the Q
+// points t = k + qH, and writes back
+//     G_r[k] = w^(-rk) sum_q w_Q^(-rq) Y[k + qH],
+// so that lane r's inverse H-point DFT of G_r gives the rows t = Q m + r.
+// NT = 6 Q W threads, tid = r * 6W + (s * 3 + c) * W + p; LDS [t][s][c][p]
 // as in k_zconv2 (lane r posts E_r at t = r H + k).  tw holds w_L^t, t < L.
-template <typename T, typename TK, int L, int W>
-__global__ void __launch_bounds__(12 * W) k_zconv2h(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
-                                                    int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
-                                                    Live lv) {
+template <typename T, typename TK, int L, int W, int Q>
+__global__ void __launch_bounds__(6 * Q * W) k_zconv2m(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
+                                                        int H2p, int K1, int kp, T srcMul,
+                                                        const cpx<T>* __restrict__ tw, Live lv) {
     typedef T v2 __attribute__((ext_vector_type(2)));
-    constexpr int H = L / 2, WP = 3 * W, W2 = 2 * WP, NT = 12 * W;
-    static_assert(L % 2 == 0 && reg_dft_ok(H), "half-length register DFT");
-    if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
-    X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
-    __shared__ cpx<T> lds[L * W2];
-    const int tid = threadIdx.x;
-    const int r = tid / W2, u = tid % W2;
-    const int lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int jp = lb / gridDim.x;
-    const int jm = (jp == 0 || 2 * jp == P1) ? -1 : P1 - jp;
-    const int col0 = (lb % gridDim.x) * W;
-    const int p = u % W, c = (u / W) % 3, sm = u / WP;
-    const int j = sm ? jm : jp;
-    const bool live = j >= 0 && col0 + p < H2;
-    const long cstride = (long)N0 * P1 * H2p, tstride = (long)P1 * H2p;
-    // row r of the pencil (row 0 when N0 = 1 leaves no row r: never read then)
-    cpx<T>* base = X + c * cstride + (long)(j < 0 ? 0 : j) * H2p + col0 + p + (r < N0 ? r : 0) * tstride;
-    {
-        cpx<T> v[H];
-#pragma unroll
-        for (int m = 0; m < H; m++) {
-            cpx<T> x = {T(0), T(0)};
-            if (live && 2 * m + r < N0) {
-                const v2 q = __builtin_nontemporal_load((const v2*)(base + 2 * m * tstride));
-                x = {q.x, q.y};
-            }
-            v[m] = x;
-        }
-        dft_reg<T, H, -1>(v);
-#pragma unroll
-        for (int k = 0; k < H; k++) lds[(r * H + k) * W2 + u] = v[k];
-    }
-    __syncthreads();
-    // product entries e = tid + q NT over (k < H, column p); the kernel values
-    // of t = k and t = k + H are loaded for all of them first (clamped to a
-    // valid column) so that they are in flight together
-    const long kplane = (long)(L / 2 + 1) * K1 * kp;
-    constexpr int NE = H * W, IT = (NE + NT - 1) / NT;
-    T kv[IT][2][6];
-#pragma unroll
-    for (int q = 0; q < IT; q++) {
-        const int e = tid + q * NT;
-        const int k = e < NE ? e / W : 0, pp = e % W;
-        const bool ok = e < NE && col0 + pp < H2;
-        const long colk = ok ? col0 + pp : col0;
-        // octant index of t = k is k, of t = k + H (mirrored when k > 0) H - k
-        const long K0 = ((long)k * K1 + jp) * kp + colk;
-        const long K1i = ((long)(H - k) * K1 + jp) * kp + colk;
-#pragma unroll
-        for (int n = 0; n < 6; n++) {
-            kv[q][0][n] = T(kern[n * kplane + K0]);
-            kv[q][1][n] = T(kern[n * kplane + K1i]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < IT; q++) {
-        const int e = tid + q * NT;
-        const int k = e < NE ? e / W : 0, pp = e % W;
-        const bool ok = e < NE && col0 + pp < H2;
-        if (!ok) continue;
-        const cpx<T> w = tw[k];
-        const cpx<T> wc = {w.x, -w.y};
-        // t = k + H > L/2 (k > 0) is a mirrored index: XZ and XY change sign
-        const T s1 = k == 0 ? T(1) : T(-1);
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) {
-            const T sj = s2 ? T(-1) : T(1);
-            cpx<T>* a = lds + k * W2 + s2 * WP + pp;
-            cpx<T>* b = lds + (k + H) * W2 + s2 * WP + pp;
-            cpx<T> x0[3], x1[3], y0[3], y1[3];
-#pragma unroll
-            for (int cc = 0; cc < 3; cc++) {
-                const cpx<T> A = cscale(a[cc * W], srcMul);
-                const cpx<T> B = cmul(w, cscale(b[cc * W], srcMul));
-                x0[cc] = {A.x + B.x, A.y + B.y};
-                x1[cc] = {A.x - B.x, A.y - B.y};
-            }
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const T si = h ? s1 : T(1);
-                const T kxx = kv[q][h][0], kyy = kv[q][h][1], kzz = kv[q][h][2];
-                const T kyz = sj * kv[q][h][3], kxz = si * kv[q][h][4], kxy = sj * si * kv[q][h][5];
-                const cpx<T>* xx = h ? x1 : x0;
-                cpx<T>* yy = h ? y1 : y0;
-                yy[0] = {kxx * xx[0].x + kxy * xx[1].x + kxz * xx[2].x, kxx * xx[0].y + kxy * xx[1].y + kxz * xx[2].y};
-                yy[1] = {kxy * xx[0].x + kyy * xx[1].x + kyz * xx[2].x, kxy * xx[0].y + kyy * xx[1].y + kyz * xx[2].y};
-                yy[2] = {kxz * xx[0].x + kyz * xx[1].x + kzz * xx[2].x, kxz * xx[0].y + kyz * xx[1].y + kzz * xx[2].y};
-            }
-#pragma unroll
-            for (int cc = 0; cc < 3; cc++) {
-                a[cc * W] = {y0[cc].x + y1[cc].x, y0[cc].y + y1[cc].y};
-                b[cc * W] = cmul(wc, cpx<T>{y0[cc].x - y1[cc].x, y0[cc].y - y1[cc].y});
-            }
-        }
-    }
-    __syncthreads();
-    {
-        cpx<T> v[H];
-#pragma unroll
-        for (int k = 0; k < H; k++) v[k] = lds[(r * H + k) * W2 + u];
-        dft_reg<T, H, +1>(v);
-        if (live) {
-#pragma unroll
-            for (int m = 0; m < H; m++)
-                if (2 * m + r < N0) {
-                    v2 q;
-                    q.x = v[m].x;
-                    q.y = v[m].y;
-                    __builtin_nontemporal_store(q, (v2*)(base + 2 * m * tstride));
-                }
-        }
-    }
-}
-
-// The same with four lanes per pencil, L = 4H: lane r (r = 0..3) takes the
-// rows t = 4m + r and does the H-point DFT E_r in registers.  The product
-// phase forms X[k + qH] = sum_r w4^(rq) (w^(rk) E_r[k]) (a 4-point DFT of
-// the twiddled E_r), applies the kernel at the four t = k + qH, and writes
-// back G_r[k] = w^(-rk) sum_q w4^(-rq) Y[k + qH], so that lane r's inverse
-// H-point DFT gives the rows t = 4m + r.  NT = 24 W threads,
-// tid = r * 6W + (s * 3 + c) * W + p; tw holds w_L^t, t < L.
-template <typename T, typename TK, int L, int W>
-__global__ void __launch_bounds__(24 * W) k_zconv2q(cpx<T>* X, const TK* __restrict__ kern, int N0, int P1, int H2,
-                                                    int H2p, int K1, int kp, T srcMul, const cpx<T>* __restrict__ tw,
-                                                    Live lv) {
-    typedef T v2 __attribute__((ext_vector_type(2)));
-    constexpr int Q = 4, H = L / Q, WP = 3 * W, W2 = 2 * WP, NT = 24 * W;
-    static_assert(L % Q == 0 && reg_dft_ok(H), "quarter-length register DFT");
+    constexpr int H = L / Q, WP = 3 * W, W2 = 2 * WP, NT = 6 * Q * W;
+    static_assert((Q == 2 || Q == 4) && L % Q == 0 && reg_dft_ok(H), "lanes per pencil, sub-DFT length");
     if (all_dead(lv, blockIdx.z, blockIdx.z)) return;
     X += (long)blockIdx.z * 3 * N0 * P1 * H2p;
     __shared__ cpx<T> lds[L * W2];
@@ -1190,6 +1064,9 @@ __global__ void __launch_bounds__(24 * W) k_zconv2q(cpx<T>* X, const TK* __restr
         for (int k = 0; k < H; k++) lds[(r * H + k) * W2 + u] = v[k];
     }
     __syncthreads();
+    // product entries e = tid + it NT over (k < H, column p); the kernel values
+    // of the Q points are loaded for all of them first (clamped to a valid
+    // column) so that they are in flight together
     const long kplane = (long)(L / 2 + 1) * K1 * kp;
     constexpr int NE = H * W, IT = (NE + NT - 1) / NT;
     T kv[IT][Q][6];
@@ -1202,7 +1079,7 @@ __global__ void __launch_bounds__(24 * W) k_zconv2q(cpx<T>* X, const TK* __restr
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const int t = k + q * H;
-            const int ii = t <= L / 2 ? t : L - t;
+            const int ii = t <= L / 2 ? t : L - t;  // octant index (t > L/2 is mirrored)
             const long K = ((long)ii * K1 + jp) * kp + colk;
 #pragma unroll
             for (int n = 0; n < 6; n++) kv[it][q][n] = T(kern[n * kplane + K]);
@@ -1214,7 +1091,7 @@ __global__ void __launch_bounds__(24 * W) k_zconv2q(cpx<T>* X, const TK* __restr
         const int k = e < NE ? e / W : 0, pp = e % W;
         const bool ok = e < NE && col0 + pp < H2;
         if (!ok) continue;
-        cpx<T> w[Q];  // w^(rk), r = 0..3
+        cpx<T> w[Q];  // w^(rk)
         w[0] = {T(1), T(0)};
 #pragma unroll
         for (int rr = 1; rr < Q; rr++) w[rr] = tw[rr * k];
@@ -1235,7 +1112,7 @@ __global__ void __launch_bounds__(24 * W) k_zconv2q(cpx<T>* X, const TK* __restr
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const int t = k + q * H;
-                const T si = t > L / 2 ? T(-1) : T(1);
+                const T si = t > L / 2 ? T(-1) : T(1);  // mirrored t: XZ and XY change sign
                 const T kxx = kv[it][q][0], kyy = kv[it][q][1], kzz = kv[it][q][2];
                 const T kyz = sj * kv[it][q][3], kxz = si * kv[it][q][4], kxy = sj * si * kv[it][q][5];
                 const cpx<T> x0 = xq[0][q], x1 = xq[1][q], x2 = xq[2][q];
@@ -1245,7 +1122,7 @@ __global__ void __launch_bounds__(24 * W) k_zconv2q(cpx<T>* X, const TK* __restr
             }
 #pragma unroll
             for (int cc = 0; cc < 3; cc++) {
-                dft_reg<T, Q, +1>(yq[cc]);  // yq[cc][r] = sum_q w4^(-rq) Y[k + qH]
+                dft_reg<T, Q, +1>(yq[cc]);  // yq[cc][r] = sum_q w_Q^(-rq) Y[k + qH]
 #pragma unroll
                 for (int rr = 0; rr < Q; rr++) {
                     const cpx<T> wc = {w[rr].x, -w[rr].y};
