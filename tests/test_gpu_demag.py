@@ -123,14 +123,16 @@ def test_golden_demag_fp32(pbc, name):
 
 @pytest.mark.parametrize("N,pbc", [((16, 32, 64), (0, 0, 0)), ((4, 8, 512), (0, 0, 0)), ((16, 32, 64), (0, 2, 0)),
                                    ((2, 512, 16), (0, 0, 0)), ((32, 16, 32), (0, 0, 0)), ((64, 8, 16), (2, 0, 0)),
-                                   ((24, 8, 16), (0, 0, 0)), ((20, 8, 16), (0, 0, 0))])
+                                   ((24, 8, 16), (0, 0, 0)), ((20, 8, 16), (0, 0, 0)), ((8, 16, 32), (0, 0, 0)),
+                                   ((64, 4, 16), (0, 0, 0))])
 def test_every_fp32_pass_variant_matches_fp64(monkeypatch, N, pbc):
     """Each registered FP32 tile variant of the row, column and axis-0 passes
     (forced by index) gives the FP64 field to FP32 accuracy (row length 512
     and column length 1024 are the 512x512x16 workload's; a periodic axis 1
     has no zero padding, so the half-length column variants fall back to the
-    full-length pass; axis-0 lengths 64 (from 32 planes, and from 64
-    periodic planes), 48 and 40 also run the two-lanes-per-pencil pass)."""
+    full-length pass; axis-0 lengths 16, 40, 48, 64 (from 32 planes, and
+    from 64 periodic planes) and 128 also run the passes with two or four
+    lanes per pencil)."""
     import ctypes as C
     from hotspin_amd._lib import lib
     cell = (2e-9, 3e-9, 4e-9)
