@@ -33,15 +33,23 @@ def _requests(n):
     return rs
 
 
-def test_serve_results_equal_solves_alone():
+@pytest.mark.parametrize("policy", ["sjf", "fifo"])
+def test_serve_results_equal_solves_alone(policy):
+    """Both admission policies (least work first under a cap on the running
+    cells, or every request at once) complete every request with the result
+    of the request solved alone; sjf keeps the running cells within its cap."""
     reqs = _requests(24)
-    srv = Server(shapes=SHAPES, slots_cells=4096, min_slots=3, max_slots=4, chunk=3)
+    cap = 3000  # cells: two or three of these requests at a time, the rest queue
+    srv = Server(shapes=SHAPES, slots_cells=4096, min_slots=3, max_slots=4, chunk=3, policy=policy,
+                 inflight_cells=cap)
     try:
         res = srv.run(reqs)
         summ = srv.summary()
     finally:
         srv.close()
-    assert summ["requests"] == 24 and summ["failed"] == 0
+    assert summ["requests"] == 24 and summ["failed"] == 0 and summ["policy"] == policy
+    if policy == "sjf":
+        assert 0 < srv.peak_running_cells <= cap
     assert all(res[r["id"]]["steps"] == r["steps"] for r in reqs)
     assert summ["p99_latency_s"] >= summ["p50_latency_s"] > 0
     for r in reqs[:: 5]:
