@@ -162,7 +162,7 @@ def ensemble_leg(steps, warmup, plain_steps=10, check_steps=20):
                             "scaled by the shape counts" % plain_steps}
 
 
-def serving_leg(nreq, plain_steps=20, check_steps=5, policy="sjf"):
+def serving_leg(nreq, plain_steps=20, check_steps=5, policy="sjf", inflight_cells=1 << 23):
     """Continuous batching of a Poisson request stream vs the plain solver.
     The first request of each shape class is also solved alone for
     `check_steps` adaptive RK23 steps and compared with the FP64 oracle (a
@@ -172,7 +172,7 @@ def serving_leg(nreq, plain_steps=20, check_steps=5, policy="sjf"):
     from hotspin_amd.serve import Server
     from oracle import pyoracle as po
     reqs = W.requests(n=nreq, rate=200.0, seed=0)
-    srv = Server(keep_results=False, policy=policy)
+    srv = Server(keep_results=False, policy=policy, inflight_cells=inflight_cells)
     for r in reqs:  # the client sends the initial state with the request
         r["m0"] = srv._initial_state(r)
     srv.run(reqs, log=log)
@@ -486,6 +486,7 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-policy", choices=("fifo", "sjf"), default="sjf")
+    ap.add_argument("--serve-inflight-cells", type=int, default=1 << 23)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-laser", action="store_true")
@@ -640,7 +641,8 @@ def main():
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ens_steps, 5)
     if not args.no_serving:
-        out["serving"] = serving_leg(args.serve_requests, policy=args.serve_policy)
+        out["serving"] = serving_leg(args.serve_requests, policy=args.serve_policy,
+                                     inflight_cells=args.serve_inflight_cells)
     if not args.no_laser:
         out["laser"] = laser_leg()
         out["laser_recipe"] = laser_recipe_leg()

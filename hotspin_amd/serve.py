@@ -51,7 +51,7 @@ class Server:
 
     def __init__(self, shapes=W.SERVE_SHAPES, cell=W.SERVE_CELL, material=W.MATERIAL_WIDE_T,
                  hbm_budget=64 << 30, slots_cells=1 << 21, min_slots=4, max_slots=64, chunk=8, tol=1e-5,
-                 dt0=1e-15, mindt=1e-20, maxdt=1e-12, keep_results=True, policy="sjf", inflight_cells=1 << 24):
+                 dt0=1e-15, mindt=1e-20, maxdt=1e-12, keep_results=True, policy="sjf", inflight_cells=1 << 23):
         self.shapes = list(shapes)
         self.cell = cell
         self.material = material
@@ -65,11 +65,12 @@ class Server:
         if policy not in ("fifo", "sjf"):
             raise ValueError("policy must be 'fifo' or 'sjf'")
         # sjf: the queued request with the least work (cells x steps) first,
-        # while the running requests hold at most inflight_cells cells (about
-        # four times what fills the device); fifo: every arrived request is
+        # while the running requests hold at most inflight_cells cells (8.4 M,
+        # twice what fills the device); fifo: every arrived request is
         # admitted as soon as its class has a slot, and all of them share the
         # device.  On the 2000-request stream of 16^3..256x256x4 solves sjf
-        # cut the p50 latency from 66 s to 21 s at the same throughput.
+        # cut the p50 latency from 66 s to 13 s at the same throughput
+        # (cap 4.2 M: 9 s, 1 % less throughput; 16.8 M: 21 s; 33.6 M: 35 s).
         self.policy = policy
         self.inflight_cells = int(inflight_cells)
         self.running_cells = 0
