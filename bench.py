@@ -13,19 +13,23 @@ Two more legs report batched vs plain per-solve throughput:
   (fixed RK4, one graph launch per step) vs the same solves run one at a
   time through the plain solver (a captured graph per solve; eager beside it);
 - "serving": the 2000-request Poisson stream served by the
-  continuous-batching loop (p50/p99 latency split into queue wait and
-  service time, cell-steps/s) vs the plain solver per request.
+  continuous-batching loop, least work first under a cap on the running
+  cells (p50/p99 latency split into queue wait and service time,
+  cell-steps/s) vs the plain solver per request.
 A "laser" leg runs the laser-heated LLB-r + two-temperature film (5 ps) and
 "laser_recipe" the reference's AM12 recipe on the same film.
 
 Every leg reports its error against the FP64 oracle on the same inputs
 ("error_vs_oracle", "max_abs_dm_vs_oracle", "max_abs_err_vs_oracle"), and
 "cpu_baseline" times the oracle on the host cores (and on one thread).
+The "legacy_abi" leg times the FP64 legacy entry points and reports the
+optional Newell demag plan beside Kernel_Arne.
 
 Prints one JSON line (see the keys below); progress goes to stderr.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-       [--ens-steps K] [--serve-requests R] [--no-ensemble] [--no-serving] [--no-laser]
+       [--ens-steps K] [--serve-requests R] [--serve-policy sjf|fifo]
+       [--serve-inflight-cells C] [--no-ensemble] [--no-serving] [--no-laser] [--no-legacy]
 """
 import argparse
 import json
